@@ -1,0 +1,188 @@
+/*
+ * liblm -- C ABI of the MI355X (gfx950) hot path of lagrangian-microbes:
+ * RK4 advection through a gridded space-time velocity field, the fixed-radius
+ * pair search, and the rock-paper-scissors species update.
+ *
+ * Conventions
+ *  - Every array argument is a caller-owned DEVICE pointer unless stated.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *    Calls are asynchronous on that stream; none of them synchronises with
+ *    the host.
+ *  - Return value: LM_OK (0) or a negative LM_ERR_* code; lm_last_error()
+ *    then holds a thread-local message.  No C++ exception crosses the ABI.
+ *  - Particle state is structure-of-arrays: lon[n], lat[n] in degrees
+ *    (longitude 0..360 E), species[n] as int8 (1 rock, 2 paper, 3 scissors).
+ *  - A velocity field is u,v (m/s) on a uniform lon/lat grid of nx*ny nodes,
+ *    stored node-major with u and v interleaved: f[(j*nx + i)*2 + {0,1}],
+ *    i along longitude, j along latitude.
+ */
+#ifndef LM_H
+#define LM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LM_ABI_VERSION 1
+
+#define LM_OK 0
+#define LM_ERR_ARG (-1)      /* invalid argument                             */
+#define LM_ERR_HIP (-2)      /* HIP runtime error                            */
+#define LM_ERR_NOMEM (-3)    /* device allocation failed                     */
+
+int lm_abi_version(void);
+const char* lm_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Velocity field                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* Uniform rectilinear lon/lat grid of a velocity field (degrees). */
+typedef struct lm_grid {
+    int32_t nx, ny;     /* nodes along longitude, latitude (each >= 2)      */
+    double x0, y0;      /* longitude, latitude of node (0, 0)               */
+    double inv_dx;      /* 1 / longitude spacing                            */
+    double inv_dy;      /* 1 / latitude spacing                             */
+} lm_grid;
+
+/* Linear-in-time field at a list of stage times, node by node:
+ *   out[s][k] = fma(w[s], levels[l+1][k] - levels[l][k], levels[l][k]),
+ *   l = lev0[s], for k < level_len (= 2*nx*ny).
+ * lev0 (int32[nstage]) and w (double[nstage]) are device arrays.
+ * Replaces the time interpolation of parcels.Field(interp_method="linear")
+ * that particle_advecter.py:182-184 sets up over the time axis built at
+ * particle_advecter.py:171-173. */
+int lm_field_blend_f64(const double* levels, int64_t level_len, const int32_t* lev0,
+                       const double* w, int32_t nstage, double* out, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* RK4 advection                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* Advance n particles by nsteps classic RK4 steps of dt seconds.
+ * Step s (0-based) samples the velocity at three stage times: field
+ * stage_fields + (2s)*stage_stride at t, + (2s+1)*stage_stride at t+dt/2 and
+ * + (2s+2)*stage_stride at t+dt (stage_stride = 0 for a steady field).
+ * Interpolation is bilinear in lon/lat; m/s become deg/s through
+ * 1852*60 m per degree and cos(lat) for the zonal component.
+ * After step s the post-step position is stored as float32 into
+ * rec_lon/rec_lat + s*rec_stride (both may be NULL), then, if walk_amp > 0,
+ * each coordinate takes a uniform random-walk increment
+ * (2u-1)*walk_amp with u from Philox4x64-10(counter = (i, step0+s, 1, 0),
+ * key = (key0, key1)), word 0 for lat and word 1 for lon.
+ * A particle whose stage position leaves the grid keeps its position for that
+ * step and increments *oob_count (device uint32).
+ * Replaces pset.execute(AdvectionRK4, runtime=dt, dt=dt)
+ * (particle_advecter.py:222-223), the per-step record loop
+ * (particle_advecter.py:232-235) and the Kh random walk
+ * (particle_advecter.py:240-242). */
+int lm_advect_rk4_f64(int64_t n, double* lon, double* lat,
+                      const double* stage_fields, int64_t stage_stride, const lm_grid* grid,
+                      int32_t nsteps, double dt,
+                      float* rec_lon, float* rec_lat, int64_t rec_stride,
+                      double walk_amp, uint64_t key0, uint64_t key1, uint64_t step0,
+                      uint32_t* oob_count, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Fixed-radius pair search                                                 */
+/* ------------------------------------------------------------------------ */
+
+typedef struct lm_pairs_ws lm_pairs_ws;
+
+/* Workspace for up to n_max particles expected inside the lon/lat box
+ * [lon_min, lon_max] x [lat_min, lat_max] (particles outside are still
+ * handled exactly, only more slowly), for radii up to max_radius (deg).
+ * Allocates device memory on the current device. */
+int lm_pairs_ws_create(int64_t n_max, double lon_min, double lon_max, double lat_min,
+                       double lat_max, double max_radius, lm_pairs_ws** out);
+void lm_pairs_ws_destroy(lm_pairs_ws* ws);
+/* Device bytes held by the workspace. */
+int64_t lm_pairs_ws_bytes(const lm_pairs_ws* ws);
+
+/* All pairs (i, j), i < j, with ||x_i - x_j||_p <= radius (closed ball) in
+ * degree space, where positions are the float32 lon/lat widened to double.
+ * p = 2: dx*dx + dy*dy <= radius*radius; p = 1: |dx| + |dy| <= radius;
+ * p = +inf: max(|dx|, |dy|) <= radius; other p >= 1:
+ * |dx|^p + |dy|^p <= radius^p.
+ * Output is sorted lexicographically by (i, j).  *npairs (device int64)
+ * receives the pair count P; the pairs are written only if P <= capacity
+ * (otherwise the caller enlarges its buffers and calls again).
+ * Replaces cKDTree(points).query_pairs(r=interaction_radius,
+ * p=interaction_norm) (interaction_simulator.py:93,98) followed by iteration
+ * over the returned set (interaction_simulator.py:104). */
+int lm_query_pairs(lm_pairs_ws* ws, int64_t n, const float* lon, const float* lat,
+                   double radius, double p, int32_t* pair_i, int32_t* pair_j,
+                   int64_t capacity, int64_t* npairs, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Rock-paper-scissors update                                               */
+/* ------------------------------------------------------------------------ */
+
+typedef struct lm_rps_ws lm_rps_ws;
+
+int lm_rps_ws_create(int64_t n_max, lm_rps_ws** out);
+void lm_rps_ws_destroy(lm_rps_ws* ws);
+int64_t lm_rps_ws_bytes(const lm_rps_ws* ws);
+
+/* Apply the rock-paper-scissors rule to the sorted pair list, with exactly the
+ * result of visiting the pairs one after another in (i, j) order: if the two
+ * species differ, draw u (pair of rank k uses word 0 of
+ * Philox4x64-10(counter = (k, step, 0, 0), key = (key0, key1)) mapped to
+ * [0,1) as (x >> 11) * 2^-53); the forward winner (rock over scissors with
+ * pRS, paper over rock with pPR, scissors over paper with pSP) wins iff
+ * u < p, and the loser takes the winner's species.  Species outside {1,2,3}
+ * never interact.  *npairs is read on the device; if it exceeds capacity the
+ * call changes nothing.
+ * Replaces the per-pair loop interaction_simulator.py:104-105 calling
+ * rock_paper_scissors_interaction (interactions.py:13-40). */
+int lm_rps_update(lm_rps_ws* ws, int64_t n, int8_t* species, const int32_t* pair_i,
+                  const int32_t* pair_j, const int64_t* npairs, int64_t capacity,
+                  double pRS, double pPR, double pSP, uint64_t key0, uint64_t key1,
+                  uint64_t step, void* stream);
+
+/* Size statistics of the interaction components of the last lm_rps_update on
+ * this workspace, written to device int64[4]: number of components with at
+ * least one pair, largest component (pairs), largest component (row heads),
+ * total pairs.  Diagnostic only. */
+int lm_rps_component_stats(lm_rps_ws* ws, int64_t* out4, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Counter-based uniforms                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* out[k] = word `word` of Philox4x64-10(counter = (c0 + k, c1, c2, c3),
+ * key = (key0, key1)) mapped to [0,1) as (x >> 11) * 2^-53, k < count.
+ * The same uniforms lm_rps_update and lm_advect_rk4_f64 draw, exposed so
+ * they can be fed to a CPU run of the rule.  No reference counterpart: the
+ * reference draws from numpy's global generator (interactions.py:20). */
+int lm_philox_uniforms(int64_t count, uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3,
+                       uint64_t key0, uint64_t key1, int32_t word, double* out, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Plain streaming copy, used to calibrate hardware byte counters.          */
+/* No reference counterpart.                                                */
+/* ------------------------------------------------------------------------ */
+int lm_copy_bytes(const void* src, void* dst, int64_t nbytes, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Per-kernel timing.  No reference counterpart (the reference times phases */
+/* with time() around each call, particle_advecter.py:221-244,              */
+/* interaction_simulator.py:91-106).                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Turn timing on (clearing earlier records) or off.  While on, every kernel
+ * launch records start/stop events on its own dispatch. */
+int lm_timing_enable(int on);
+/* Waits for the recorded launches and writes, per kernel id k < nkinds, the
+ * launch count and total milliseconds.  Returns the number of kernel ids. */
+int lm_timing_summary(int64_t* counts, double* total_ms, int32_t nkinds);
+/* Name of kernel id k ("" if out of range). */
+const char* lm_timing_kernel_name(int32_t kid);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LM_H */

@@ -1,0 +1,376 @@
+// Fixed-radius pair search in lon/lat degree space.
+//
+// Cell list with cells of side h >= radius over a fixed box, stored as a
+// counting sort into T = 2^k buckets (bucket = dense cell id mod T, so the
+// three cells of a stencil row are three consecutive buckets; when the box has
+// fewer than T cells the table is dense and nothing aliases):
+//   1. bin_count    : bucket counts (atomics; the scatter returns them to 0)
+//   2. scan         : bucket starts
+//   3. bin_scatter  : records {x, y, index} grouped by bucket
+//   4. pair_count   : per particle i, the neighbours j > i within radius over
+//                     the 3x3 cell stencil (run in bucket order for locality);
+//                     up to kNbr of them are kept in a scratch row
+//   5. scan         : row offsets and the pair count P
+//   6. pair_emit    : per particle, its neighbours sorted ascending, written at
+//                     its row offset; rows with more than kNbr neighbours are
+//                     re-searched by selection.  The result is the (i, j)
+//                     lexicographic pair list, independent of the atomic order
+//                     inside a bucket.
+// Distances use the float32 positions widened to double, with the same
+// operations cKDTree applies: dx*dx + dy*dy <= r*r for p = 2.  Since a
+// difference of two nearby floats is exact in double and its square is exact,
+// the test is independent of FMA contraction.
+#include "lm_common.h"
+#include "lm_scan.h"
+
+#include <cmath>
+
+namespace lm {
+
+constexpr int kNbr = 8;  // neighbours kept per particle before the selection path
+
+struct CellGrid {
+    double x0, y0, inv_h;
+    int32_t nx, ny;
+    uint32_t mask;  // T - 1
+    uint32_t T;
+};
+
+enum MetricKind { kL2 = 0, kL1 = 1, kLinf = 2, kLp = 3 };
+
+struct Metric {
+    int kind;
+    double r;    // radius
+    double r2;   // radius*radius (p = 2) or radius^p (general p)
+    double p;
+};
+
+__device__ __forceinline__ void cell_of(const CellGrid& g, float xf, float yf, int* cx, int* cy) {
+    const double fx = ((double)xf - g.x0) * g.inv_h;
+    const double fy = ((double)yf - g.y0) * g.inv_h;
+    int ix = 0, iy = 0;
+    if (fx >= 1.0) ix = fx >= (double)(g.nx - 1) ? g.nx - 1 : (int)fx;
+    if (fy >= 1.0) iy = fy >= (double)(g.ny - 1) ? g.ny - 1 : (int)fy;
+    *cx = ix;
+    *cy = iy;
+}
+
+__device__ __forceinline__ uint32_t bucket_of(const CellGrid& g, int cx, int cy) {
+    return ((uint32_t)cy * (uint32_t)g.nx + (uint32_t)cx) & g.mask;
+}
+
+__device__ __forceinline__ bool within(const Metric& m, float xa, float ya, float xb, float yb) {
+    const double dx = (double)xb - (double)xa;
+    const double dy = (double)yb - (double)ya;
+    switch (m.kind) {
+        case kL2: return dx * dx + dy * dy <= m.r2;
+        case kL1: return fabs(dx) + fabs(dy) <= m.r;
+        case kLinf: return fmax(fabs(dx), fabs(dy)) <= m.r;
+        default: return pow(fabs(dx), m.p) + pow(fabs(dy), m.p) <= m.r2;
+    }
+}
+
+// Visit the record slots of the (up to) 3x3 cells around (cx, cy).  Slot
+// ranges are clamped to nrec so that even inconsistent bucket starts cannot
+// index past the records.
+template <typename F>
+__device__ __forceinline__ void for_stencil(const CellGrid& g, const uint32_t* __restrict__ start,
+                                            uint32_t nrec, int cx, int cy, F&& visit) {
+    const int c0 = cx > 0 ? cx - 1 : 0;
+    const int c1 = cx < g.nx - 1 ? cx + 1 : g.nx - 1;
+#pragma unroll
+    for (int dyc = -1; dyc <= 1; ++dyc) {
+        const int row = cy + dyc;
+        if (row < 0 || row >= g.ny) continue;
+        const uint32_t b0 = bucket_of(g, c0, row);
+        const uint32_t b1 = bucket_of(g, c1, row);
+        if (b0 <= b1) {
+            const uint32_t e = min(start[b1 + 1], nrec);
+            for (uint32_t s = start[b0]; s < e; ++s) visit(s);
+        } else {  // the row wraps around the bucket table
+            const uint32_t e0 = min(start[g.T], nrec);
+            for (uint32_t s = start[b0]; s < e0; ++s) visit(s);
+            const uint32_t e1 = min(start[b1 + 1], nrec);
+            for (uint32_t s = start[0]; s < e1; ++s) visit(s);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+bin_count_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y, CellGrid g,
+                 uint32_t* __restrict__ count) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    int cx, cy;
+    cell_of(g, x[i], y[i], &cx, &cy);
+    atomicAdd(&count[bucket_of(g, cx, cy)], 1u);
+}
+
+__global__ void __launch_bounds__(kBlock)
+bin_scatter_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y, CellGrid g,
+                   const uint32_t* __restrict__ start, uint32_t* __restrict__ count,
+                   float4* __restrict__ recs) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float xf = x[i], yf = y[i];
+    int cx, cy;
+    cell_of(g, xf, yf, &cx, &cy);
+    const uint32_t b = bucket_of(g, cx, cy);
+    const uint32_t slot = start[b] + atomicSub(&count[b], 1u) - 1u;
+    if (slot < (uint64_t)n) recs[slot] = make_float4(xf, yf, __uint_as_float((uint32_t)i), 0.0f);
+}
+
+__global__ void __launch_bounds__(kBlock)
+pair_count_kernel(int64_t n, CellGrid g, Metric m, const uint32_t* __restrict__ start,
+                  const float4* __restrict__ recs, uint32_t* __restrict__ nbr_cnt,
+                  uint32_t* __restrict__ nbr) {
+    const int64_t s0 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (s0 >= n) return;
+    const float4 me = recs[s0];
+    const uint32_t i = __float_as_uint(me.z);
+    if (i >= (uint64_t)n) return;
+    int cx, cy;
+    cell_of(g, me.x, me.y, &cx, &cy);
+    uint32_t cnt = 0;
+    uint32_t* row = nbr + (size_t)i * kNbr;
+    for_stencil(g, start, (uint32_t)n, cx, cy, [&](uint32_t s) {
+        const float4 o = recs[s];
+        const uint32_t j = __float_as_uint(o.z);
+        if (j > i && within(m, me.x, me.y, o.x, o.y)) {
+            if (cnt < kNbr) row[cnt] = j;
+            ++cnt;
+        }
+    });
+    nbr_cnt[i] = cnt;
+}
+
+__global__ void __launch_bounds__(kBlock)
+pair_emit_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y, CellGrid g,
+                 Metric m, const uint32_t* __restrict__ start, const float4* __restrict__ recs,
+                 const uint32_t* __restrict__ nbr_cnt, const uint32_t* __restrict__ nbr,
+                 const uint32_t* __restrict__ row_off, int64_t capacity,
+                 int32_t* __restrict__ pair_i, int32_t* __restrict__ pair_j) {
+    const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i64 >= n) return;
+    if ((int64_t)row_off[n] > capacity) return;  // caller enlarges and calls again
+    const uint32_t i = (uint32_t)i64;
+    const uint32_t c = nbr_cnt[i];
+    if (c == 0) return;
+    const uint32_t o = row_off[i];
+    if (c <= (uint32_t)kNbr) {
+        uint32_t v[kNbr];
+        const uint32_t* row = nbr + (size_t)i * kNbr;
+#pragma unroll
+        for (int t = 0; t < kNbr; ++t) v[t] = (uint32_t)t < c ? row[t] : 0xffffffffu;
+        // insertion network on a fixed-size register array
+#pragma unroll
+        for (int a = 1; a < kNbr; ++a)
+#pragma unroll
+            for (int b = a; b > 0; --b) {
+                const uint32_t lo = min(v[b - 1], v[b]), hi = max(v[b - 1], v[b]);
+                v[b - 1] = lo;
+                v[b] = hi;
+            }
+#pragma unroll
+        for (int t = 0; t < kNbr; ++t)
+            if ((uint32_t)t < c) {
+                pair_i[o + t] = (int32_t)i;
+                pair_j[o + t] = (int32_t)v[t];
+            }
+        return;
+    }
+    // Dense neighbourhood: emit in ascending order by repeated selection.
+    const float xf = x[i], yf = y[i];
+    int cx, cy;
+    cell_of(g, xf, yf, &cx, &cy);
+    uint32_t last = i;
+    for (uint32_t t = 0; t < c; ++t) {
+        uint32_t best = 0xffffffffu;
+        for_stencil(g, start, (uint32_t)n, cx, cy, [&](uint32_t s) {
+            const float4 rec = recs[s];
+            const uint32_t j = __float_as_uint(rec.z);
+            if (j > last && j < best && within(m, xf, yf, rec.x, rec.y)) best = j;
+        });
+        pair_i[o + t] = (int32_t)i;
+        pair_j[o + t] = (int32_t)best;
+        last = best;
+    }
+}
+
+__global__ void set_i64_kernel(int64_t* p, int64_t v) { *p = v; }
+
+}  // namespace lm
+
+struct lm_pairs_ws {
+    int64_t n_max = 0;
+    double max_radius = 0.0;
+    lm::CellGrid grid{};
+    uint32_t* count = nullptr;     // [T]       zero between calls
+    uint32_t* start = nullptr;     // [T + 1]
+    uint32_t* scan_tmp = nullptr;  // block sums
+    float4* recs = nullptr;        // [n_max]
+    uint32_t* nbr_cnt = nullptr;   // [n_max]
+    uint32_t* nbr = nullptr;       // [n_max * kNbr]
+    uint32_t* row_off = nullptr;   // [n_max + 1]
+    int64_t bytes = 0;
+    bool dirty = false;            // a call stopped midway: counts must be re-zeroed
+};
+
+static void pairs_ws_free(lm_pairs_ws* w) {
+    if (!w) return;
+    (void)hipFree(w->count);
+    (void)hipFree(w->start);
+    (void)hipFree(w->scan_tmp);
+    (void)hipFree(w->recs);
+    (void)hipFree(w->nbr_cnt);
+    (void)hipFree(w->nbr);
+    (void)hipFree(w->row_off);
+    delete w;
+}
+
+template <typename T>
+static int ws_alloc(T** p, int64_t count, int64_t* bytes) {
+    const size_t b = (size_t)(count < 1 ? 1 : count) * sizeof(T);
+    hipError_t e = hipMalloc((void**)p, b);
+    if (e != hipSuccess) {
+        lm::set_error("hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
+        return LM_ERR_NOMEM;
+    }
+    *bytes += (int64_t)b;
+    return LM_OK;
+}
+
+static uint32_t next_pow2_u32(uint64_t v) {
+    uint64_t p = 1;
+    while (p < v) p <<= 1;
+    return (uint32_t)p;
+}
+
+extern "C" {
+
+int lm_pairs_ws_create(int64_t n_max, double lon_min, double lon_max, double lat_min, double lat_max,
+                       double max_radius, lm_pairs_ws** out) {
+    LM_REQUIRE(out != nullptr, LM_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    LM_REQUIRE(n_max >= 0 && n_max < (int64_t)0x7fffffff, LM_ERR_ARG, "n_max out of range");
+    LM_REQUIRE(std::isfinite(lon_min) && std::isfinite(lon_max) && std::isfinite(lat_min) &&
+                   std::isfinite(lat_max) && lon_max >= lon_min && lat_max >= lat_min,
+               LM_ERR_ARG, "invalid bounding box");
+    LM_REQUIRE(std::isfinite(max_radius) && max_radius > 0.0, LM_ERR_ARG, "max_radius must be > 0");
+
+    lm_pairs_ws* w = new lm_pairs_ws();
+    w->n_max = n_max;
+    w->max_radius = max_radius;
+    // cells strictly larger than the radius so rounding in the cell index can
+    // never separate two points within the radius by more than one cell
+    double h = max_radius * (1.0 + 1e-6);
+    const double ext = (lon_max - lon_min) > (lat_max - lat_min) ? (lon_max - lon_min) : (lat_max - lat_min);
+    if (ext / h > 30000.0) h = ext / 30000.0;
+    lm::CellGrid& g = w->grid;
+    g.inv_h = 1.0 / h;
+    g.x0 = lon_min - h;
+    g.y0 = lat_min - h;
+    g.nx = (int32_t)std::floor((lon_max - lon_min) * g.inv_h) + 3;
+    g.ny = (int32_t)std::floor((lat_max - lat_min) * g.inv_h) + 3;
+    const uint64_t cells = (uint64_t)g.nx * (uint64_t)g.ny;
+    uint64_t want = (uint64_t)n_max;
+    if (want < 4ull * (uint64_t)g.nx + 4) want = 4ull * (uint64_t)g.nx + 4;
+    if (want < 1024) want = 1024;
+    uint32_t T = next_pow2_u32(want);
+    if ((uint64_t)T >= cells) T = next_pow2_u32(cells);
+    if (T < 1024) T = 1024;
+    g.T = T;
+    g.mask = T - 1;
+
+    int rc = LM_OK;
+    int64_t nb = lm::scan_num_blocks((int64_t)T);
+    const int64_t nb2 = lm::scan_num_blocks(n_max);
+    if (nb2 > nb) nb = nb2;
+    if ((rc = ws_alloc(&w->count, T, &w->bytes)) || (rc = ws_alloc(&w->start, (int64_t)T + 4, &w->bytes)) ||
+        (rc = ws_alloc(&w->scan_tmp, nb, &w->bytes)) || (rc = ws_alloc(&w->recs, n_max, &w->bytes)) ||
+        (rc = ws_alloc(&w->nbr_cnt, n_max + 4, &w->bytes)) ||
+        (rc = ws_alloc(&w->nbr, n_max * lm::kNbr, &w->bytes)) ||
+        (rc = ws_alloc(&w->row_off, n_max + 4, &w->bytes))) {
+        pairs_ws_free(w);
+        return rc;
+    }
+    hipError_t e = hipMemset(w->count, 0, sizeof(uint32_t) * T);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        lm::set_error("workspace init failed: %s", hipGetErrorString(e));
+        pairs_ws_free(w);
+        return LM_ERR_HIP;
+    }
+    *out = w;
+    return LM_OK;
+}
+
+void lm_pairs_ws_destroy(lm_pairs_ws* ws) { pairs_ws_free(ws); }
+
+int64_t lm_pairs_ws_bytes(const lm_pairs_ws* ws) { return ws ? ws->bytes : 0; }
+
+int lm_query_pairs(lm_pairs_ws* ws, int64_t n, const float* lon, const float* lat, double radius,
+                   double p, int32_t* pair_i, int32_t* pair_j, int64_t capacity, int64_t* npairs,
+                   void* stream) {
+    LM_REQUIRE(ws != nullptr, LM_ERR_ARG, "workspace is NULL");
+    LM_REQUIRE(npairs != nullptr, LM_ERR_ARG, "npairs is NULL");
+    LM_REQUIRE(n >= 0 && n <= ws->n_max, LM_ERR_ARG, "n=%lld exceeds the workspace (n_max=%lld)",
+               (long long)n, (long long)ws->n_max);
+    LM_REQUIRE(radius >= 0.0 && radius <= ws->max_radius, LM_ERR_ARG,
+               "radius %g outside [0, %g] of the workspace", radius, ws->max_radius);
+    LM_REQUIRE(p >= 1.0, LM_ERR_ARG, "Minkowski p must be >= 1 (got %g)", p);
+    LM_REQUIRE(capacity >= 0, LM_ERR_ARG, "negative capacity");
+    hipStream_t s = (hipStream_t)stream;
+    if (n == 0) {
+        lm::set_i64_kernel<<<1, 1, 0, s>>>(npairs, 0);
+        LM_LAUNCH_CHECK();
+        return LM_OK;
+    }
+    LM_REQUIRE(lon && lat, LM_ERR_ARG, "NULL positions");
+    LM_REQUIRE(capacity == 0 || (pair_i && pair_j), LM_ERR_ARG, "NULL pair buffers");
+
+    lm::Metric m;
+    m.r = radius;
+    m.p = p;
+    if (p == 2.0) {
+        m.kind = lm::kL2;
+        m.r2 = radius * radius;
+    } else if (p == 1.0) {
+        m.kind = lm::kL1;
+        m.r2 = radius;
+    } else if (std::isinf(p)) {
+        m.kind = lm::kLinf;
+        m.r2 = radius;
+    } else {
+        m.kind = lm::kLp;
+        m.r2 = std::pow(radius, p);
+    }
+    const lm::CellGrid& g = ws->grid;
+    const unsigned gn = lm::grid_for(n);
+    if (ws->dirty) {
+        LM_HIP_CHECK(hipMemsetAsync(ws->count, 0, sizeof(uint32_t) * g.T, s));
+        ws->dirty = false;
+    }
+    ws->dirty = true;  // until every launch below went through
+
+    LM_LAUNCH(lm::K_BIN_COUNT, lm::bin_count_kernel, dim3(gn), dim3(lm::kBlock), 0, s, n, lon, lat, g, ws->count);
+    LM_LAUNCH_CHECK();
+    int rc = lm::exclusive_scan_u32(ws->count, (int64_t)g.T, ws->start, ws->scan_tmp, nullptr, s);
+    if (rc) return rc;
+    LM_LAUNCH(lm::K_BIN_SCATTER, lm::bin_scatter_kernel, dim3(gn), dim3(lm::kBlock), 0, s, n, lon, lat, g,
+              (const uint32_t*)ws->start, ws->count, ws->recs);
+    LM_LAUNCH_CHECK();
+    LM_LAUNCH(lm::K_PAIR_COUNT, lm::pair_count_kernel, dim3(gn), dim3(lm::kBlock), 0, s, n, g, m,
+              (const uint32_t*)ws->start, (const float4*)ws->recs, ws->nbr_cnt, ws->nbr);
+    LM_LAUNCH_CHECK();
+    rc = lm::exclusive_scan_u32(ws->nbr_cnt, n, ws->row_off, ws->scan_tmp, npairs, s);
+    if (rc) return rc;
+    LM_LAUNCH(lm::K_PAIR_EMIT, lm::pair_emit_kernel, dim3(gn), dim3(lm::kBlock), 0, s, n, lon, lat, g, m,
+              (const uint32_t*)ws->start, (const float4*)ws->recs, (const uint32_t*)ws->nbr_cnt,
+              (const uint32_t*)ws->nbr, (const uint32_t*)ws->row_off, capacity, pair_i, pair_j);
+    LM_LAUNCH_CHECK();
+    ws->dirty = false;
+    return LM_OK;
+}
+
+}  // extern "C"

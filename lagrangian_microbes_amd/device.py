@@ -1,0 +1,262 @@
+"""Device-side operators over torch tensors, one per liblm entry point.
+
+These are thin: validate on the host, hand raw pointers and the current
+stream to liblm, and return.  Nothing here synchronises with the host except
+where a data-dependent size must come back (PairSearch.query_sync).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import LmGrid, c_vp, check, ptr, require, stream_ptr
+
+
+def lib():
+    return _lib.load()
+
+
+# ---------------------------------------------------------------------------
+# velocity field on the device
+# ---------------------------------------------------------------------------
+class DeviceField:
+    """A VelocityField's levels on the GPU as interleaved (u, v) float64.
+
+    ``stage_fields(t0_rel, dt, nsteps)`` returns the 2*nsteps+1 node-wise
+    time-blended fields a run of RK4 steps samples (t0, t0+dt/2, ..., t0+n dt),
+    or the single level of a steady field, with the stride to pass to
+    ``advect_rk4_f64``.
+    """
+
+    def __init__(self, field, device=None):
+        self.field = field
+        self.device = torch.device(device or "cuda")
+        inter = field.interleaved()  # [nt, ny, nx, 2] float64
+        self.levels = torch.from_numpy(np.ascontiguousarray(inter)).to(self.device)
+        self.level_len = int(np.prod(inter.shape[1:]))
+        self.grid = LmGrid(field.nx, field.ny, field.x0, field.y0, 1.0 / field.dx, 1.0 / field.dy)
+        self._stage_buf = None
+
+    @property
+    def steady(self):
+        return self.field.nt == 1
+
+    def stage_fields(self, t0_rel, dt, nsteps):
+        if self.steady:
+            return self.levels[0], 0
+        lev0, w = self.field.stage_schedule(t0_rel, dt, nsteps)
+        ns = lev0.size
+        need = ns * self.level_len
+        if self._stage_buf is None or self._stage_buf.numel() < need:
+            self._stage_buf = torch.empty(need, dtype=torch.float64, device=self.device)
+        out = self._stage_buf[:need]
+        lev0_d = torch.from_numpy(lev0.astype(np.int32)).to(self.device)
+        w_d = torch.from_numpy(w.astype(np.float64)).to(self.device)
+        field_blend_f64(self.levels, self.level_len, lev0_d, w_d, out, lev0_host=lev0)
+        return out, self.level_len
+
+
+def field_blend_f64(levels, level_len, lev0, w, out, lev0_host=None):
+    """Node-wise time blend (lm_field_blend_f64); lev0_host, if given, is the
+    host copy of lev0 used for the range check (saves a device read-back)."""
+    require(levels, torch.float64, name="levels")
+    require(lev0, torch.int32, name="lev0")
+    require(w, torch.float64, numel=lev0.numel(), name="w")
+    nstage = lev0.numel()
+    require(out, torch.float64, min_numel=nstage * level_len, name="out")
+    nt = levels.numel() // level_len
+    l0 = np.asarray(lev0_host) if lev0_host is not None else lev0.cpu().numpy()
+    if l0.size != nstage:
+        raise ValueError("lev0_host does not match lev0")
+    if nstage and (l0.min() < 0 or l0.max() + 1 >= nt):
+        raise ValueError("lev0 out of range of the field's time levels")
+    for a in range(0, nstage, 65535):
+        b = min(nstage, a + 65535)
+        check(lib().lm_field_blend_f64(ptr(levels), level_len, c_vp(lev0.data_ptr() + 4 * a),
+                                       c_vp(w.data_ptr() + 8 * a), b - a,
+                                       c_vp(out.data_ptr() + 8 * a * level_len), stream_ptr()))
+
+
+def advect_rk4_f64(lon, lat, stage_fields, stage_stride, grid, nsteps, dt, rec_lon=None, rec_lat=None,
+                   rec_stride=None, walk_amp=0.0, key=(0, 0), step0=0, oob=None):
+    """RK4 steps on device lon/lat (float64, in place); see lm_advect_rk4_f64."""
+    n = lon.numel()
+    require(lon, torch.float64, name="lon")
+    require(lat, torch.float64, numel=n, name="lat")
+    require(stage_fields, torch.float64, name="stage_fields")
+    flen = 2 * grid.nx * grid.ny
+    if stage_stride == 0:
+        if stage_fields.numel() < flen:
+            raise ValueError("stage field smaller than the grid")
+    elif stage_fields.numel() < (2 * nsteps) * stage_stride + flen:
+        raise ValueError("not enough stage fields for nsteps")
+    if oob is None:
+        oob = torch.zeros(1, dtype=torch.int32, device=lon.device)
+    require(oob, torch.int32, min_numel=1, name="oob")
+    if rec_lon is not None:
+        rec_stride = n if rec_stride is None else rec_stride
+        require(rec_lon, torch.float32, name="rec_lon", min_numel=(nsteps - 1) * rec_stride + n)
+        require(rec_lat, torch.float32, name="rec_lat", min_numel=(nsteps - 1) * rec_stride + n)
+    else:
+        rec_stride = 0
+    check(lib().lm_advect_rk4_f64(n, ptr(lon), ptr(lat), ptr(stage_fields), int(stage_stride),
+                                  ctypes.byref(grid), int(nsteps), float(dt), ptr(rec_lon), ptr(rec_lat),
+                                  int(rec_stride), float(walk_amp), int(key[0]), int(key[1]), int(step0),
+                                  ptr(oob), stream_ptr()))
+    return oob
+
+
+def philox_uniforms(count, counter, key, word=0, device=None):
+    """word `word` of Philox4x64-10((c0 + k, c1, c2, c3), key) as doubles in [0,1)."""
+    out = torch.empty(int(count), dtype=torch.float64, device=device or "cuda")
+    c0, c1, c2, c3 = (int(c) for c in counter)
+    check(lib().lm_philox_uniforms(int(count), c0, c1, c2, c3, int(key[0]), int(key[1]), int(word),
+                                   ptr(out), stream_ptr()))
+    return out
+
+
+def copy_bytes(src, dst, nbytes):
+    check(lib().lm_copy_bytes(ptr(src), ptr(dst), int(nbytes), stream_ptr()))
+
+
+# ---------------------------------------------------------------------------
+# pair search
+# ---------------------------------------------------------------------------
+class PairSearch:
+    """Fixed-radius pair search (replaces cKDTree(...).query_pairs).
+
+    ``bbox`` = (lon_min, lon_max, lat_min, lat_max) where the particles are
+    expected; particles outside are still handled exactly.
+    """
+
+    def __init__(self, n_max, bbox, max_radius, device=None):
+        self.n_max = int(n_max)
+        self.max_radius = float(max_radius)
+        self.bbox = tuple(float(b) for b in bbox)
+        h = c_vp()
+        check(lib().lm_pairs_ws_create(self.n_max, self.bbox[0], self.bbox[1], self.bbox[2], self.bbox[3],
+                                       self.max_radius, ctypes.byref(h)))
+        self._ws = h
+        self.device = torch.device(device or "cuda")
+        self.pair_i = None
+        self.pair_j = None
+        self.npairs = torch.zeros(1, dtype=torch.int64, device=self.device)
+
+    def __del__(self):
+        ws = getattr(self, "_ws", None)
+        if ws and _lib._lib is not None:
+            _lib._lib.lm_pairs_ws_destroy(ws)
+            self._ws = None
+
+    @property
+    def workspace_bytes(self):
+        return int(lib().lm_pairs_ws_bytes(self._ws))
+
+    def query(self, lon, lat, radius, p, pair_i, pair_j, npairs):
+        """Asynchronous: pairs into caller buffers (capacity = their length)."""
+        n = lon.numel()
+        require(lon, torch.float32, name="lon")
+        require(lat, torch.float32, numel=n, name="lat")
+        require(pair_i, torch.int32, name="pair_i")
+        require(pair_j, torch.int32, numel=pair_i.numel(), name="pair_j")
+        require(npairs, torch.int64, min_numel=1, name="npairs")
+        if n > self.n_max:
+            raise ValueError(f"{n} particles exceed the workspace's {self.n_max}")
+        check(lib().lm_query_pairs(self._ws, n, ptr(lon), ptr(lat), float(radius), float(p), ptr(pair_i),
+                                   ptr(pair_j), pair_i.numel(), ptr(npairs), stream_ptr()))
+
+    def ensure_capacity(self, cap):
+        if self.pair_i is None or self.pair_i.numel() < cap:
+            cap = max(int(cap), 1024)
+            self.pair_i = torch.empty(cap, dtype=torch.int32, device=self.device)
+            self.pair_j = torch.empty(cap, dtype=torch.int32, device=self.device)
+
+    def query_sync(self, lon, lat, radius, p=2.0):
+        """Pairs as exact-length device tensors (reads the count back)."""
+        n = lon.numel()
+        self.ensure_capacity(max(2 * n, 1024))
+        for _ in range(2):
+            self.query(lon, lat, radius, p, self.pair_i, self.pair_j, self.npairs)
+            P = int(self.npairs.item())
+            if P <= self.pair_i.numel():
+                return self.pair_i[:P], self.pair_j[:P]
+            self.ensure_capacity(P + P // 4 + 1024)
+        raise RuntimeError("pair buffer growth did not converge")
+
+
+# ---------------------------------------------------------------------------
+# rock-paper-scissors
+# ---------------------------------------------------------------------------
+class RPSUpdater:
+    """Rock-paper-scissors over a sorted device pair list (replaces the
+    per-pair Python loop over rock_paper_scissors_interaction)."""
+
+    def __init__(self, n_max, device=None):
+        self.n_max = int(n_max)
+        h = c_vp()
+        check(lib().lm_rps_ws_create(self.n_max, ctypes.byref(h)))
+        self._ws = h
+        self.device = torch.device(device or "cuda")
+        self._stats = torch.zeros(4, dtype=torch.int64, device=self.device)
+
+    def __del__(self):
+        ws = getattr(self, "_ws", None)
+        if ws and _lib._lib is not None:
+            _lib._lib.lm_rps_ws_destroy(ws)
+            self._ws = None
+
+    @property
+    def workspace_bytes(self):
+        return int(lib().lm_rps_ws_bytes(self._ws))
+
+    def update(self, species, pair_i, pair_j, npairs, pRS, pPR, pSP, key, step):
+        n = species.numel()
+        require(species, torch.int8, name="species")
+        require(pair_i, torch.int32, name="pair_i")
+        require(pair_j, torch.int32, numel=pair_i.numel(), name="pair_j")
+        require(npairs, torch.int64, min_numel=1, name="npairs")
+        if n > self.n_max:
+            raise ValueError(f"{n} particles exceed the workspace's {self.n_max}")
+        check(lib().lm_rps_update(self._ws, n, ptr(species), ptr(pair_i), ptr(pair_j), ptr(npairs),
+                                  pair_i.numel(), float(pRS), float(pPR), float(pSP), int(key[0]),
+                                  int(key[1]), int(step), stream_ptr()))
+
+    def component_stats(self):
+        """(components, max pairs in one, max row heads in one, total pairs) of the last update."""
+        check(lib().lm_rps_component_stats(self._ws, ptr(self._stats), stream_ptr()))
+        return tuple(int(v) for v in self._stats.cpu().tolist())
+
+
+# ---------------------------------------------------------------------------
+# per-kernel timing
+# ---------------------------------------------------------------------------
+class KernelTimer:
+    """Context manager: every liblm launch inside is timed with events on its
+    own dispatch; ``summary()`` returns {kernel: (launches, total_ms)}."""
+
+    def __enter__(self):
+        check(lib().lm_timing_enable(1))
+        return self
+
+    def __exit__(self, *exc):
+        self._summary = self._read()
+        check(lib().lm_timing_enable(0))
+        return False
+
+    @staticmethod
+    def _read():
+        nk = 32
+        counts = np.zeros(nk, dtype=np.int64)
+        ms = np.zeros(nk, dtype=np.float64)
+        got = lib().lm_timing_summary(counts.ctypes.data_as(c_vp), ms.ctypes.data_as(c_vp), nk)
+        if got < 0:
+            check(got)
+        out = {}
+        for k in range(min(got, nk)):
+            if counts[k]:
+                out[lib().lm_timing_kernel_name(k).decode()] = (int(counts[k]), float(ms[k]))
+        return out
+
+    def summary(self):
+        return getattr(self, "_summary", None) or self._read()

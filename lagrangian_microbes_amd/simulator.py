@@ -1,0 +1,167 @@
+"""Fused advection + interactions on the GPU (no intermediate files).
+
+The reference runs two passes joined through particle_data.nc: advection
+(ParticleAdvecter.time_step, particle_advecter.py:123-260) then interactions
+(InteractionSimulator.time_step, interaction_simulator.py:54-122), where step i
+of the interactions uses the float32 positions recorded after advection step
+i.  ``FusedRunner`` does both per output chunk on the device: one RK4 launch
+advances every particle through the chunk and records float32 positions,
+then each step's pair search and rock-paper-scissors update read those
+records.  The species and pair lists are therefore identical to the two-pass
+result on the same inputs.  ``MicrobeSimulator`` wraps it with the outputs
+of both passes (particle_data.nc and microbe_data.nc).
+"""
+import logging
+import os
+from time import time
+
+import numpy as np
+import torch
+
+from . import device as dev
+from .engine import InteractionEngine, new_key, walk_amplitude
+from .interactions import rock_paper_scissors_interaction
+from .netcdf_io import write_particle_dataset
+from .utils import pretty_time
+from .velocity_fields import VelocityField
+
+logger = logging.getLogger(__name__)
+
+
+class FusedRunner:
+    """Device state and buffers for a fused run.
+
+    lon0/lat0: float64 initial positions; species0: int8; field: VelocityField;
+    params: {"pRS", "pPR", "pSP"}; key: Philox key of the pair uniforms;
+    walk_key / Kh (deg^2/s): the random walk (off when Kh == 0).
+    """
+
+    def __init__(self, lon0, lat0, species0, field, radius, params, key, dt_s, norm=2.0, chunk=100, Kh=0.0,
+                 walk_key=(0, 0x6B68), device=None, bbox=None):
+        self.device = torch.device(device or "cuda")
+        self.n = int(np.asarray(lon0).size)
+        self.lon0 = np.asarray(lon0, dtype=np.float64)
+        self.lat0 = np.asarray(lat0, dtype=np.float64)
+        self.species0 = np.asarray(species0, dtype=np.int8)
+        self.params = dict(params)
+        self.key = tuple(key)
+        self.dt_s = float(dt_s)
+        self.chunk = int(chunk)
+        self.walk_amp = walk_amplitude(Kh, self.dt_s)
+        self.walk_key = tuple(walk_key)
+        self.dfield = dev.DeviceField(field, self.device)
+        self.engine = InteractionEngine(self.n, bbox or field.bounds, radius, norm, self.device)
+        self.rec_x = torch.empty((self.chunk, self.n), dtype=torch.float32, device=self.device)
+        self.rec_y = torch.empty((self.chunk, self.n), dtype=torch.float32, device=self.device)
+        self.rec_s = torch.empty((self.chunk, self.n), dtype=torch.int8, device=self.device)
+        self.npairs = torch.zeros(self.chunk, dtype=torch.int64, device=self.device)
+        self.oob = torch.zeros(1, dtype=torch.int32, device=self.device)
+        self.reset()
+
+    def reset(self):
+        """Back to the initial state, step 0 and the field's first time level."""
+        self.lon = torch.from_numpy(self.lon0.copy()).to(self.device)
+        self.lat = torch.from_numpy(self.lat0.copy()).to(self.device)
+        self.species = torch.from_numpy(self.species0.copy()).to(self.device)
+        self.step = 0          # global step index: keys the uniforms
+        self.clock_steps = 0   # steps since the field's first time level
+        self.oob.zero_()
+
+    def advance(self, nsteps):
+        """Run up to self.chunk steps; returns the number of steps done.
+
+        After the call rec_x/rec_y/rec_s[:nsteps] hold the step records and
+        npairs[:nsteps] the pair counts.  Nothing is synchronised except the
+        chunk-end check of the pair counts inside the interaction engine.
+        """
+        k = min(int(nsteps), self.chunk)
+        if k <= 0:
+            return 0
+        stage, stride = self.dfield.stage_fields(self.clock_steps * self.dt_s, self.dt_s, k)
+        dev.advect_rk4_f64(self.lon, self.lat, stage, stride, self.dfield.grid, k, self.dt_s, self.rec_x,
+                           self.rec_y, self.n, self.walk_amp, self.walk_key, self.step, self.oob)
+        self.engine.run_chunk(self.rec_x[:k], self.rec_y[:k], self.species, self.params, self.key, self.step,
+                              rec_species=self.rec_s[:k], npairs=self.npairs[:k])
+        self.step += k
+        self.clock_steps += k
+        return k
+
+    def check_domain(self):
+        n_oob = int(self.oob.item())
+        if n_oob:
+            raise RuntimeError(f"{n_oob} particle-step(s) left the velocity field's domain "
+                               f"{self.dfield.field.bounds}")
+
+
+class MicrobeSimulator:
+    """Advection and pair interactions of one population, fused on the GPU.
+
+    Arguments follow ParticleAdvecter and InteractionSimulator; the
+    interaction must be the rock-paper-scissors plugin tuple.  ``time_step``
+    writes particle_data.nc and microbe_data.nc with the layout of the two
+    reference passes (set write_outputs=False to keep the records in memory
+    only) and leaves the final species in microbe_properties["species"].
+    """
+
+    def __init__(self, particle_lons, particle_lats, velocity_field, pair_interaction, interaction_radius,
+                 interaction_norm=2, output_dir=".", output_chunk_iters=100, Kh=0, seed=None, philox_key=None,
+                 device=None):
+        assert isinstance(velocity_field, VelocityField), "velocity_field must be a VelocityField"
+        assert particle_lons.size == particle_lats.size
+        fn, params, props = pair_interaction
+        if fn is not rock_paper_scissors_interaction:
+            raise ValueError("the fused GPU path runs the rock-paper-scissors interaction only; use "
+                             "ParticleAdvecter + InteractionSimulator for other callbacks")
+        assert output_chunk_iters >= 1
+        self.velocity_field = velocity_field
+        self.microbe_properties = props
+        self.params = params
+        self.output_dir = os.path.abspath(output_dir)
+        os.makedirs(self.output_dir, exist_ok=True)
+        self.radius = float(interaction_radius)
+        self.norm = float(interaction_norm)
+        self.chunk = int(output_chunk_iters)
+        self.Kh = Kh / 1e10
+        self.particle_lons = np.asarray(particle_lons, dtype=np.float64)
+        self.particle_lats = np.asarray(particle_lats, dtype=np.float64)
+        self.philox_key = tuple(philox_key) if philox_key is not None else new_key()
+        self.walk_key = (new_key(seed)[0], 0x6B68)
+        self.device = torch.device(device or "cuda")
+        self.iteration = 0
+
+    def time_step(self, start_time, end_time, dt, write_outputs=True):
+        dt_s = dt.total_seconds()
+        iters = max(0, (end_time - start_time) // dt)
+        runner = FusedRunner(self.particle_lons, self.particle_lats, self.microbe_properties["species"],
+                             self.velocity_field, self.radius, self.params, self.philox_key, dt_s, self.norm,
+                             self.chunk, self.Kh, self.walk_key, self.device)
+        runner.step = self.iteration  # uniforms keyed by the global step; the clock restarts at 0
+        n = runner.n
+        lon = np.zeros((n, iters), dtype=np.float32)
+        lat = np.zeros((n, iters), dtype=np.float32)
+        spc = np.zeros((n, iters), dtype=np.int8)
+        self.pairs_per_step = np.zeros(iters, dtype=np.int64)
+        tic = time()
+        done = 0
+        while done < iters:
+            k = runner.advance(iters - done)
+            runner.check_domain()
+            lon[:, done:done + k] = runner.rec_x[:k].cpu().numpy().T
+            lat[:, done:done + k] = runner.rec_y[:k].cpu().numpy().T
+            spc[:, done:done + k] = runner.rec_s[:k].cpu().numpy().T
+            self.pairs_per_step[done:done + k] = runner.npairs[:k].cpu().numpy()
+            done += k
+        logger.info("Fused advection + interactions, {:d} steps: {:s}".format(iters, pretty_time(time() - tic)))
+        self.microbe_properties["species"][:] = runner.species.cpu().numpy()
+        self.final_lons = runner.lon.cpu().numpy()
+        self.final_lats = runner.lat.cpu().numpy()
+        # a later call continues from the float64 end state
+        self.particle_lons, self.particle_lats = self.final_lons, self.final_lats
+        self.iteration += iters
+        self.records = {"longitude": lon, "latitude": lat, "species": spc}
+        if write_outputs:
+            times = [start_time + k * dt for k in range(iters)]
+            write_particle_dataset(os.path.join(self.output_dir, "particle_data.nc"),
+                                   {"longitude": lon, "latitude": lat}, times, n)
+            write_particle_dataset(os.path.join(self.output_dir, "microbe_data.nc"),
+                                   {"longitude": lon, "latitude": lat, "species": spc}, times, n)

@@ -1,0 +1,250 @@
+/*
+ * ORACLE -- test infrastructure only.  A plain-C CPU restatement of the
+ * lagrangian-microbes hot path, used by tests/, by __graft_entry__.smoke() and
+ * by bench.py's cpu_baseline leg as the checker.  The product (the package
+ * lagrangian_microbes_amd and liblm.so) never loads it.
+ *
+ * What it restates (reference file:line):
+ *  - RK4 advection as parcels.AdvectionRK4 on a RectilinearZGrid with
+ *    mesh="spherical" and linear interpolation (call sites
+ *    particle_advecter.py:177-187,222-223; parcels 2.0.0beta2 is pinned at
+ *    environment.yml:80 and is not available here): classic RK4, bilinear in
+ *    lon/lat, linear in time, u/(1852*60*cos(lat)) and v/(1852*60) to turn
+ *    m/s into deg/s.  The operation order is the one liblm documents, so a
+ *    GPU run on the same inputs must agree to the last bit.
+ *  - The post-step float32 record (particle_advecter.py:232-235) and the Kh
+ *    random walk (particle_advecter.py:240-242) with keyed Philox uniforms.
+ *  - The rock-paper-scissors rule (interactions.py:13-40) applied pair by pair
+ *    in sorted (i, j) order as the loop at interaction_simulator.py:104-105
+ *    does over the pair set, with keyed Philox uniforms instead of numpy's
+ *    global generator (interactions.py:20).
+ *  - query_pairs semantics (interaction_simulator.py:93,98): all i < j with
+ *    distance <= r, brute force.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fPIC -shared (oracle/build.py).  Every
+ * fused multiply-add is an explicit fma() (correctly rounded in C99), so the
+ * results do not depend on the compiler contracting expressions.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x64-10 (Salmon et al., SC'11; Random123 constants)                 */
+/* ------------------------------------------------------------------------ */
+static void mulhilo64(uint64_t a, uint64_t b, uint64_t* hi, uint64_t* lo) {
+    unsigned __int128 p = (unsigned __int128)a * (unsigned __int128)b;
+    *lo = (uint64_t)p;
+    *hi = (uint64_t)(p >> 64);
+}
+
+void oracle_philox4x64_10(const uint64_t ctr_in[4], const uint64_t key_in[2], uint64_t out[4]) {
+    uint64_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint64_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t hi0, lo0, hi1, lo1;
+        mulhilo64(0xD2E7470EE14C6C93ull, c0, &hi0, &lo0);
+        mulhilo64(0xCA5A826395121157ull, c2, &hi1, &lo1);
+        uint64_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B97F4A7C15ull;
+        k1 += 0xBB67AE8584CAA73Bull;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+static double u01(uint64_t x) { return (double)(x >> 11) * (1.0 / 9007199254740992.0); }
+
+static double philox_u01(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3, uint64_t k0, uint64_t k1,
+                         int word) {
+    uint64_t ctr[4] = {c0, c1, c2, c3}, key[2] = {k0, k1}, out[4];
+    oracle_philox4x64_10(ctr, key, out);
+    return u01(out[word]);
+}
+
+void oracle_philox_uniforms(int64_t count, uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3,
+                            uint64_t k0, uint64_t k1, int word, double* out) {
+    for (int64_t k = 0; k < count; ++k) out[k] = philox_u01(c0 + (uint64_t)k, c1, c2, c3, k0, k1, word);
+}
+
+/* ------------------------------------------------------------------------ */
+/* cos(lat deg): Taylor series of cos through x^22, Horner in x^2            */
+/* ------------------------------------------------------------------------ */
+double oracle_cos_deg(double lat_deg) {
+    const double x = lat_deg * 0.017453292519943295;
+    const double z = x * x;
+    double p = -0x1.0ce396db7f853p-70;
+    p = fma(p, z, 0x1.e542ba4020225p-62);
+    p = fma(p, z, -0x1.6827863b97d97p-53);
+    p = fma(p, z, 0x1.ae7f3e733b81fp-45);
+    p = fma(p, z, -0x1.93974a8c07c9dp-37);
+    p = fma(p, z, 0x1.1eed8eff8d898p-29);
+    p = fma(p, z, -0x1.27e4fb7789f5cp-22);
+    p = fma(p, z, 0x1.a01a01a01a01ap-16);
+    p = fma(p, z, -0x1.6c16c16c16c17p-10);
+    p = fma(p, z, 0x1.5555555555555p-5);
+    p = fma(p, z, -0.5);
+    return fma(p, z, 1.0);
+}
+
+/* ------------------------------------------------------------------------ */
+/* velocity field: node-wise time blend, bilinear sample, RK4                */
+/* ------------------------------------------------------------------------ */
+void oracle_field_blend_f64(const double* levels, int64_t len, const int32_t* lev0, const double* w,
+                            int32_t nstage, double* out) {
+    for (int32_t s = 0; s < nstage; ++s) {
+        const double* a = levels + (int64_t)lev0[s] * len;
+        for (int64_t k = 0; k < len; ++k) {
+            const double f0 = a[k], f1 = a[len + k];
+            out[(int64_t)s * len + k] = fma(w[s], f1 - f0, f0);
+        }
+    }
+}
+
+typedef struct {
+    int32_t nx, ny;
+    double x0, y0, inv_dx, inv_dy;
+} grid_t;
+
+#define INV_M_PER_DEG 0x1.2df73a213fa8bp-17 /* 1 / (1852 * 60) */
+
+static int sample_uv(const double* f, const grid_t* g, double x, double y, double* ud, double* vd) {
+    const double fx = (x - g->x0) * g->inv_dx;
+    const double fy = (y - g->y0) * g->inv_dy;
+    if (!(fx >= 0.0 && fx <= (double)(g->nx - 1) && fy >= 0.0 && fy <= (double)(g->ny - 1))) return 0;
+    int i = (int)fx, j = (int)fy;
+    if (i > g->nx - 2) i = g->nx - 2;
+    if (j > g->ny - 2) j = g->ny - 2;
+    const double xi = fx - (double)i, eta = fy - (double)j;
+    const double* r0 = f + 2 * ((int64_t)j * g->nx + i);
+    const double* r1 = r0 + 2 * (int64_t)g->nx;
+    /* a = node (i,j), b = (i+1,j), c = (i,j+1), d = (i+1,j+1); [0]=u, [1]=v */
+    const double ub = fma(xi, r0[2] - r0[0], r0[0]);
+    const double ut = fma(xi, r1[2] - r1[0], r1[0]);
+    const double u = fma(eta, ut - ub, ub);
+    const double vb = fma(xi, r0[3] - r0[1], r0[1]);
+    const double vt = fma(xi, r1[3] - r1[1], r1[1]);
+    const double v = fma(eta, vt - vb, vb);
+    *ud = (u * INV_M_PER_DEG) / oracle_cos_deg(y);
+    *vd = v * INV_M_PER_DEG;
+    return 1;
+}
+
+static int rk4_step(const double* f0, const double* fh, const double* f1, const grid_t* g, double hh,
+                    double h, double h6, double* x, double* y) {
+    double u1, v1, u2, v2, u3, v3, u4, v4;
+    const double x0 = *x, y0 = *y;
+    if (!sample_uv(f0, g, x0, y0, &u1, &v1)) return 0;
+    if (!sample_uv(fh, g, fma(u1, hh, x0), fma(v1, hh, y0), &u2, &v2)) return 0;
+    if (!sample_uv(fh, g, fma(u2, hh, x0), fma(v2, hh, y0), &u3, &v3)) return 0;
+    if (!sample_uv(f1, g, fma(u3, h, x0), fma(v3, h, y0), &u4, &v4)) return 0;
+    const double su = fma(2.0, u2 + u3, u1 + u4);
+    const double sv = fma(2.0, v2 + v3, v1 + v4);
+    *x = fma(su, h6, x0);
+    *y = fma(sv, h6, y0);
+    return 1;
+}
+
+/* Returns the number of out-of-grid particle-steps. */
+int64_t oracle_advect_rk4_f64(int64_t n, double* lon, double* lat, const double* stage_fields,
+                              int64_t stage_stride, int32_t nx, int32_t ny, double x0, double y0,
+                              double inv_dx, double inv_dy, int32_t nsteps, double dt, float* rec_lon,
+                              float* rec_lat, int64_t rec_stride, double walk_amp, uint64_t k0,
+                              uint64_t k1, uint64_t step0) {
+    grid_t g = {nx, ny, x0, y0, inv_dx, inv_dy};
+    const double hh = 0.5 * dt, h6 = dt / 6.0;
+    int64_t oob = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        double x = lon[i], y = lat[i];
+        for (int32_t s = 0; s < nsteps; ++s) {
+            const double* f0 = stage_fields + (int64_t)(2 * s) * stage_stride;
+            if (!rk4_step(f0, f0 + stage_stride, f0 + 2 * stage_stride, &g, hh, dt, h6, &x, &y)) ++oob;
+            if (rec_lon) {
+                rec_lon[(int64_t)s * rec_stride + i] = (float)x;
+                rec_lat[(int64_t)s * rec_stride + i] = (float)y;
+            }
+            if (walk_amp > 0.0) {
+                uint64_t ctr[4] = {(uint64_t)i, step0 + (uint64_t)s, 1, 0}, key[2] = {k0, k1}, o[4];
+                oracle_philox4x64_10(ctr, key, o);
+                y = fma(fma(2.0, u01(o[0]), -1.0), walk_amp, y);
+                x = fma(fma(2.0, u01(o[1]), -1.0), walk_amp, x);
+            }
+        }
+        lon[i] = x;
+        lat[i] = y;
+    }
+    return oob;
+}
+
+/* ------------------------------------------------------------------------ */
+/* rock-paper-scissors, sequential over the sorted pair list                 */
+/* ------------------------------------------------------------------------ */
+enum { ROCK = 1, PAPER = 2, SCISSORS = 3 };
+
+/* Pair k of the list draws word 0 of Philox(counter = (k, step, 0, 0)). */
+void oracle_rps_sequential(int64_t n, int8_t* species, const int32_t* pi, const int32_t* pj, int64_t npairs,
+                           double pRS, double pPR, double pSP, uint64_t k0, uint64_t k1, uint64_t step) {
+    (void)n;
+    for (int64_t k = 0; k < npairs; ++k) {
+        const int32_t p1 = pi[k], p2 = pj[k];
+        if (species[p1] == species[p2]) continue; /* interactions.py:17 */
+        const int s1 = species[p1], s2 = species[p2];
+        const double r = philox_u01((uint64_t)k, step, 0, 0, k0, k1, 0);
+        int32_t winner = -1; /* interactions.py:22-35 */
+        if (s1 == ROCK && s2 == SCISSORS)
+            winner = r < pRS ? p1 : p2;
+        else if (s1 == ROCK && s2 == PAPER)
+            winner = r < pPR ? p2 : p1;
+        else if (s1 == PAPER && s2 == ROCK)
+            winner = r < pPR ? p1 : p2;
+        else if (s1 == PAPER && s2 == SCISSORS)
+            winner = r < pSP ? p2 : p1;
+        else if (s1 == SCISSORS && s2 == ROCK)
+            winner = r < pRS ? p2 : p1;
+        else if (s1 == SCISSORS && s2 == PAPER)
+            winner = r < pSP ? p1 : p2;
+        if (winner == p1) /* interactions.py:37-40 */
+            species[p2] = species[p1];
+        else if (winner == p2)
+            species[p1] = species[p2];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* brute-force pair search (query_pairs semantics)                           */
+/* ------------------------------------------------------------------------ */
+/* kind: 0 p=2, 1 p=1, 2 p=inf, 3 general p.  Writes up to cap pairs in (i,j)
+ * order and returns the total count. */
+int64_t oracle_pairs_bruteforce(int64_t n, const float* x, const float* y, double r, double p, int32_t* out_i,
+                                int32_t* out_j, int64_t cap) {
+    int kind = (p == 2.0) ? 0 : (p == 1.0) ? 1 : isinf(p) ? 2 : 3;
+    const double r2 = kind == 0 ? r * r : kind == 3 ? pow(r, p) : r;
+    int64_t cnt = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double xi = x[i], yi = y[i];
+        for (int64_t j = i + 1; j < n; ++j) {
+            const double dx = (double)x[j] - xi, dy = (double)y[j] - yi;
+            int hit;
+            switch (kind) {
+                case 0: hit = dx * dx + dy * dy <= r2; break;
+                case 1: hit = fabs(dx) + fabs(dy) <= r2; break;
+                case 2: hit = fmax(fabs(dx), fabs(dy)) <= r2; break;
+                default: hit = pow(fabs(dx), p) + pow(fabs(dy), p) <= r2; break;
+            }
+            if (hit) {
+                if (cnt < cap) {
+                    out_i[cnt] = (int32_t)i;
+                    out_j[cnt] = (int32_t)j;
+                }
+                ++cnt;
+            }
+        }
+    }
+    return cnt;
+}

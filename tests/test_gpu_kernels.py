@@ -1,0 +1,233 @@
+"""GPU parity of each liblm entry point against the oracle (bit-exact)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from oracle.pairs import sorted_kdtree_pairs
+from helpers import grid_of, oracle_advect
+from lagrangian_microbes_amd import configs, device as dev
+from lagrangian_microbes_amd.velocity_fields import VelocityField
+
+pytestmark = pytest.mark.gpu
+R = configs.KM_DEG
+
+
+def T(a, device="cuda"):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(device)
+
+
+# ---------------------------------------------------------------- philox
+def test_philox_uniforms_bit_exact(gpu):
+    for (c, key, word) in [((0, 7, 0, 0), (1, 0), 0), ((123456789, 3, 1, 0), (99, 5), 1),
+                           ((2 ** 63, 1, 2, 3), (2 ** 62, 2 ** 61), 3)]:
+        g = dev.philox_uniforms(5000, c, key, word).cpu().numpy()
+        o = oracle.philox_uniforms(5000, *c, key[0], key[1], word)
+        assert np.array_equal(g, o)
+
+
+# ---------------------------------------------------------------- advection
+def gpu_advect(f, lon, lat, dt, nsteps, walk_amp=0.0, key=(0, 0), step0=0):
+    df = dev.DeviceField(f)
+    stage, stride = df.stage_fields(0.0, dt, nsteps)
+    x, y = T(lon.astype(np.float64)), T(lat.astype(np.float64))
+    n = x.numel()
+    rl = torch.empty((nsteps, n), dtype=torch.float32, device="cuda")
+    rt = torch.empty((nsteps, n), dtype=torch.float32, device="cuda")
+    oob = dev.advect_rk4_f64(x, y, stage, stride, df.grid, nsteps, dt, rl, rt, n, walk_amp, key, step0)
+    return x.cpu().numpy(), y.cpu().numpy(), rl.cpu().numpy(), rt.cpu().numpy(), int(oob.item())
+
+
+@pytest.mark.parametrize("name,steps,stride", [("rps_10k_steady_24h", 24, 1), ("rps_490k_30day", 720, 97)])
+def test_advect_bit_exact(gpu, name, steps, stride):
+    w = configs.WORKLOADS[name]()
+    f = w.velocity_field()
+    lon, lat = w.particles()
+    lon, lat = lon[::stride], lat[::stride]
+    g = gpu_advect(f, lon, lat, w.dt, steps)
+    o = oracle_advect(f, lon, lat, w.dt, steps)
+    assert g[4] == 0 and o[4] == 0
+    err = max(np.max(np.abs(g[0] - o[0])), np.max(np.abs(g[1] - o[1])))
+    assert err <= 1e-9, err  # stated tolerance; the operation order is shared, so expect 0
+    assert err == 0.0, f"not bit-exact: {err}"
+    assert np.array_equal(g[2], o[2]) and np.array_equal(g[3], o[3])
+
+
+def test_advect_random_walk_and_step_offset(gpu):
+    w = configs.rps_10k_steady()
+    f = w.velocity_field()
+    lon, lat = w.particles()
+    g = gpu_advect(f, lon, lat, w.dt, 5, walk_amp=1e-3, key=(4, 0x6B68), step0=17)
+    o = oracle_advect(f, lon, lat, w.dt, 5, walk_amp=1e-3, key=(4, 0x6B68), step0=17)
+    assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1]) and np.array_equal(g[2], o[2])
+
+
+def test_advect_out_of_grid(gpu):
+    lons = np.arange(200.0, 220.01, 0.5)
+    lats = np.arange(20.0, 40.01, 0.5)
+    f = VelocityField(lons, lats, np.zeros((lats.size, lons.size)), np.ones((lats.size, lons.size)))
+    g = gpu_advect(f, np.array([210.0, 210.0]), np.array([30.0, 39.99]), 3600.0, 2)
+    assert g[4] == 2 and g[1][1] == 39.99
+
+
+def test_field_blend_bit_exact(gpu):
+    rng = np.random.default_rng(0)
+    levels = rng.normal(size=(5, 33, 47, 2))
+    lev0 = np.array([0, 1, 3, 3, 2], np.int32)
+    wts = rng.uniform(size=5)
+    out = torch.empty(5 * 33 * 47 * 2, dtype=torch.float64, device="cuda")
+    dev.field_blend_f64(T(levels), 33 * 47 * 2, T(lev0), T(wts), out)
+    assert np.array_equal(out.cpu().numpy().reshape(5, 33, 47, 2), oracle.field_blend_f64(levels, lev0, wts))
+
+
+# ---------------------------------------------------------------- pairs
+def gpu_pairs(x, y, r, p=2.0, bbox=None, max_radius=None):
+    x32, y32 = np.asarray(x, np.float32), np.asarray(y, np.float32)
+    if bbox is None:
+        bbox = (float(x32.min()), float(x32.max()), float(y32.min()), float(y32.max())) if x32.size else (0, 1, 0, 1)
+    ps = dev.PairSearch(max(x32.size, 1), bbox, max_radius or max(r, 1e-6))
+    pi, pj = ps.query_sync(T(x32), T(y32), r, p)
+    return pi.cpu().numpy(), pj.cpu().numpy()
+
+
+@pytest.mark.parametrize("p", [2.0, 1.0, np.inf, 3.0])
+@pytest.mark.parametrize("seed,n,spread", [(0, 5000, 0.3), (1, 20000, 0.5), (2, 3000, 0.02)])
+def test_pairs_match_bruteforce(gpu, seed, n, spread, p):
+    rng = np.random.default_rng(seed)
+    x = (210.0 + rng.uniform(0, spread, n)).astype(np.float32)
+    y = (30.0 + rng.uniform(0, spread, n)).astype(np.float32)
+    gi, gj = gpu_pairs(x, y, R, p)
+    bi, bj = oracle.pairs_bruteforce(x, y, R, p)
+    assert len(bi) > 0
+    assert np.array_equal(gi, bi) and np.array_equal(gj, bj)
+    if p != 3.0:
+        ki, kj = sorted_kdtree_pairs(x, y, R, p)
+        assert np.array_equal(gi, ki) and np.array_equal(gj, kj)
+
+
+def test_pairs_dense_cluster_selection_path(gpu):
+    # 400 points inside one radius: every particle has far more than 8 neighbours
+    rng = np.random.default_rng(3)
+    x = (210.0 + rng.uniform(0, R / 3, 400)).astype(np.float32)
+    y = (30.0 + rng.uniform(0, R / 3, 400)).astype(np.float32)
+    gi, gj = gpu_pairs(x, y, R)
+    assert len(gi) == 400 * 399 // 2
+    bi, bj = oracle.pairs_bruteforce(x, y, R)
+    assert np.array_equal(gi, bi) and np.array_equal(gj, bj)
+
+
+def test_pairs_ties_duplicates_and_outside_box(gpu):
+    r = 2.0 ** -7
+    g = np.arange(40, dtype=np.float64) * r
+    X, Y = np.meshgrid(210.0 + g, 30.0 + g, indexing="ij")
+    x, y = X.ravel().astype(np.float32), Y.ravel().astype(np.float32)
+    x = np.concatenate([x, x[:50], np.float32([150.0, 150.0, 260.0])])  # duplicates and far outliers
+    y = np.concatenate([y, y[:50], np.float32([10.0, 10.0, 60.0])])
+    gi, gj = gpu_pairs(x, y, r, bbox=(210.0, 210.3, 30.0, 30.3))
+    bi, bj = oracle.pairs_bruteforce(x, y, r)
+    assert np.array_equal(gi, bi) and np.array_equal(gj, bj)
+    assert (len(x) - 3, len(x) - 2) in set(zip(gi.tolist(), gj.tolist()))  # coincident outliers pair
+
+
+def test_pairs_empty_and_tiny(gpu):
+    for n in (1, 2):
+        x = np.full(n, 210.0, np.float32)
+        gi, gj = gpu_pairs(x, x, R, bbox=(209, 211, 209, 211))
+        assert len(gi) == n * (n - 1) // 2
+    ps = dev.PairSearch(16, (0, 1, 0, 1), R)
+    e = torch.empty(0, dtype=torch.float32, device="cuda")
+    pi, pj = ps.query_sync(e, e, R)
+    assert pi.numel() == 0
+
+
+def test_pairs_capacity_overflow_reports_count(gpu):
+    rng = np.random.default_rng(4)
+    x = T((210.0 + rng.uniform(0, R, 300)).astype(np.float32))
+    y = T((30.0 + rng.uniform(0, R, 300)).astype(np.float32))
+    ps = dev.PairSearch(300, (210, 210.01, 30, 30.01), R)
+    pi = torch.full((10,), -7, dtype=torch.int32, device="cuda")
+    pj = torch.full((10,), -7, dtype=torch.int32, device="cuda")
+    npairs = torch.zeros(1, dtype=torch.int64, device="cuda")
+    ps.query(x, y, R, 2.0, pi, pj, npairs)
+    bi, _ = oracle.pairs_bruteforce(x.cpu().numpy(), y.cpu().numpy(), R)
+    assert int(npairs.item()) == len(bi) > 10
+    assert torch.all(pi == -7)  # nothing written past the capacity check
+
+
+# ---------------------------------------------------------------- rps
+def random_pairs(rng, n, m, clustered=False):
+    pairs = set()
+    while len(pairs) < m:
+        if clustered:
+            a = int(rng.integers(0, n))
+            b = int(min(n - 1, a + rng.integers(1, 4)))
+        else:
+            a, b = (int(v) for v in rng.integers(0, n, 2))
+        if a != b:
+            pairs.add((min(a, b), max(a, b)))
+    arr = np.array(sorted(pairs), dtype=np.int32)
+    return arr[:, 0].copy(), arr[:, 1].copy()
+
+
+def gpu_rps(species, pi, pj, pRS, pPR, pSP, key, step, updater=None):
+    n = species.size
+    up = updater or dev.RPSUpdater(n)
+    sp = T(species.astype(np.int8))
+    npairs = torch.tensor([len(pi)], dtype=torch.int64, device="cuda")
+    gi = T(pi.astype(np.int32)) if len(pi) else torch.zeros(1, dtype=torch.int32, device="cuda")
+    gj = T(pj.astype(np.int32)) if len(pj) else torch.zeros(1, dtype=torch.int32, device="cuda")
+    up.update(sp, gi, gj, npairs, pRS, pPR, pSP, key, step)
+    return sp.cpu().numpy(), up
+
+
+@pytest.mark.parametrize("seed,n,m,clustered", [(0, 1000, 800, False), (1, 20000, 30000, False),
+                                                (2, 5000, 12000, True), (3, 300, 4000, False)])
+def test_rps_matches_sequential_oracle(gpu, seed, n, m, clustered):
+    rng = np.random.default_rng(seed)
+    pi, pj = random_pairs(rng, n, m, clustered)
+    sp = rng.integers(0, 5, n).astype(np.int8)
+    args = (0.55, 0.45, 0.65, (seed, 3), 7 + seed)
+    g, up = gpu_rps(sp, pi, pj, *args)
+    o = oracle.rps_sequential(sp, pi, pj, *args)
+    assert np.array_equal(g, o)
+    comps, max_pairs, max_heads, total = up.component_stats()
+    assert total == m and comps >= 1
+
+
+def test_rps_giant_chain_component(gpu):
+    # one component of 20,000 pairs: a chain plus random chords
+    n = 10000
+    rng = np.random.default_rng(9)
+    pairs = {(k, k + 1) for k in range(n - 1)}
+    while len(pairs) < 20000:
+        a, b = sorted(int(v) for v in rng.integers(0, n, 2))
+        if a != b:
+            pairs.add((a, b))
+    arr = np.array(sorted(pairs), dtype=np.int32)
+    sp = rng.integers(1, 4, n).astype(np.int8)
+    g, up = gpu_rps(sp, arr[:, 0], arr[:, 1], 0.5, 0.5, 0.5, (1, 1), 3)
+    o = oracle.rps_sequential(sp, arr[:, 0], arr[:, 1], 0.5, 0.5, 0.5, (1, 1), 3)
+    assert np.array_equal(g, o)
+    assert up.component_stats()[1] == 20000
+
+
+def test_rps_empty_and_overflow(gpu):
+    sp = np.array([1, 2, 3, 1], np.int8)
+    g, up = gpu_rps(sp, np.zeros(0, np.int32), np.zeros(0, np.int32), 0.5, 0.5, 0.5, (0, 0), 0)
+    assert np.array_equal(g, sp)
+    # npairs beyond the capacity: no change
+    s = T(sp)
+    pi = T(np.array([0], np.int32)); pj = T(np.array([1], np.int32))
+    up.update(s, pi, pj, torch.tensor([5], dtype=torch.int64, device="cuda"), 1, 1, 1, (0, 0), 0)
+    assert np.array_equal(s.cpu().numpy(), sp)
+
+
+def test_rps_workspace_reuse(gpu):
+    rng = np.random.default_rng(11)
+    up = dev.RPSUpdater(5000)
+    for step in range(4):
+        n = [5000, 1200, 5000, 17][step]
+        pi, pj = random_pairs(rng, n, min(3 * n, n * (n - 1) // 2 - 1), clustered=step % 2 == 0)
+        sp = rng.integers(1, 4, n).astype(np.int8)
+        g, _ = gpu_rps(sp, pi, pj, 0.6, 0.5, 0.4, (2, 2), step, updater=up)
+        assert np.array_equal(g, oracle.rps_sequential(sp, pi, pj, 0.6, 0.5, 0.4, (2, 2), step))
