@@ -156,6 +156,9 @@ def test_pairs_capacity_overflow_reports_count(gpu):
 
 # ---------------------------------------------------------------- rps
 def random_pairs(rng, n, m, clustered=False):
+    # clustered pairs (a, a+1..a+3) exist only 3n-6 times: ask for fewer
+    limit = 3 * n - 6 if clustered else n * (n - 1) // 2
+    assert m <= limit, f"{m} distinct pairs requested, only {limit} possible"
     pairs = set()
     while len(pairs) < m:
         if clustered:
@@ -227,7 +230,8 @@ def test_rps_workspace_reuse(gpu):
     up = dev.RPSUpdater(5000)
     for step in range(4):
         n = [5000, 1200, 5000, 17][step]
-        pi, pj = random_pairs(rng, n, min(3 * n, n * (n - 1) // 2 - 1), clustered=step % 2 == 0)
+        clustered = step % 2 == 0
+        pi, pj = random_pairs(rng, n, 2 * n if clustered else min(3 * n, n * (n - 1) // 2), clustered=clustered)
         sp = rng.integers(1, 4, n).astype(np.int8)
         g, _ = gpu_rps(sp, pi, pj, 0.6, 0.5, 0.4, (2, 2), step, updater=up)
         assert np.array_equal(g, oracle.rps_sequential(sp, pi, pj, 0.6, 0.5, 0.4, (2, 2), step))
