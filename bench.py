@@ -38,7 +38,23 @@ def parse():
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel on BYTES bytes 10 times (byte-counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration(nbytes):
+    """Plain copy of a known size, to scale hardware byte counters."""
+    import torch
+    from lagrangian_microbes_amd import device as dev
+    nbytes = (nbytes // 16) * 16
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(10):
+        dev.copy_bytes(src, dst, nbytes)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "launches": 10,
+                      "read_bytes_per_launch": nbytes, "write_bytes_per_launch": nbytes}), flush=True)
 
 
 def run_steps(runner, k, total):
@@ -134,6 +150,9 @@ def main():
 
     _lib.load()
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     w = configs.WORKLOADS[args.workload]()
     lon, lat = w.particles()
     log(f"[bench] workload {w.name}: {w.n} particles, {w.steps} steps available; building runner")

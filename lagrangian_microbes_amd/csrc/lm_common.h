@@ -69,6 +69,7 @@ enum KernelId {
     K_RPS_SCATTER,
     K_RPS_PLAY,
     K_RPS_PLAY_WIDE,
+    K_RPS_PLAY_LARGE,
     K_PHILOX,
     K_COPY,
     K_NUM_KERNELS

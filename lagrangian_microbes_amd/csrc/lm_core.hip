@@ -25,7 +25,7 @@ void set_error(const char* fmt, ...) {
 static const char* const kKernelNames[K_NUM_KERNELS] = {
     "advect_rk4_f64", "field_blend_f64", "bin_count", "bin_scatter", "pair_count", "pair_emit",
     "scan_reduce",    "scan_apply",      "rps_init",  "rps_union",   "rps_heads",  "rps_scatter",
-    "rps_play",       "rps_play_wide",   "philox",    "copy16"};
+    "rps_play",       "rps_play_wide",   "rps_play_large", "philox", "copy16"};
 
 struct TimedLaunch {
     int kid;

@@ -27,7 +27,7 @@
 
 namespace lm {
 
-constexpr int kNbr = 8;  // neighbours kept per particle before the selection path
+constexpr int kNbr = 16;  // neighbours kept per particle before the selection path
 
 struct CellGrid {
     double x0, y0, inv_h;
@@ -144,6 +144,30 @@ pair_count_kernel(int64_t n, CellGrid g, Metric m, const uint32_t* __restrict__ 
     nbr_cnt[i] = cnt;
 }
 
+// Sort the first c (<= K) entries of a scratch row in registers and write
+// them as row o.. of the pair list.
+template <int K>
+__device__ __forceinline__ void emit_sorted(const uint32_t* row, uint32_t c, uint32_t i, uint32_t o,
+                                            int32_t* __restrict__ pair_i, int32_t* __restrict__ pair_j) {
+    uint32_t v[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) v[t] = (uint32_t)t < c ? row[t] : 0xffffffffu;
+#pragma unroll
+    for (int a = 1; a < K; ++a)
+#pragma unroll
+        for (int b = a; b > 0; --b) {
+            const uint32_t lo = min(v[b - 1], v[b]), hi = max(v[b - 1], v[b]);
+            v[b - 1] = lo;
+            v[b] = hi;
+        }
+#pragma unroll
+    for (int t = 0; t < K; ++t)
+        if ((uint32_t)t < c) {
+            pair_i[o + t] = (int32_t)i;
+            pair_j[o + t] = (int32_t)v[t];
+        }
+}
+
 __global__ void __launch_bounds__(kBlock)
 pair_emit_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y, CellGrid g,
                  Metric m, const uint32_t* __restrict__ start, const float4* __restrict__ recs,
@@ -157,43 +181,51 @@ pair_emit_kernel(int64_t n, const float* __restrict__ x, const float* __restrict
     const uint32_t c = nbr_cnt[i];
     if (c == 0) return;
     const uint32_t o = row_off[i];
-    if (c <= (uint32_t)kNbr) {
-        uint32_t v[kNbr];
-        const uint32_t* row = nbr + (size_t)i * kNbr;
-#pragma unroll
-        for (int t = 0; t < kNbr; ++t) v[t] = (uint32_t)t < c ? row[t] : 0xffffffffu;
-        // insertion network on a fixed-size register array
-#pragma unroll
-        for (int a = 1; a < kNbr; ++a)
-#pragma unroll
-            for (int b = a; b > 0; --b) {
-                const uint32_t lo = min(v[b - 1], v[b]), hi = max(v[b - 1], v[b]);
-                v[b - 1] = lo;
-                v[b] = hi;
-            }
-#pragma unroll
-        for (int t = 0; t < kNbr; ++t)
-            if ((uint32_t)t < c) {
-                pair_i[o + t] = (int32_t)i;
-                pair_j[o + t] = (int32_t)v[t];
-            }
+    const uint32_t* row = nbr + (size_t)i * kNbr;
+    if (c <= 2) {
+        const uint32_t a = row[0], b = c == 2 ? row[1] : 0xffffffffu;
+        pair_i[o] = (int32_t)i;
+        pair_j[o] = (int32_t)min(a, b);
+        if (c == 2) {
+            pair_i[o + 1] = (int32_t)i;
+            pair_j[o + 1] = (int32_t)max(a, b);
+        }
         return;
     }
-    // Dense neighbourhood: emit in ascending order by repeated selection.
+    if (c <= 4) return emit_sorted<4>(row, c, i, o, pair_i, pair_j);
+    if (c <= 8) return emit_sorted<8>(row, c, i, o, pair_i, pair_j);
+    if (c <= (uint32_t)kNbr) return emit_sorted<kNbr>(row, c, i, o, pair_i, pair_j);
+    // Dense neighbourhood: emit in ascending order, kNbr at a time, each pass
+    // keeping the kNbr smallest partners above the last one written.
     const float xf = x[i], yf = y[i];
     int cx, cy;
     cell_of(g, xf, yf, &cx, &cy);
     uint32_t last = i;
-    for (uint32_t t = 0; t < c; ++t) {
-        uint32_t best = 0xffffffffu;
+    for (uint32_t t = 0; t < c;) {
+        uint32_t v[kNbr];
+#pragma unroll
+        for (int q = 0; q < kNbr; ++q) v[q] = 0xffffffffu;
         for_stencil(g, start, (uint32_t)n, cx, cy, [&](uint32_t s) {
             const float4 rec = recs[s];
-            const uint32_t j = __float_as_uint(rec.z);
-            if (j > last && j < best && within(m, xf, yf, rec.x, rec.y)) best = j;
+            uint32_t j = __float_as_uint(rec.z);
+            if (j > last && j < v[kNbr - 1] && within(m, xf, yf, rec.x, rec.y)) {
+#pragma unroll
+                for (int q = 0; q < kNbr; ++q) {  // insert into the sorted register array
+                    const uint32_t lo = min(v[q], j), hi = max(v[q], j);
+                    v[q] = lo;
+                    j = hi;
+                }
+            }
         });
-        pair_i[o + t] = (int32_t)i;
-        pair_j[o + t] = (int32_t)best;
-        last = best;
+        const uint32_t take = min((uint32_t)kNbr, c - t);
+#pragma unroll
+        for (int q = 0; q < kNbr; ++q)
+            if ((uint32_t)q < take) {
+                pair_i[o + t + q] = (int32_t)i;
+                pair_j[o + t + q] = (int32_t)v[q];
+                last = v[q];
+            }
+        t += take;
     }
 }
 

@@ -214,6 +214,37 @@ def test_rps_giant_chain_component(gpu):
     assert up.component_stats()[1] == 20000
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+def test_rps_every_replay_tier(gpu, seed):
+    """Components of controlled sizes, indices interleaved across components:
+    narrow (<= 8 heads), mid (<= 128 heads, <= 512 pairs), mid spilled to large
+    (<= 128 heads, > 512 pairs), large (<= 2048 heads) and the serial fallback."""
+    rng = np.random.default_rng(seed)
+    sizes = [2, 3, 5, 9] * 50 + [12, 40, 100] * 10 + [60] * 3 + [400, 1500] + [3000]
+    n = sum(sizes)
+    perm = rng.permutation(n)
+    pairs = set()
+    at = 0
+    for s in sizes:
+        ids = perm[at:at + s]
+        at += s
+        for a in range(1, s):  # spanning tree keeps the component connected
+            b = int(rng.integers(0, a))
+            pairs.add((int(min(ids[a], ids[b])), int(max(ids[a], ids[b]))))
+        extra = 12 * s if s == 60 else s  # dense 60-particle groups exceed 512 pairs
+        for _ in range(extra):
+            a, b = (int(v) for v in rng.choice(ids, 2, replace=False))
+            pairs.add((min(a, b), max(a, b)))
+    arr = np.array(sorted(pairs), dtype=np.int32)
+    sp = rng.integers(1, 4, n).astype(np.int8)
+    args = (0.5, 0.6, 0.4, (seed, 9), 21)
+    g, up = gpu_rps(sp, arr[:, 0], arr[:, 1], *args)
+    o = oracle.rps_sequential(sp, arr[:, 0], arr[:, 1], *args)
+    assert np.array_equal(g, o)
+    comps, max_pairs, max_heads, total = up.component_stats()
+    assert comps == len(sizes) and total == len(arr)
+
+
 def test_rps_empty_and_overflow(gpu):
     sp = np.array([1, 2, 3, 1], np.int8)
     g, up = gpu_rps(sp, np.zeros(0, np.int32), np.zeros(0, np.int32), 0.5, 0.5, 0.5, (0, 0), 0)
