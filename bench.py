@@ -68,25 +68,40 @@ def run_steps(runner, k, total):
 
 
 def algorithmic_bytes(kernel, n, pairs, chunk, T, field_len):
-    """Compulsory bytes one launch of `kernel` must move."""
-    P = pairs
-    table = {
-        # lon/lat f64 read + write once per launch, float32 records per step,
-        # the 2*chunk+1 blended stage fields read once
+    """Compulsory bytes one launch of `kernel` must move.
+
+    Per step: n particles, P pairs, T cell buckets, U = min(n, 2P) particles
+    with a pair.  Batched kernels cover `chunk` steps per launch; the replay
+    kernels (rps_play*) run once per step.  The advection launch reads and
+    writes the float64 state once and writes float32 records every step.
+    """
+    P, U = pairs, min(n, 2 * pairs)
+    per_step = {
+        "bin_count": 8 * n + 8 * min(n, T),                     # positions, bucket counters
+        "bin_scatter": 8 * n + 8 * min(n, T) + 16 * n,          # positions, starts, records
+        "pair_count": 16 * n + 4 * n + 4 * P,                   # records, counts, neighbour rows
+        "pair_emit": 8 * n + 4 * P + 8 * P,                     # counts+offsets, rows, pair list
+        "rps_init": 8 * n,                                      # parent, slot
+        "rps_union": 8 * P + P + 8 * U,                         # pairs, outcome bits, parents
+        "rps_claim": 8 * P + 24 * U,                            # pairs, rows, slots, labels, counts
+        "rps_group": 4 * P + 20 * U,                            # heads, ids, offsets
+        "rps_order": 20 * U,                                    # heads, rows, row offsets
+        "rps_fill": 8 * P + P + 12 * U + 8 * P,                 # pairs, bits, lookups, records
+        "rps_deps": 16 * P,                                     # records in, masks out
+    }
+    per_launch = {
         "advect_rk4_f64": 32 * n + 8 * n * chunk + 8 * field_len * (2 * chunk + 1),
         "field_blend_f64": 24 * field_len * (2 * chunk + 1),
-        "bin_count": 8 * n + 8 * min(n, T),
-        "bin_scatter": 8 * n + 8 * min(n, T) + 16 * n,
-        # records read once, neighbour counts and rows written
-        "pair_count": 16 * n + 4 * n + 4 * P,
-        "pair_emit": 4 * n + 4 * n + 4 * P + 8 * P,
-        "rps_union": 8 * P + 8 * min(n, 2 * P),
-        "rps_heads": 4 * P + 16 * min(n, P),
-        "rps_scatter": 4 * P + 16 * min(n, P),
-        "rps_play": 8 * n + 4 * P + 12 * min(n, P) + 2 * min(n, 2 * P),
-        "rps_init": 4 * n,
+        # untouched particles copied (slot check + species), components: ids,
+        # species in and out, pair records
+        "rps_play": 4 * n + 2 * (n - U) + 6 * U + 8 * P,
+        "rps_play_wide": 6 * U + 16 * P,
     }
-    return table.get(kernel)
+    if kernel in per_launch:
+        return per_launch[kernel]
+    if kernel in per_step:
+        return per_step[kernel] * chunk
+    return None
 
 
 def cpu_baseline(w, runner, steps):

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define LM_ABI_VERSION 1
+#define LM_ABI_VERSION 2
 
 #define LM_OK 0
 #define LM_ERR_ARG (-1)      /* invalid argument                             */
@@ -91,12 +91,13 @@ int lm_advect_rk4_f64(int64_t n, double* lon, double* lat,
 
 typedef struct lm_pairs_ws lm_pairs_ws;
 
-/* Workspace for up to n_max particles expected inside the lon/lat box
- * [lon_min, lon_max] x [lat_min, lat_max] (particles outside are still
- * handled exactly, only more slowly), for radii up to max_radius (deg).
- * Allocates device memory on the current device. */
-int lm_pairs_ws_create(int64_t n_max, double lon_min, double lon_max, double lat_min,
-                       double lat_max, double max_radius, lm_pairs_ws** out);
+/* Workspace for up to n_max particles and up to steps_max position snapshots
+ * per call, expected inside the lon/lat box [lon_min, lon_max] x
+ * [lat_min, lat_max] (particles outside are still handled exactly, only more
+ * slowly), for radii up to max_radius (deg).  Allocates device memory on the
+ * current device (about 80 bytes per particle and snapshot). */
+int lm_pairs_ws_create(int64_t n_max, int32_t steps_max, double lon_min, double lon_max,
+                       double lat_min, double lat_max, double max_radius, lm_pairs_ws** out);
 void lm_pairs_ws_destroy(lm_pairs_ws* ws);
 /* Device bytes held by the workspace. */
 int64_t lm_pairs_ws_bytes(const lm_pairs_ws* ws);
@@ -116,13 +117,24 @@ int lm_query_pairs(lm_pairs_ws* ws, int64_t n, const float* lon, const float* la
                    double radius, double p, int32_t* pair_i, int32_t* pair_j,
                    int64_t capacity, int64_t* npairs, void* stream);
 
+/* The same for nsteps snapshots at once: snapshot s is lon/lat + s*pos_stride,
+ * its pairs go to pair_i/pair_j + s*pair_stride (capacity pair_stride) and
+ * its count to npairs[s].  One launch per phase covers every snapshot.
+ * Replaces the per-step query of interaction_simulator.py:88-98 over a run of
+ * steps. */
+int lm_query_pairs_batch(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float* lon,
+                         const float* lat, int64_t pos_stride, double radius, double p,
+                         int32_t* pair_i, int32_t* pair_j, int64_t pair_stride, int64_t* npairs,
+                         void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Rock-paper-scissors update                                               */
 /* ------------------------------------------------------------------------ */
 
 typedef struct lm_rps_ws lm_rps_ws;
 
-int lm_rps_ws_create(int64_t n_max, lm_rps_ws** out);
+/* Workspace for up to n_max particles and steps_max steps per prepare. */
+int lm_rps_ws_create(int64_t n_max, int32_t steps_max, lm_rps_ws** out);
 void lm_rps_ws_destroy(lm_rps_ws* ws);
 int64_t lm_rps_ws_bytes(const lm_rps_ws* ws);
 
@@ -142,11 +154,29 @@ int lm_rps_update(lm_rps_ws* ws, int64_t n, int8_t* species, const int32_t* pair
                   double pRS, double pPR, double pSP, uint64_t key0, uint64_t key1,
                   uint64_t step, void* stream);
 
-/* Size statistics of the interaction components of the last lm_rps_update on
- * this workspace, written to device int64[4]: number of components with at
- * least one pair, largest component (pairs), largest component (row heads),
- * total pairs.  Diagnostic only. */
-int lm_rps_component_stats(lm_rps_ws* ws, int64_t* out4, void* stream);
+/* The same rule over nsteps consecutive steps, split into the part that only
+ * depends on the pairs and the part that needs the species:
+ *   lm_rps_prepare  for pair lists pair_i/pair_j + s*pair_stride with counts
+ *                   npairs[s] (s < nsteps) at global steps step0 + s: groups
+ *                   the pairs by connected component in the order they are to
+ *                   be visited and draws their uniforms -- all steps at once;
+ *   lm_rps_play     step s of the last prepare: species_out = species_in after
+ *                   that step's pairs (species_in == species_out is allowed).
+ * lm_rps_update(.., step) is lm_rps_prepare(nsteps = 1, step0 = step) followed
+ * by lm_rps_play(0, species, species).  A step whose count exceeds
+ * pair_stride is left unchanged by play. */
+int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair_i,
+                   const int32_t* pair_j, int64_t pair_stride, const int64_t* npairs,
+                   double pRS, double pPR, double pSP, uint64_t key0, uint64_t key1,
+                   uint64_t step0, void* stream);
+int lm_rps_play(lm_rps_ws* ws, int32_t step, int64_t n, const int8_t* species_in,
+                int8_t* species_out, void* stream);
+
+/* Size statistics of the interaction components of step `step` of the last
+ * prepare, written to device int64[4]: components with at least one pair,
+ * largest component (pairs), largest component (particles), total pairs.
+ * Diagnostic only. */
+int lm_rps_component_stats(lm_rps_ws* ws, int32_t step, int64_t* out4, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Counter-based uniforms                                                   */

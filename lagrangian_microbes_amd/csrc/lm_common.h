@@ -70,6 +70,9 @@ enum KernelId {
     K_RPS_PLAY,
     K_RPS_PLAY_WIDE,
     K_RPS_PLAY_LARGE,
+    K_RPS_ORDER,
+    K_RPS_FILL,
+    K_RPS_DEPS,
     K_PHILOX,
     K_COPY,
     K_NUM_KERNELS

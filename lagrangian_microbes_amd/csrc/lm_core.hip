@@ -24,8 +24,9 @@ void set_error(const char* fmt, ...) {
 // ---------------------------------------------------------------------------
 static const char* const kKernelNames[K_NUM_KERNELS] = {
     "advect_rk4_f64", "field_blend_f64", "bin_count", "bin_scatter", "pair_count", "pair_emit",
-    "scan_reduce",    "scan_apply",      "rps_init",  "rps_union",   "rps_heads",  "rps_scatter",
-    "rps_play",       "rps_play_wide",   "rps_play_large", "philox", "copy16"};
+    "scan_reduce",    "scan_apply",      "rps_init",  "rps_union",   "rps_claim",  "rps_group",
+    "rps_play",       "rps_play_wide",   "rps_play_large", "rps_order", "rps_fill", "rps_deps", "philox",
+    "copy16"};
 
 struct TimedLaunch {
     int kid;
@@ -60,9 +61,13 @@ void timing_events(int kid, hipEvent_t* start, hipEvent_t* stop) {
 // ---------------------------------------------------------------------------
 // scan
 // ---------------------------------------------------------------------------
+// Segmented: segment g = blockIdx.y scans in[g*in_stride .. + n) into
+// out[g*out_stride .. + n], out[g*out_stride + n] = total.
 __global__ void __launch_bounds__(kBlock)
-scan_reduce_kernel(const uint32_t* __restrict__ in, int64_t n, uint32_t* __restrict__ block_sums) {
+scan_reduce_kernel(const uint32_t* __restrict__ in, int64_t n, int64_t in_stride,
+                   uint32_t* __restrict__ block_sums) {
     __shared__ uint32_t lds[kBlock / 64];
+    in += (int64_t)blockIdx.y * in_stride;
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
     uint32_t s = 0;
     if (base + kScanItems <= n) {
@@ -74,13 +79,17 @@ scan_reduce_kernel(const uint32_t* __restrict__ in, int64_t n, uint32_t* __restr
             if (base + k < n) s += in[base + k];
     }
     s = block_sum(s, lds);
-    if (threadIdx.x == 0) block_sums[blockIdx.x] = s;
+    if (threadIdx.x == 0) block_sums[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = s;
 }
 
 __global__ void __launch_bounds__(kBlock)
-scan_apply_kernel(const uint32_t* __restrict__ in, int64_t n, const uint32_t* __restrict__ block_sums,
-                  uint32_t* __restrict__ out, int64_t* __restrict__ total64) {
+scan_apply_kernel(const uint32_t* __restrict__ in, int64_t n, int64_t in_stride,
+                  const uint32_t* __restrict__ block_sums, uint32_t* __restrict__ out, int64_t out_stride,
+                  int64_t* __restrict__ total64) {
     __shared__ uint32_t lds[kBlock / 64];
+    in += (int64_t)blockIdx.y * in_stride;
+    out += (int64_t)blockIdx.y * out_stride;
+    block_sums += (int64_t)blockIdx.y * gridDim.x;
     // offset of this tile = sum of the preceding tiles' sums
     uint32_t pre = 0;
     for (unsigned b = threadIdx.x; b < blockIdx.x; b += kBlock) pre += block_sums[b];
@@ -107,8 +116,8 @@ scan_apply_kernel(const uint32_t* __restrict__ in, int64_t n, const uint32_t* __
 #pragma unroll
         for (int k = 0; k < kScanItems; ++k) { o[k] = run; run += v[k]; }
         uint4* q = reinterpret_cast<uint4*>(out + base);
-        // out is n+1 long and tiles start at multiples of kScanTile, so the
-        // 16-byte stores are aligned whenever out is.
+        // tiles start at multiples of kScanTile and strides are multiples of
+        // 4, so the 16-byte stores are aligned whenever out is
         q[0] = make_uint4(o[0], o[1], o[2], o[3]);
         q[1] = make_uint4(o[4], o[5], o[6], o[7]);
     } else {
@@ -121,21 +130,31 @@ scan_apply_kernel(const uint32_t* __restrict__ in, int64_t n, const uint32_t* __
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
         const uint32_t total = pre + tile_total;
         out[n] = total;
-        if (total64) *total64 = (int64_t)total;
+        if (total64) total64[blockIdx.y] = (int64_t)total;
     }
+}
+
+int exclusive_scan_u32_batched(const uint32_t* in, int64_t n, int64_t in_stride, uint32_t* out, int64_t out_stride,
+                               int32_t nseg, uint32_t* block_sums, int64_t* total64, hipStream_t stream) {
+    const int64_t nb = scan_num_blocks(n);
+    LM_REQUIRE(((uintptr_t)in & 15) == 0 && ((uintptr_t)out & 15) == 0 && (in_stride % 4) == 0 &&
+                   (out_stride % 4) == 0,
+               LM_ERR_ARG, "scan buffers must be 16-byte aligned");
+    LM_REQUIRE(nseg >= 1 && nseg <= 65535, LM_ERR_ARG, "1..65535 scan segments");
+    LM_REQUIRE(out_stride >= n + 1, LM_ERR_ARG, "scan output stride too small");
+    LM_LAUNCH(K_SCAN_REDUCE, scan_reduce_kernel, dim3((unsigned)nb, (unsigned)nseg), dim3(kBlock), 0, stream, in, n,
+              in_stride, block_sums);
+    LM_LAUNCH_CHECK();
+    LM_LAUNCH(K_SCAN_APPLY, scan_apply_kernel, dim3((unsigned)nb, (unsigned)nseg), dim3(kBlock), 0, stream, in, n,
+              in_stride, block_sums, out, out_stride, total64);
+    LM_LAUNCH_CHECK();
+    return LM_OK;
 }
 
 int exclusive_scan_u32(const uint32_t* in, int64_t n, uint32_t* out, uint32_t* block_sums,
                        int64_t* total64, hipStream_t stream) {
-    const int64_t nb = scan_num_blocks(n);
-    LM_REQUIRE(((uintptr_t)in & 15) == 0 && ((uintptr_t)out & 15) == 0, LM_ERR_ARG,
-               "scan buffers must be 16-byte aligned");
-    LM_LAUNCH(K_SCAN_REDUCE, scan_reduce_kernel, dim3((unsigned)nb), dim3(kBlock), 0, stream, in, n, block_sums);
-    LM_LAUNCH_CHECK();
-    LM_LAUNCH(K_SCAN_APPLY, scan_apply_kernel, dim3((unsigned)nb), dim3(kBlock), 0, stream, in, n, block_sums, out,
-              total64);
-    LM_LAUNCH_CHECK();
-    return LM_OK;
+    const int64_t stride = (n + 4) & ~(int64_t)3;
+    return exclusive_scan_u32_batched(in, n, stride, out, stride, 1, block_sums, total64, stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,4 +1,4 @@
-// Fixed-radius pair search in lon/lat degree space.
+// Fixed-radius pair search in lon/lat degree space, for one or many steps.
 //
 // Cell list with cells of side h >= radius over a fixed box, stored as a
 // counting sort into T = 2^k buckets (bucket = dense cell id mod T, so the
@@ -13,9 +13,12 @@
 //   5. scan         : row offsets and the pair count P
 //   6. pair_emit    : per particle, its neighbours sorted ascending, written at
 //                     its row offset; rows with more than kNbr neighbours are
-//                     re-searched by selection.  The result is the (i, j)
+//                     re-searched kNbr at a time.  The result is the (i, j)
 //                     lexicographic pair list, independent of the atomic order
 //                     inside a bucket.
+// A batch of S steps (S position snapshots) runs every phase once with
+// blockIdx.y = step, each step with its own tables and pair region, so a
+// launch carries S times the parallelism of one step.
 // Distances use the float32 positions widened to double, with the same
 // operations cKDTree applies: dx*dx + dy*dy <= r*r for p = 2.  Since a
 // difference of two nearby floats is exact in double and its square is exact,
@@ -27,7 +30,7 @@
 
 namespace lm {
 
-constexpr int kNbr = 16;  // neighbours kept per particle before the selection path
+constexpr int kNbr = 8;  // neighbours kept per particle before the selection path
 
 struct CellGrid {
     double x0, y0, inv_h;
@@ -43,6 +46,16 @@ struct Metric {
     double r;    // radius
     double r2;   // radius*radius (p = 2) or radius^p (general p)
     double p;
+};
+
+// Per-step strides of a batch (elements).
+struct Strides {
+    int64_t pos;    // between position snapshots
+    int64_t tab;    // bucket tables (>= T + 1, multiple of 4)
+    int64_t rec;    // bucket-ordered records
+    int64_t row;    // per-particle counts / offsets (>= n + 1, multiple of 4)
+    int64_t nbr;    // neighbour scratch rows
+    int64_t pair;   // pair list capacity per step
 };
 
 __device__ __forceinline__ void cell_of(const CellGrid& g, float xf, float yf, int* cx, int* cy) {
@@ -97,51 +110,55 @@ __device__ __forceinline__ void for_stencil(const CellGrid& g, const uint32_t* _
 }
 
 __global__ void __launch_bounds__(kBlock)
-bin_count_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y, CellGrid g,
+bin_count_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y, CellGrid g, Strides st,
                  uint32_t* __restrict__ count) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
+    const int64_t s = blockIdx.y;
     int cx, cy;
-    cell_of(g, x[i], y[i], &cx, &cy);
-    atomicAdd(&count[bucket_of(g, cx, cy)], 1u);
+    cell_of(g, x[s * st.pos + i], y[s * st.pos + i], &cx, &cy);
+    atomicAdd(&count[s * st.tab + bucket_of(g, cx, cy)], 1u);
 }
 
 __global__ void __launch_bounds__(kBlock)
-bin_scatter_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y, CellGrid g,
-                   const uint32_t* __restrict__ start, uint32_t* __restrict__ count,
-                   float4* __restrict__ recs) {
+bin_scatter_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y, CellGrid g, Strides st,
+                   const uint32_t* __restrict__ start, uint32_t* __restrict__ count, float4* __restrict__ recs) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const float xf = x[i], yf = y[i];
+    const int64_t s = blockIdx.y;
+    const float xf = x[s * st.pos + i], yf = y[s * st.pos + i];
     int cx, cy;
     cell_of(g, xf, yf, &cx, &cy);
     const uint32_t b = bucket_of(g, cx, cy);
-    const uint32_t slot = start[b] + atomicSub(&count[b], 1u) - 1u;
-    if (slot < (uint64_t)n) recs[slot] = make_float4(xf, yf, __uint_as_float((uint32_t)i), 0.0f);
+    const uint32_t slot = start[s * st.tab + b] + atomicSub(&count[s * st.tab + b], 1u) - 1u;
+    if (slot < (uint64_t)n) recs[s * st.rec + slot] = make_float4(xf, yf, __uint_as_float((uint32_t)i), 0.0f);
 }
 
 __global__ void __launch_bounds__(kBlock)
-pair_count_kernel(int64_t n, CellGrid g, Metric m, const uint32_t* __restrict__ start,
-                  const float4* __restrict__ recs, uint32_t* __restrict__ nbr_cnt,
+pair_count_kernel(int64_t n, CellGrid g, Metric m, Strides st, const uint32_t* __restrict__ start_all,
+                  const float4* __restrict__ recs_all, uint32_t* __restrict__ nbr_cnt,
                   uint32_t* __restrict__ nbr) {
     const int64_t s0 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (s0 >= n) return;
+    const int64_t s = blockIdx.y;
+    const uint32_t* start = start_all + s * st.tab;
+    const float4* recs = recs_all + s * st.rec;
     const float4 me = recs[s0];
     const uint32_t i = __float_as_uint(me.z);
     if (i >= (uint64_t)n) return;
     int cx, cy;
     cell_of(g, me.x, me.y, &cx, &cy);
     uint32_t cnt = 0;
-    uint32_t* row = nbr + (size_t)i * kNbr;
-    for_stencil(g, start, (uint32_t)n, cx, cy, [&](uint32_t s) {
-        const float4 o = recs[s];
+    uint32_t* row = nbr + s * st.nbr + (int64_t)i * kNbr;
+    for_stencil(g, start, (uint32_t)n, cx, cy, [&](uint32_t q) {
+        const float4 o = recs[q];
         const uint32_t j = __float_as_uint(o.z);
         if (j > i && within(m, me.x, me.y, o.x, o.y)) {
             if (cnt < kNbr) row[cnt] = j;
             ++cnt;
         }
     });
-    nbr_cnt[i] = cnt;
+    nbr_cnt[s * st.row + i] = cnt;
 }
 
 // Sort the first c (<= K) entries of a scratch row in registers and write
@@ -169,19 +186,23 @@ __device__ __forceinline__ void emit_sorted(const uint32_t* row, uint32_t c, uin
 }
 
 __global__ void __launch_bounds__(kBlock)
-pair_emit_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y, CellGrid g,
-                 Metric m, const uint32_t* __restrict__ start, const float4* __restrict__ recs,
-                 const uint32_t* __restrict__ nbr_cnt, const uint32_t* __restrict__ nbr,
-                 const uint32_t* __restrict__ row_off, int64_t capacity,
-                 int32_t* __restrict__ pair_i, int32_t* __restrict__ pair_j) {
+pair_emit_kernel(int64_t n, const float* __restrict__ x_all, const float* __restrict__ y_all, CellGrid g,
+                 Metric m, Strides st, const uint32_t* __restrict__ start_all, const float4* __restrict__ recs_all,
+                 const uint32_t* __restrict__ nbr_cnt_all, const uint32_t* __restrict__ nbr_all,
+                 const uint32_t* __restrict__ row_off_all, int32_t* __restrict__ pair_i_all,
+                 int32_t* __restrict__ pair_j_all) {
     const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i64 >= n) return;
-    if ((int64_t)row_off[n] > capacity) return;  // caller enlarges and calls again
+    const int64_t s = blockIdx.y;
+    const uint32_t* row_off = row_off_all + s * st.row;
+    if ((int64_t)row_off[n] > st.pair) return;  // caller enlarges and calls again
     const uint32_t i = (uint32_t)i64;
-    const uint32_t c = nbr_cnt[i];
+    const uint32_t c = nbr_cnt_all[s * st.row + i];
     if (c == 0) return;
     const uint32_t o = row_off[i];
-    const uint32_t* row = nbr + (size_t)i * kNbr;
+    int32_t* pair_i = pair_i_all + s * st.pair;
+    int32_t* pair_j = pair_j_all + s * st.pair;
+    const uint32_t* row = nbr_all + s * st.nbr + (int64_t)i * kNbr;
     if (c <= 2) {
         const uint32_t a = row[0], b = c == 2 ? row[1] : 0xffffffffu;
         pair_i[o] = (int32_t)i;
@@ -193,33 +214,35 @@ pair_emit_kernel(int64_t n, const float* __restrict__ x, const float* __restrict
         return;
     }
     if (c <= 4) return emit_sorted<4>(row, c, i, o, pair_i, pair_j);
-    if (c <= 8) return emit_sorted<8>(row, c, i, o, pair_i, pair_j);
     if (c <= (uint32_t)kNbr) return emit_sorted<kNbr>(row, c, i, o, pair_i, pair_j);
-    // Dense neighbourhood: emit in ascending order, kNbr at a time, each pass
-    // keeping the kNbr smallest partners above the last one written.
-    const float xf = x[i], yf = y[i];
+    // Dense neighbourhood: emit in ascending order, 16 at a time, each pass
+    // keeping the 16 smallest partners above the last one written.
+    constexpr int kSel = 16;
+    const uint32_t* start = start_all + s * st.tab;
+    const float4* recs = recs_all + s * st.rec;
+    const float xf = x_all[s * st.pos + i], yf = y_all[s * st.pos + i];
     int cx, cy;
     cell_of(g, xf, yf, &cx, &cy);
     uint32_t last = i;
     for (uint32_t t = 0; t < c;) {
-        uint32_t v[kNbr];
+        uint32_t v[kSel];
 #pragma unroll
-        for (int q = 0; q < kNbr; ++q) v[q] = 0xffffffffu;
-        for_stencil(g, start, (uint32_t)n, cx, cy, [&](uint32_t s) {
-            const float4 rec = recs[s];
+        for (int q = 0; q < kSel; ++q) v[q] = 0xffffffffu;
+        for_stencil(g, start, (uint32_t)n, cx, cy, [&](uint32_t q0) {
+            const float4 rec = recs[q0];
             uint32_t j = __float_as_uint(rec.z);
-            if (j > last && j < v[kNbr - 1] && within(m, xf, yf, rec.x, rec.y)) {
+            if (j > last && j < v[kSel - 1] && within(m, xf, yf, rec.x, rec.y)) {
 #pragma unroll
-                for (int q = 0; q < kNbr; ++q) {  // insert into the sorted register array
+                for (int q = 0; q < kSel; ++q) {  // insert into the sorted register array
                     const uint32_t lo = min(v[q], j), hi = max(v[q], j);
                     v[q] = lo;
                     j = hi;
                 }
             }
         });
-        const uint32_t take = min((uint32_t)kNbr, c - t);
+        const uint32_t take = min((uint32_t)kSel, c - t);
 #pragma unroll
-        for (int q = 0; q < kNbr; ++q)
+        for (int q = 0; q < kSel; ++q)
             if ((uint32_t)q < take) {
                 pair_i[o + t + q] = (int32_t)i;
                 pair_j[o + t + q] = (int32_t)v[q];
@@ -229,21 +252,25 @@ pair_emit_kernel(int64_t n, const float* __restrict__ x, const float* __restrict
     }
 }
 
-__global__ void set_i64_kernel(int64_t* p, int64_t v) { *p = v; }
+__global__ void zero_i64_kernel(int64_t* p, int64_t count) {
+    for (int64_t k = threadIdx.x; k < count; k += blockDim.x) p[k] = 0;
+}
 
 }  // namespace lm
 
 struct lm_pairs_ws {
     int64_t n_max = 0;
+    int32_t steps_max = 0;
     double max_radius = 0.0;
     lm::CellGrid grid{};
-    uint32_t* count = nullptr;     // [T]       zero between calls
-    uint32_t* start = nullptr;     // [T + 1]
+    int64_t tab_stride = 0, row_stride = 0;
+    uint32_t* count = nullptr;     // [steps][tab]   zero between calls
+    uint32_t* start = nullptr;     // [steps][tab]
     uint32_t* scan_tmp = nullptr;  // block sums
-    float4* recs = nullptr;        // [n_max]
-    uint32_t* nbr_cnt = nullptr;   // [n_max]
-    uint32_t* nbr = nullptr;       // [n_max * kNbr]
-    uint32_t* row_off = nullptr;   // [n_max + 1]
+    float4* recs = nullptr;        // [steps][n_max]
+    uint32_t* nbr_cnt = nullptr;   // [steps][row]
+    uint32_t* nbr = nullptr;       // [steps][n_max * kNbr]
+    uint32_t* row_off = nullptr;   // [steps][row]
     int64_t bytes = 0;
     bool dirty = false;            // a call stopped midway: counts must be re-zeroed
 };
@@ -280,11 +307,12 @@ static uint32_t next_pow2_u32(uint64_t v) {
 
 extern "C" {
 
-int lm_pairs_ws_create(int64_t n_max, double lon_min, double lon_max, double lat_min, double lat_max,
-                       double max_radius, lm_pairs_ws** out) {
+int lm_pairs_ws_create(int64_t n_max, int32_t steps_max, double lon_min, double lon_max, double lat_min,
+                       double lat_max, double max_radius, lm_pairs_ws** out) {
     LM_REQUIRE(out != nullptr, LM_ERR_ARG, "out is NULL");
     *out = nullptr;
     LM_REQUIRE(n_max >= 0 && n_max < (int64_t)0x7fffffff, LM_ERR_ARG, "n_max out of range");
+    LM_REQUIRE(steps_max >= 1 && steps_max <= 65535, LM_ERR_ARG, "steps_max must be in 1..65535");
     LM_REQUIRE(std::isfinite(lon_min) && std::isfinite(lon_max) && std::isfinite(lat_min) &&
                    std::isfinite(lat_max) && lon_max >= lon_min && lat_max >= lat_min,
                LM_ERR_ARG, "invalid bounding box");
@@ -292,6 +320,7 @@ int lm_pairs_ws_create(int64_t n_max, double lon_min, double lon_max, double lat
 
     lm_pairs_ws* w = new lm_pairs_ws();
     w->n_max = n_max;
+    w->steps_max = steps_max;
     w->max_radius = max_radius;
     // cells strictly larger than the radius so rounding in the cell index can
     // never separate two points within the radius by more than one cell
@@ -313,20 +342,24 @@ int lm_pairs_ws_create(int64_t n_max, double lon_min, double lon_max, double lat
     if (T < 1024) T = 1024;
     g.T = T;
     g.mask = T - 1;
+    w->tab_stride = lm::scan_row_stride((int64_t)T);
+    w->row_stride = lm::scan_row_stride(n_max);
 
-    int rc = LM_OK;
+    const int64_t S = steps_max;
     int64_t nb = lm::scan_num_blocks((int64_t)T);
     const int64_t nb2 = lm::scan_num_blocks(n_max);
     if (nb2 > nb) nb = nb2;
-    if ((rc = ws_alloc(&w->count, T, &w->bytes)) || (rc = ws_alloc(&w->start, (int64_t)T + 4, &w->bytes)) ||
-        (rc = ws_alloc(&w->scan_tmp, nb, &w->bytes)) || (rc = ws_alloc(&w->recs, n_max, &w->bytes)) ||
-        (rc = ws_alloc(&w->nbr_cnt, n_max + 4, &w->bytes)) ||
-        (rc = ws_alloc(&w->nbr, n_max * lm::kNbr, &w->bytes)) ||
-        (rc = ws_alloc(&w->row_off, n_max + 4, &w->bytes))) {
+    int rc = LM_OK;
+    if ((rc = ws_alloc(&w->count, S * w->tab_stride, &w->bytes)) ||
+        (rc = ws_alloc(&w->start, S * w->tab_stride, &w->bytes)) ||
+        (rc = ws_alloc(&w->scan_tmp, S * nb, &w->bytes)) || (rc = ws_alloc(&w->recs, S * n_max, &w->bytes)) ||
+        (rc = ws_alloc(&w->nbr_cnt, S * w->row_stride, &w->bytes)) ||
+        (rc = ws_alloc(&w->nbr, S * n_max * lm::kNbr, &w->bytes)) ||
+        (rc = ws_alloc(&w->row_off, S * w->row_stride, &w->bytes))) {
         pairs_ws_free(w);
         return rc;
     }
-    hipError_t e = hipMemset(w->count, 0, sizeof(uint32_t) * T);
+    hipError_t e = hipMemset(w->count, 0, sizeof(uint32_t) * (size_t)(S * w->tab_stride));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         lm::set_error("workspace init failed: %s", hipGetErrorString(e));
@@ -341,25 +374,29 @@ void lm_pairs_ws_destroy(lm_pairs_ws* ws) { pairs_ws_free(ws); }
 
 int64_t lm_pairs_ws_bytes(const lm_pairs_ws* ws) { return ws ? ws->bytes : 0; }
 
-int lm_query_pairs(lm_pairs_ws* ws, int64_t n, const float* lon, const float* lat, double radius,
-                   double p, int32_t* pair_i, int32_t* pair_j, int64_t capacity, int64_t* npairs,
-                   void* stream) {
+int lm_query_pairs_batch(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float* lon, const float* lat,
+                         int64_t pos_stride, double radius, double p, int32_t* pair_i, int32_t* pair_j,
+                         int64_t pair_stride, int64_t* npairs, void* stream) {
     LM_REQUIRE(ws != nullptr, LM_ERR_ARG, "workspace is NULL");
     LM_REQUIRE(npairs != nullptr, LM_ERR_ARG, "npairs is NULL");
+    LM_REQUIRE(nsteps >= 0 && nsteps <= ws->steps_max, LM_ERR_ARG, "nsteps=%d outside 0..%d of the workspace",
+               nsteps, ws->steps_max);
     LM_REQUIRE(n >= 0 && n <= ws->n_max, LM_ERR_ARG, "n=%lld exceeds the workspace (n_max=%lld)",
                (long long)n, (long long)ws->n_max);
     LM_REQUIRE(radius >= 0.0 && radius <= ws->max_radius, LM_ERR_ARG,
                "radius %g outside [0, %g] of the workspace", radius, ws->max_radius);
     LM_REQUIRE(p >= 1.0, LM_ERR_ARG, "Minkowski p must be >= 1 (got %g)", p);
-    LM_REQUIRE(capacity >= 0, LM_ERR_ARG, "negative capacity");
+    LM_REQUIRE(pair_stride >= 0, LM_ERR_ARG, "negative pair capacity");
+    LM_REQUIRE(nsteps <= 1 || pos_stride >= n, LM_ERR_ARG, "position stride < n");
     hipStream_t s = (hipStream_t)stream;
+    if (nsteps == 0) return LM_OK;
     if (n == 0) {
-        lm::set_i64_kernel<<<1, 1, 0, s>>>(npairs, 0);
+        hipLaunchKernelGGL(lm::zero_i64_kernel, dim3(1), dim3(64), 0, s, npairs, (int64_t)nsteps);
         LM_LAUNCH_CHECK();
         return LM_OK;
     }
     LM_REQUIRE(lon && lat, LM_ERR_ARG, "NULL positions");
-    LM_REQUIRE(capacity == 0 || (pair_i && pair_j), LM_ERR_ARG, "NULL pair buffers");
+    LM_REQUIRE(pair_stride == 0 || (pair_i && pair_j), LM_ERR_ARG, "NULL pair buffers");
 
     lm::Metric m;
     m.r = radius;
@@ -378,31 +415,45 @@ int lm_query_pairs(lm_pairs_ws* ws, int64_t n, const float* lon, const float* la
         m.r2 = std::pow(radius, p);
     }
     const lm::CellGrid& g = ws->grid;
-    const unsigned gn = lm::grid_for(n);
+    lm::Strides st;
+    st.pos = pos_stride;
+    st.tab = ws->tab_stride;
+    st.rec = ws->n_max;
+    st.row = ws->row_stride;
+    st.nbr = ws->n_max * lm::kNbr;
+    st.pair = pair_stride;
+    const dim3 grid(lm::grid_for(n), (unsigned)nsteps);
     if (ws->dirty) {
-        LM_HIP_CHECK(hipMemsetAsync(ws->count, 0, sizeof(uint32_t) * g.T, s));
+        LM_HIP_CHECK(hipMemsetAsync(ws->count, 0, sizeof(uint32_t) * (size_t)(ws->steps_max * ws->tab_stride), s));
         ws->dirty = false;
     }
     ws->dirty = true;  // until every launch below went through
 
-    LM_LAUNCH(lm::K_BIN_COUNT, lm::bin_count_kernel, dim3(gn), dim3(lm::kBlock), 0, s, n, lon, lat, g, ws->count);
+    LM_LAUNCH(lm::K_BIN_COUNT, lm::bin_count_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, g, st, ws->count);
     LM_LAUNCH_CHECK();
-    int rc = lm::exclusive_scan_u32(ws->count, (int64_t)g.T, ws->start, ws->scan_tmp, nullptr, s);
+    int rc = lm::exclusive_scan_u32_batched(ws->count, (int64_t)g.T, st.tab, ws->start, st.tab, nsteps,
+                                            ws->scan_tmp, nullptr, s);
     if (rc) return rc;
-    LM_LAUNCH(lm::K_BIN_SCATTER, lm::bin_scatter_kernel, dim3(gn), dim3(lm::kBlock), 0, s, n, lon, lat, g,
+    LM_LAUNCH(lm::K_BIN_SCATTER, lm::bin_scatter_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, g, st,
               (const uint32_t*)ws->start, ws->count, ws->recs);
     LM_LAUNCH_CHECK();
-    LM_LAUNCH(lm::K_PAIR_COUNT, lm::pair_count_kernel, dim3(gn), dim3(lm::kBlock), 0, s, n, g, m,
+    LM_LAUNCH(lm::K_PAIR_COUNT, lm::pair_count_kernel, grid, dim3(lm::kBlock), 0, s, n, g, m, st,
               (const uint32_t*)ws->start, (const float4*)ws->recs, ws->nbr_cnt, ws->nbr);
     LM_LAUNCH_CHECK();
-    rc = lm::exclusive_scan_u32(ws->nbr_cnt, n, ws->row_off, ws->scan_tmp, npairs, s);
+    rc = lm::exclusive_scan_u32_batched(ws->nbr_cnt, n, st.row, ws->row_off, st.row, nsteps, ws->scan_tmp, npairs,
+                                        s);
     if (rc) return rc;
-    LM_LAUNCH(lm::K_PAIR_EMIT, lm::pair_emit_kernel, dim3(gn), dim3(lm::kBlock), 0, s, n, lon, lat, g, m,
+    LM_LAUNCH(lm::K_PAIR_EMIT, lm::pair_emit_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, g, m, st,
               (const uint32_t*)ws->start, (const float4*)ws->recs, (const uint32_t*)ws->nbr_cnt,
-              (const uint32_t*)ws->nbr, (const uint32_t*)ws->row_off, capacity, pair_i, pair_j);
+              (const uint32_t*)ws->nbr, (const uint32_t*)ws->row_off, pair_i, pair_j);
     LM_LAUNCH_CHECK();
     ws->dirty = false;
     return LM_OK;
+}
+
+int lm_query_pairs(lm_pairs_ws* ws, int64_t n, const float* lon, const float* lat, double radius, double p,
+                   int32_t* pair_i, int32_t* pair_j, int64_t capacity, int64_t* npairs, void* stream) {
+    return lm_query_pairs_batch(ws, 1, n, lon, lat, n, radius, p, pair_i, pair_j, capacity, npairs, stream);
 }
 
 }  // extern "C"

@@ -21,6 +21,16 @@ inline int64_t scan_num_blocks(int64_t n) {
 int exclusive_scan_u32(const uint32_t* in, int64_t n, uint32_t* out, uint32_t* block_sums,
                        int64_t* total64, hipStream_t stream);
 
+// nseg independent scans: segment g reads in + g*in_stride and writes
+// out + g*out_stride (n + 1 entries); totals (if not NULL) to total64[g].
+// block_sums must hold nseg * scan_num_blocks(n) entries; strides are
+// multiples of 4 and out_stride >= n + 1.
+int exclusive_scan_u32_batched(const uint32_t* in, int64_t n, int64_t in_stride, uint32_t* out, int64_t out_stride,
+                               int32_t nseg, uint32_t* block_sums, int64_t* total64, hipStream_t stream);
+
+// Row stride of per-particle arrays that are scanned (n + 1 entries, 16-byte rows).
+inline int64_t scan_row_stride(int64_t n) { return (n + 4) & ~(int64_t)3; }
+
 // ---- wave/block building blocks (wave64) ---------------------------------
 __device__ inline uint32_t wave_inclusive_scan(uint32_t x) {
     const int lane = threadIdx.x & 63;

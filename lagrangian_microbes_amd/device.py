@@ -130,13 +130,14 @@ class PairSearch:
     expected; particles outside are still handled exactly.
     """
 
-    def __init__(self, n_max, bbox, max_radius, device=None):
+    def __init__(self, n_max, bbox, max_radius, device=None, steps_max=1):
         self.n_max = int(n_max)
+        self.steps_max = int(steps_max)
         self.max_radius = float(max_radius)
         self.bbox = tuple(float(b) for b in bbox)
         h = c_vp()
-        check(lib().lm_pairs_ws_create(self.n_max, self.bbox[0], self.bbox[1], self.bbox[2], self.bbox[3],
-                                       self.max_radius, ctypes.byref(h)))
+        check(lib().lm_pairs_ws_create(self.n_max, self.steps_max, self.bbox[0], self.bbox[1], self.bbox[2],
+                                       self.bbox[3], self.max_radius, ctypes.byref(h)))
         self._ws = h
         self.device = torch.device(device or "cuda")
         self.pair_i = None
@@ -166,6 +167,24 @@ class PairSearch:
         check(lib().lm_query_pairs(self._ws, n, ptr(lon), ptr(lat), float(radius), float(p), ptr(pair_i),
                                    ptr(pair_j), pair_i.numel(), ptr(npairs), stream_ptr()))
 
+    def query_batch(self, lon, lat, radius, p, pair_i, pair_j, npairs):
+        """Asynchronous, one snapshot per row: lon/lat [S, n] float32, pairs
+        [S, cap] int32 (capacity cap per step), npairs [S] int64."""
+        if lon.dim() != 2:
+            raise ValueError("lon/lat must be [steps, n]")
+        S, n = lon.shape
+        require(lon, torch.float32, name="lon")
+        require(lat, torch.float32, numel=S * n, name="lat")
+        if pair_i.dim() != 2 or pair_i.shape[0] < S:
+            raise ValueError("pair buffers must be [steps, capacity]")
+        require(pair_i, torch.int32, name="pair_i")
+        require(pair_j, torch.int32, numel=pair_i.numel(), name="pair_j")
+        require(npairs, torch.int64, min_numel=S, name="npairs")
+        if n > self.n_max or S > self.steps_max:
+            raise ValueError(f"{S} x {n} exceeds the workspace's {self.steps_max} x {self.n_max}")
+        check(lib().lm_query_pairs_batch(self._ws, S, n, ptr(lon), ptr(lat), n, float(radius), float(p),
+                                         ptr(pair_i), ptr(pair_j), pair_i.shape[1], ptr(npairs), stream_ptr()))
+
     def ensure_capacity(self, cap):
         if self.pair_i is None or self.pair_i.numel() < cap:
             cap = max(int(cap), 1024)
@@ -192,10 +211,11 @@ class RPSUpdater:
     """Rock-paper-scissors over a sorted device pair list (replaces the
     per-pair Python loop over rock_paper_scissors_interaction)."""
 
-    def __init__(self, n_max, device=None):
+    def __init__(self, n_max, device=None, steps_max=1):
         self.n_max = int(n_max)
+        self.steps_max = int(steps_max)
         h = c_vp()
-        check(lib().lm_rps_ws_create(self.n_max, ctypes.byref(h)))
+        check(lib().lm_rps_ws_create(self.n_max, self.steps_max, ctypes.byref(h)))
         self._ws = h
         self.device = torch.device(device or "cuda")
         self._stats = torch.zeros(4, dtype=torch.int64, device=self.device)
@@ -222,9 +242,29 @@ class RPSUpdater:
                                   pair_i.numel(), float(pRS), float(pPR), float(pSP), int(key[0]),
                                   int(key[1]), int(step), stream_ptr()))
 
-    def component_stats(self):
-        """(components, max pairs in one, max row heads in one, total pairs) of the last update."""
-        check(lib().lm_rps_component_stats(self._ws, ptr(self._stats), stream_ptr()))
+    def prepare(self, n, pair_i, pair_j, npairs, pRS, pPR, pSP, key, step0):
+        """Group the pairs of S steps ([S, cap] buffers, counts npairs[S])."""
+        if pair_i.dim() != 2:
+            raise ValueError("pair buffers must be [steps, capacity]")
+        S, cap = pair_i.shape
+        require(pair_i, torch.int32, name="pair_i")
+        require(pair_j, torch.int32, numel=pair_i.numel(), name="pair_j")
+        require(npairs, torch.int64, min_numel=S, name="npairs")
+        if n > self.n_max or S > self.steps_max:
+            raise ValueError(f"{S} x {n} exceeds the workspace's {self.steps_max} x {self.n_max}")
+        check(lib().lm_rps_prepare(self._ws, S, int(n), ptr(pair_i), ptr(pair_j), cap, ptr(npairs), float(pRS),
+                                   float(pPR), float(pSP), int(key[0]), int(key[1]), int(step0), stream_ptr()))
+
+    def play(self, step, species_in, species_out):
+        """Step `step` of the last prepare: species_out = species_in after its pairs."""
+        n = species_in.numel()
+        require(species_in, torch.int8, name="species_in")
+        require(species_out, torch.int8, numel=n, name="species_out")
+        check(lib().lm_rps_play(self._ws, int(step), n, ptr(species_in), ptr(species_out), stream_ptr()))
+
+    def component_stats(self, step=0):
+        """(components, max pairs in one, max particles in one, total pairs) of a prepared step."""
+        check(lib().lm_rps_component_stats(self._ws, int(step), ptr(self._stats), stream_ptr()))
         return tuple(int(v) for v in self._stats.cpu().tolist())
 
 

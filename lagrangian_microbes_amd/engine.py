@@ -22,15 +22,18 @@ from . import device as dev
 
 
 class InteractionEngine:
-    """Pair search + rock-paper-scissors for n particles on the device."""
+    """Pair search + rock-paper-scissors for n particles on the device, run
+    over batches of up to steps_max steps."""
 
-    def __init__(self, n, bbox, radius, norm=2.0, device=None, initial_capacity=None):
+    def __init__(self, n, bbox, radius, norm=2.0, device=None, steps_max=1, initial_capacity=None):
         self.n = int(n)
         self.radius = float(radius)
         self.norm = float(norm)
+        self.steps_max = int(steps_max)
         self.device = torch.device(device or "cuda")
-        self.search = dev.PairSearch(self.n, bbox, self.radius, self.device)
-        self.rps = dev.RPSUpdater(self.n, self.device)
+        self.search = dev.PairSearch(self.n, bbox, self.radius, self.device, self.steps_max)
+        self.rps = dev.RPSUpdater(self.n, self.device, self.steps_max)
+        self.npairs = torch.zeros(self.steps_max, dtype=torch.int64, device=self.device)
         self.capacity = 0
         self._grow(initial_capacity or max(2 * self.n, 1024))
 
@@ -38,38 +41,44 @@ class InteractionEngine:
         cap = int(cap)
         if cap > self.capacity:
             self.capacity = cap
-            self.pair_i = torch.empty(cap, dtype=torch.int32, device=self.device)
-            self.pair_j = torch.empty(cap, dtype=torch.int32, device=self.device)
-
-    def pairs_step(self, x32, y32, npairs_slot):
-        self.search.query(x32, y32, self.radius, self.norm, self.pair_i, self.pair_j, npairs_slot)
-
-    def rps_step(self, species, npairs_slot, params, key, step):
-        self.rps.update(species, self.pair_i, self.pair_j, npairs_slot, params["pRS"], params["pPR"],
-                        params["pSP"], key, step)
+            self.pair_i = torch.empty((self.steps_max, cap), dtype=torch.int32, device=self.device)
+            self.pair_j = torch.empty((self.steps_max, cap), dtype=torch.int32, device=self.device)
 
     def run_chunk(self, rec_x, rec_y, species, params, key, step0, rec_species=None, npairs=None):
-        """Steps s = 0..len(rec_x)-1 over recorded float32 positions rec_x[s], rec_y[s].
+        """Steps s = 0..S-1 over recorded float32 positions rec_x[s], rec_y[s] ([S, n]).
 
-        species (int8, device) is updated in place; rec_species[s] receives the
-        species after step s.  Returns the per-step pair counts (device int64).
+        The pair search and the grouping of every step run once for the whole
+        batch; the replay runs step by step.  species (int8, device) ends as
+        the state after the last step; rec_species[s] (if given) receives the
+        state after step s.  One host synchronisation per call, to check the
+        pair counts against the buffers (the batch is redone with larger
+        buffers if a step overflowed).  Returns the per-step pair counts.
         """
-        steps = rec_x.shape[0]
-        if npairs is None:
-            npairs = torch.zeros(steps, dtype=torch.int64, device=self.device)
-        start = species.clone()
+        S = rec_x.shape[0]
+        if S == 0:
+            return self.npairs[:0]
+        if S > self.steps_max:
+            raise ValueError(f"{S} steps exceed the engine's batch of {self.steps_max}")
+        start = species.clone() if rec_species is None else None
         while True:
-            for s in range(steps):
-                slot = npairs[s:s + 1]
-                self.pairs_step(rec_x[s], rec_y[s], slot)
-                self.rps_step(species, slot, params, key, step0 + s)
-                if rec_species is not None:
-                    rec_species[s].copy_(species)
-            worst = int(npairs.max().item()) if steps else 0
+            pi, pj, cnt = self.pair_i[:S], self.pair_j[:S], self.npairs[:S]
+            self.search.query_batch(rec_x, rec_y, self.radius, self.norm, pi, pj, cnt)
+            self.rps.prepare(self.n, pi, pj, cnt, params["pRS"], params["pPR"], params["pSP"], key, step0)
+            sp_in = species
+            for s in range(S):
+                out = rec_species[s] if rec_species is not None else species
+                self.rps.play(s, sp_in, out)
+                sp_in = out
+            worst = int(cnt.max().item())
             if worst <= self.capacity:
-                return npairs
+                if rec_species is not None:
+                    species.copy_(rec_species[S - 1])
+                if npairs is not None:
+                    npairs.copy_(cnt)
+                return cnt
             self._grow(worst + worst // 4 + 1024)
-            species.copy_(start)
+            if start is not None:
+                species.copy_(start)
 
 
 def bbox_with_margin(lon_min, lon_max, lat_min, lat_max, margin):

@@ -57,6 +57,7 @@ class InteractionSimulator:
         self.iteration = 0
         self.philox_key = tuple(philox_key) if philox_key is not None else new_key()
         self.device = torch.device(device or "cuda")
+        self.batch_steps = 100  # steps whose pair search and grouping share one set of launches
 
     def time_step(self, start_time, end_time, dt):
         path = os.path.join(self.advection_dir, "particle_data.nc")
@@ -84,7 +85,8 @@ class InteractionSimulator:
             finite = np.isfinite(plon[:, cols]) & np.isfinite(plat[:, cols])
             bbox = (float(plon[:, cols][finite].min(initial=0.0)), float(plon[:, cols][finite].max(initial=0.0)),
                     float(plat[:, cols][finite].min(initial=0.0)), float(plat[:, cols][finite].max(initial=0.0)))
-            engine = InteractionEngine(N, bbox, self.interaction_radius, self.interaction_norm, self.device)
+            engine = InteractionEngine(N, bbox, self.interaction_radius, self.interaction_norm, self.device,
+                                       steps_max=min(self.batch_steps, len(steps)))
             if self.pair_interaction is rock_paper_scissors_interaction:
                 self._run_rps(engine, rec_x, rec_y, steps, mspecies)
             else:
@@ -105,11 +107,16 @@ class InteractionSimulator:
         props = self.microbe_properties
         species = torch.from_numpy(np.ascontiguousarray(props["species"], dtype=np.int8)).to(self.device)
         rec_s = torch.empty((len(steps), species.numel()), dtype=torch.int8, device=self.device)
-        npairs = engine.run_chunk(rec_x, rec_y, species, self.pair_interaction_parameters, self.philox_key,
-                                  steps[0], rec_species=rec_s)
+        counts = []
+        B = engine.steps_max
+        for a in range(0, len(steps), B):
+            b = min(a + B, len(steps))
+            cnt = engine.run_chunk(rec_x[a:b], rec_y[a:b], species, self.pair_interaction_parameters,
+                                   self.philox_key, steps[a], rec_species=rec_s[a:b])
+            counts.append(cnt.cpu().numpy().copy())
         mspecies[:, steps] = rec_s.cpu().numpy().T
         props["species"][:] = species.cpu().numpy()
-        self.pairs_per_step = npairs.cpu().numpy()
+        self.pairs_per_step = np.concatenate(counts)
 
     def _run_generic(self, engine, rec_x, rec_y, steps, mspecies):
         props = self.microbe_properties

@@ -50,7 +50,8 @@ class FusedRunner:
         self.walk_amp = walk_amplitude(Kh, self.dt_s)
         self.walk_key = tuple(walk_key)
         self.dfield = dev.DeviceField(field, self.device)
-        self.engine = InteractionEngine(self.n, bbox or field.bounds, radius, norm, self.device)
+        self.engine = InteractionEngine(self.n, bbox or field.bounds, radius, norm, self.device,
+                                        steps_max=self.chunk)
         self.rec_x = torch.empty((self.chunk, self.n), dtype=torch.float32, device=self.device)
         self.rec_y = torch.empty((self.chunk, self.n), dtype=torch.float32, device=self.device)
         self.rec_s = torch.empty((self.chunk, self.n), dtype=torch.int8, device=self.device)
