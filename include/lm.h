@@ -197,6 +197,22 @@ int lm_philox_uniforms(int64_t count, uint64_t c0, uint64_t c1, uint64_t c2, uin
 int lm_copy_bytes(const void* src, void* dst, int64_t nbytes, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Segmented stable radix sort of (uint32 key, uint32 value) pairs: the     */
+/* grouping primitive under the pair search and the RPS grouping.  Exposed  */
+/* for testing.  No reference counterpart (the reference sorts with numpy / */
+/* set iteration, interaction_simulator.py:74).                             */
+/* ------------------------------------------------------------------------ */
+
+/* Sorts nseg segments in place (segment g at keys + g*stride, vals +
+ * g*stride) by bits [0, key_bits) of the key, stably.  Segment g holds
+ * seg_len[g] elements (device int64 array; uniform_len if seg_len is NULL),
+ * at most max_len <= stride.  keys_tmp / vals_tmp are buffers of the same
+ * layout.  Allocates its scratch and synchronises the stream. */
+int lm_sort_pairs_u32(int32_t nseg, const int64_t* seg_len, int64_t uniform_len, int64_t max_len, int64_t stride,
+                      int32_t key_bits, uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp,
+                      void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Per-kernel timing.  No reference counterpart (the reference times phases */
 /* with time() around each call, particle_advecter.py:221-244,              */
 /* interaction_simulator.py:91-106).                                        */

@@ -48,6 +48,7 @@ _SIGNATURES = {
     "lm_rps_component_stats": (ctypes.c_int, [c_vp, c_i32, c_vp, c_vp]),
     "lm_philox_uniforms": (ctypes.c_int, [c_i64, c_u64, c_u64, c_u64, c_u64, c_u64, c_u64, c_i32, c_vp, c_vp]),
     "lm_copy_bytes": (ctypes.c_int, [c_vp, c_vp, c_i64, c_vp]),
+    "lm_sort_pairs_u32": (ctypes.c_int, [c_i32, c_vp, c_i64, c_i64, c_i64, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "lm_timing_enable": (ctypes.c_int, [ctypes.c_int]),
     "lm_timing_summary": (ctypes.c_int, [c_vp, c_vp, c_i32]),
     "lm_timing_kernel_name": (ctypes.c_char_p, [c_i32]),

@@ -65,16 +65,16 @@ enum KernelId {
     K_SCAN_APPLY,
     K_RPS_INIT,
     K_RPS_UNION,
-    K_RPS_HEADS,
-    K_RPS_SCATTER,
+    K_RPS_KEYS,
+    K_RPS_SEGMENTS,
     K_RPS_PLAY,
     K_RPS_PLAY_WIDE,
-    K_RPS_PLAY_LARGE,
-    K_RPS_ORDER,
     K_RPS_FILL,
     K_RPS_DEPS,
     K_PHILOX,
     K_COPY,
+    K_SORT_HIST,
+    K_SORT_SCATTER,
     K_NUM_KERNELS
 };
 bool timing_on();

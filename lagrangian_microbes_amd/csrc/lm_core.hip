@@ -24,9 +24,9 @@ void set_error(const char* fmt, ...) {
 // ---------------------------------------------------------------------------
 static const char* const kKernelNames[K_NUM_KERNELS] = {
     "advect_rk4_f64", "field_blend_f64", "bin_count", "bin_scatter", "pair_count", "pair_emit",
-    "scan_reduce",    "scan_apply",      "rps_init",  "rps_union",   "rps_claim",  "rps_group",
-    "rps_play",       "rps_play_wide",   "rps_play_large", "rps_order", "rps_fill", "rps_deps", "philox",
-    "copy16"};
+    "scan_reduce",    "scan_apply",      "rps_init",  "rps_union",   "rps_keys",   "rps_segments",
+    "rps_play",       "rps_play_wide",   "rps_fill",  "rps_deps",    "philox",
+    "copy16", "sort_hist", "sort_scatter"};
 
 struct TimedLaunch {
     int kid;

@@ -3,52 +3,54 @@
 //
 // A pair only reads and writes the species of its two particles, so pairs in
 // different connected components of the pair graph commute; inside a component
-// the sequential order is kept.  Pairs are sorted by (i, j), so a component's
-// pairs in global order are the rows (runs of equal i) of its members taken in
-// ascending i.
+// the sequential order is kept.
 //
 // Everything that depends only on the pairs runs once for a batch of steps
 // (blockIdx.y = step), with the parallelism of all steps at once:
-//   prepare 1. init     parent[v] = v, local slots cleared
-//           2. union    lock-free union-find, hooking the larger root under the
-//                       smaller (every root is the smallest index of its
-//                       component); the same pass draws each pair's uniform
-//                       (Philox keyed by pair rank and step) and keeps its three
-//                       comparisons with pRS, pPR, pSP as bits
-//           3. claim    each particle of a pair takes a local slot in its
-//                       component; each row head counts itself and its row
-//           4. scans    head, particle and pair offsets per component
-//           5. group    heads and particle ids grouped by component; small
-//                       components (<= 8 heads, <= 32 pairs) and the rest listed
-//           6. order    heads sorted per component (registers / LDS bitonic) and
-//                       each row's position in its component's pair sequence
-//           7. fill     pair records (slot_i | bits << 29, slot_j) in component
-//                       order
-//           8. deps     for the wide components, per 64-pair window, the bitmask
-//                       of earlier pairs of the window that share a particle
+//   prepare 1. init      parent[v] = v; touched marks and slots cleared
+//           2. union     lock-free union-find, hooking the larger root under the
+//                        smaller, so every root is the smallest index of its
+//                        component; the same pass draws each pair's uniform
+//                        (Philox keyed by pair rank and step), keeps its three
+//                        comparisons with pRS, pPR, pSP as bits and marks both
+//                        particles as touched
+//           3. keys      sort keys: each touched particle's root (untouched
+//                        particles: n, after every root) and each pair's root
+//           4. sorts     stable radix sorts of particles and of pairs by root:
+//                        a component's particles become one run in index
+//                        order -- its root first -- and its pairs one run in
+//                        pair order
+//           5. segments  each particle's position in the sorted particles (its
+//                        slot is that minus its root's position) and the end of
+//                        every component's runs
+//           6. fill      pair records (slot_i | bits << 29, slot_j) in run
+//                        order; each component listed once, as small (<= 32
+//                        pairs) or wide, with its two runs
+//           7. deps      for the wide components, per 64-pair window, the
+//                        bitmask of earlier pairs of the window sharing a slot
+//   No global atomics but the union's CAS and one add per wavefront for the
+//   component lists.
 // Only the replay itself runs per step (two launches):
-//   play    narrow      one lane per small component: species into LDS by slot,
-//                       pairs replayed in order; particles without pairs copied
-//           wide        one wavefront per larger component: species into LDS,
-//                       each window replayed in rounds -- a pair fires once every
-//                       earlier pair of the window it depends on has fired, which
-//                       is exactly the sequential order
-// Counters return to zero within a call.
+//   play    narrow       one lane per small component: species into LDS by
+//                        slot, pairs replayed in order; particles without pairs
+//                        copied
+//           wide         one wavefront per larger component: species into LDS,
+//                        each window replayed in rounds -- a pair fires once
+//                        every earlier pair of the window it depends on has
+//                        fired, which is exactly the sequential order
 #include "lm_common.h"
 #include "lm_scan.h"
+#include "lm_sort.h"
 
 namespace lm {
 
 constexpr uint32_t kUnset = 0xffffffffu;
-constexpr uint32_t kClaimed = 0xfffffffeu;
-constexpr uint32_t kNarrowHeads = 8;
 constexpr uint32_t kNarrowPairs = 32;
 constexpr uint32_t kNarrowSlots = kNarrowPairs + 1;  // a component of P pairs has <= P + 1 particles
-constexpr uint32_t kSortHeads = 2048;                // LDS head sort of the wide tier
 constexpr uint32_t kWideSlots = 16384;               // LDS species slots of the wide replay
 constexpr uint32_t kSlotMask = (1u << 29) - 1u;
 constexpr unsigned kPairGrid = 256;                  // blocks per step for pair-parallel passes
-constexpr unsigned kCompGrid = 128;                  // blocks per step for component-parallel passes
+constexpr unsigned kWideBlocks = 4096;               // one-wavefront blocks of the wide replay
 
 // Per-step strides (elements) of a batch.
 struct RpsStrides {
@@ -56,14 +58,22 @@ struct RpsStrides {
     int64_t pair;  // per-pair arrays (the pair capacity of a step)
 };
 
-// Per-step views of the workspace arrays.
+// Workspace arrays; per step s a per-particle array starts at s * row, a
+// per-pair array at s * pair.  A component is described by a uint4
+// {first pair, pairs, first particle, particles} of its runs.
 struct RpsArrays {
-    uint32_t *parent, *label, *loc, *rowbeg, *rowend, *rowoff;
-    uint32_t *hcount, *hstart, *pcount, *pstart, *qcount, *qstart;
-    uint32_t *members, *ids, *narrow, *wide, *lists;
-    uint8_t* bits;
-    uint2* rec;
-    unsigned long long* deps;
+    uint32_t *parent, *loc, *pend, *qend;  // per particle
+    uint8_t* touched;                      // per particle
+    uint32_t *pkey, *pval, *ptk, *ptv;     // particle sort buffers (per particle)
+    uint32_t *qkey, *qval, *qtk, *qtv;     // pair sort buffers (per pair)
+    const uint32_t *spk, *ids;             // sorted particle roots / indices (two of the above)
+    const uint32_t *sqk, *sqv;             // sorted pair roots / ranks
+    uint4 *narrow, *wide;                  // component lists (per particle)
+    uint32_t* lists;                       // [step][4]: narrow count, wide count
+    int64_t* plen;                         // [step]: pairs taken (0 for an overflowed step)
+    uint8_t* bits;                         // per pair (by rank)
+    uint2* rec;                            // per pair (run order)
+    unsigned long long* deps;              // per pair (run order, wide components)
 };
 
 __device__ __forceinline__ uint32_t find_root(const uint32_t* parent_c, uint32_t v) {
@@ -118,9 +128,10 @@ __device__ __forceinline__ void play_pair(int& si, int& sj, uint32_t bits) {
         si = sj;
 }
 
-// Wavefront-aggregated append of v to list (one atomic per wavefront).
-__device__ __forceinline__ void wave_append(bool flag, uint32_t v, uint32_t* __restrict__ list,
-                                            uint32_t* __restrict__ count) {
+// Wavefront-aggregated append of v to list (one atomic per wavefront); every
+// lane of the wavefront must call it.
+__device__ __forceinline__ void wave_append(bool flag, uint4 v, uint4* __restrict__ list,
+                                            uint32_t* __restrict__ count, uint32_t cap) {
     const uint64_t mask = __ballot(flag);
     if (mask == 0) return;
     const uint32_t lane = __lane_id();
@@ -128,35 +139,9 @@ __device__ __forceinline__ void wave_append(bool flag, uint32_t v, uint32_t* __r
     uint32_t base = 0;
     if (lane == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
     base = __shfl(base, leader, 64);
-    if (flag) list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = v;
+    const uint32_t at = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    if (flag && at < cap) list[at] = v;
 }
-
-#define RPS_STEP_VIEW(A, st, s)                                                                 \
-    const int64_t _ro = (s) * (st).row, _po = (s) * (st).pair;                                  \
-    uint32_t* parent = (A).parent + _ro;                                                        \
-    uint32_t* label = (A).label + _ro;                                                          \
-    uint32_t* loc = (A).loc + _ro;                                                              \
-    uint32_t* rowbeg = (A).rowbeg + _ro;                                                        \
-    uint32_t* rowend = (A).rowend + _ro;                                                        \
-    uint32_t* rowoff = (A).rowoff + _ro;                                                        \
-    uint32_t* hcount = (A).hcount + _ro;                                                        \
-    uint32_t* hstart = (A).hstart + _ro;                                                        \
-    uint32_t* pcount = (A).pcount + _ro;                                                        \
-    uint32_t* pstart = (A).pstart + _ro;                                                        \
-    uint32_t* qcount = (A).qcount + _ro;                                                        \
-    uint32_t* qstart = (A).qstart + _ro;                                                        \
-    uint32_t* members = (A).members + _ro;                                                      \
-    uint32_t* ids = (A).ids + _ro;                                                              \
-    uint32_t* narrow = (A).narrow + _ro;                                                        \
-    uint32_t* wide = (A).wide + _ro;                                                            \
-    uint32_t* lists = (A).lists + 4 * (s);                                                      \
-    uint8_t* bits = (A).bits + _po;                                                             \
-    uint2* rec = (A).rec + _po;                                                                 \
-    unsigned long long* deps = (A).deps + _po;                                                  \
-    (void)parent; (void)label; (void)loc; (void)rowbeg; (void)rowend; (void)rowoff;             \
-    (void)hcount; (void)hstart; (void)pcount; (void)pstart; (void)qcount; (void)qstart;         \
-    (void)members; (void)ids; (void)narrow; (void)wide; (void)lists; (void)bits; (void)rec;     \
-    (void)deps
 
 // ---------------------------------------------------------------------------
 // prepare
@@ -164,227 +149,124 @@ __device__ __forceinline__ void wave_append(bool flag, uint32_t v, uint32_t* __r
 __global__ void __launch_bounds__(kBlock)
 rps_init_kernel(int64_t n, RpsArrays A, RpsStrides st) {
     const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int64_t s = blockIdx.y;
-    RPS_STEP_VIEW(A, st, s);
+    const int64_t s = blockIdx.y, ro = s * st.row;
     if (v < n) {
-        parent[v] = (uint32_t)v;
-        loc[v] = kUnset;
+        A.parent[ro + v] = (uint32_t)v;
+        A.loc[ro + v] = kUnset;
+        A.touched[ro + v] = 0;
     }
-    if (v < 4) lists[v] = 0;
+    if (v < 4) A.lists[4 * s + v] = 0;
 }
 
 __global__ void __launch_bounds__(kBlock)
 rps_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __restrict__ pj_all,
                  const int64_t* __restrict__ npairs, RpsArrays A, RpsStrides st, double pRS, double pPR,
                  double pSP, uint64_t k0, uint64_t k1, uint64_t step0) {
-    const int64_t s = blockIdx.y;
-    RPS_STEP_VIEW(A, st, s);
+    const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
     const int64_t P = step_pairs(npairs, s, st.pair);
-    const int32_t* pi = pi_all + _po;
-    const int32_t* pj = pj_all + _po;
+    if (blockIdx.x == 0 && threadIdx.x == 0) A.plen[s] = P < 0 ? 0 : P;
+    const int32_t* pi = pi_all + po;
+    const int32_t* pj = pj_all + po;
+    uint32_t* parent = A.parent + ro;
+    uint8_t* touched = A.touched + ro;
+    uint8_t* bits = A.bits + po;
     for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
         const uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
         const double u =
             u01_from_u64(philox4x64_10((uint64_t)k, step0 + (uint64_t)s, kStreamPairs, 0, k0, k1).v[0]);
         bits[k] = (uint8_t)((u < pRS ? 1u : 0u) | (u < pPR ? 2u : 0u) | (u < pSP ? 4u : 0u));
-        if (a < (uint64_t)n && b < (uint64_t)n && a != b) unite(parent, a, b);
-    }
-}
-
-__device__ __forceinline__ void claim_slot(uint32_t* loc, uint32_t* label, uint32_t* parent, uint32_t* pcount,
-                                           uint32_t x) {
-    if (atomicCAS(loc + x, kUnset, kClaimed) == kUnset) {
-        const uint32_t L = find_root(parent, x);
-        label[x] = L;
-        loc[x] = atomicAdd(pcount + L, 1u);
-    }
-}
-
-__global__ void __launch_bounds__(kBlock)
-rps_claim_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __restrict__ pj_all,
-                 const int64_t* __restrict__ npairs, RpsArrays A, RpsStrides st) {
-    const int64_t s = blockIdx.y;
-    RPS_STEP_VIEW(A, st, s);
-    const int64_t P = step_pairs(npairs, s, st.pair);
-    const int32_t* pi = pi_all + _po;
-    const int32_t* pj = pj_all + _po;
-    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
-        const uint32_t h = (uint32_t)pi[k], j = (uint32_t)pj[k];
-        if (h >= (uint64_t)n || j >= (uint64_t)n) continue;
-        if (k == 0 || (uint32_t)pi[k - 1] != h) {  // row head: its row is the run of equal i
-            int64_t e = k + 1;
-            while (e < P && (uint32_t)pi[e] == h) ++e;
-            const uint32_t L = find_root(parent, h);
-            rowbeg[h] = (uint32_t)k;
-            rowend[h] = (uint32_t)e;
-            atomicAdd(hcount + L, 1u);
-            atomicAdd(qcount + L, (uint32_t)(e - k));
+        if (a < (uint64_t)n && b < (uint64_t)n) {
+            touched[a] = 1;
+            touched[b] = 1;
+            if (a != b) unite(parent, a, b);
         }
-        claim_slot(loc, label, parent, pcount, h);
-        claim_slot(loc, label, parent, pcount, j);
     }
 }
 
-// Blocks [0, kPairGrid) group heads by component (pairs in parallel); the rest
-// place particle ids at their slots and list each component once (particles in
-// parallel).
+// Blocks [0, nbp): particle keys; the rest: pair keys.
 __global__ void __launch_bounds__(kBlock)
-rps_group_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int64_t* __restrict__ npairs,
-                 RpsArrays A, RpsStrides st) {
-    const int64_t s = blockIdx.y;
-    RPS_STEP_VIEW(A, st, s);
-    const int64_t P = step_pairs(npairs, s, st.pair);
-    if (P < 0) return;
-    if (blockIdx.x < kPairGrid) {
-        const int32_t* pi = pi_all + _po;
-        for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < P; k += (int64_t)kPairGrid * kBlock) {
-            const uint32_t h = (uint32_t)pi[k];
-            if (h >= (uint64_t)n || !(k == 0 || (uint32_t)pi[k - 1] != h)) continue;
-            const uint32_t L = label[h];
-            const uint32_t slot = hstart[L] + atomicSub(hcount + L, 1u) - 1u;
-            if (slot < (uint64_t)n) members[slot] = h;
+rps_keys_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __restrict__ pj_all, RpsArrays A,
+                RpsStrides st, unsigned nbp) {
+    const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
+    const uint32_t* parent = A.parent + ro;
+    if (blockIdx.x < nbp) {
+        const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+        if (v < n) {
+            A.pkey[ro + v] = A.touched[ro + v] ? find_root(parent, (uint32_t)v) : (uint32_t)n;
+            A.pval[ro + v] = (uint32_t)v;
         }
         return;
     }
-    const int64_t nb = gridDim.x - kPairGrid;
-    for (int64_t x = (int64_t)(blockIdx.x - kPairGrid) * kBlock + threadIdx.x; x - threadIdx.x < n;
-         x += nb * kBlock) {
-        bool to_narrow = false, to_wide = false;
-        if (x < n && loc[x] != kUnset) {
-            const uint32_t L = label[x];
-            const uint32_t slot = pstart[L] + loc[x];
-            if (slot < (uint64_t)n) ids[slot] = (uint32_t)x;
-            if (L == (uint32_t)x) {
-                const uint32_t m = hstart[x + 1] - hstart[x];
-                const uint32_t np = qstart[x + 1] - qstart[x];
-                to_narrow = m <= kNarrowHeads && np <= kNarrowPairs;
-                to_wide = !to_narrow;
-                pcount[x] = 0;
-                qcount[x] = 0;
-            }
-        }
-        wave_append(to_narrow, (uint32_t)x, narrow, lists + 0);
-        wave_append(to_wide, (uint32_t)x, wide, lists + 1);
+    const int64_t P = A.plen[s];
+    const int32_t* pi = pi_all + po;
+    const int32_t* pj = pj_all + po;
+    for (int64_t k = (int64_t)(blockIdx.x - nbp) * kBlock + threadIdx.x; k < P;
+         k += (int64_t)(gridDim.x - nbp) * kBlock) {
+        const uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
+        A.qkey[po + k] = (a < (uint64_t)n && b < (uint64_t)n) ? find_root(parent, a) : (uint32_t)n;
+        A.qval[po + k] = (uint32_t)k;
     }
 }
 
-// In-place ascending heapsort of a[0..m) (one lane; O(m log m)).
-__device__ void heapsort_u32(uint32_t* a, uint32_t m) {
-    auto sift = [&](uint32_t root, uint32_t end) {
-        for (;;) {
-            uint32_t c = 2 * root + 1;
-            if (c >= end) return;
-            if (c + 1 < end && a[c + 1] > a[c]) ++c;
-            if (a[root] >= a[c]) return;
-            const uint32_t t = a[root];
-            a[root] = a[c];
-            a[c] = t;
-            root = c;
-        }
-    };
-    for (uint32_t q = m / 2; q-- > 0;) sift(q, m);
-    for (uint32_t e = m; e-- > 1;) {
-        const uint32_t t = a[0];
-        a[0] = a[e];
-        a[e] = t;
-        sift(0, e);
-    }
-}
-
-// 64-lane blocks.  Blocks [0, kCompGrid): one lane per narrow component --
-// sort its <= 8 heads in registers and lay its rows out in order.  Blocks from
-// kCompGrid on: one wavefront per wide component, heads sorted in LDS (heapsort
-// in place beyond kSortHeads), row offsets by a wavefront scan.
-__global__ void __launch_bounds__(64)
-rps_order_kernel(int64_t n, RpsArrays A, RpsStrides st) {
-    __shared__ uint32_t heads[kSortHeads];
-    const int64_t s = blockIdx.y;
-    RPS_STEP_VIEW(A, st, s);
-    const uint32_t lane = threadIdx.x;
-    if (blockIdx.x < kCompGrid) {
-        const uint32_t cnt = lists[0];
-        for (uint32_t c = blockIdx.x * 64 + lane; c < cnt; c += kCompGrid * 64) {
-            const uint32_t L = narrow[c];
-            const uint32_t hb = hstart[L], m = min(hstart[L + 1] - hb, kNarrowHeads);
-            uint32_t h[kNarrowHeads];
-#pragma unroll
-            for (uint32_t t = 0; t < kNarrowHeads; ++t) h[t] = t < m ? members[hb + t] : 0xffffffffu;
-#pragma unroll
-            for (uint32_t a = 1; a < kNarrowHeads; ++a)
-#pragma unroll
-                for (uint32_t b = a; b > 0; --b) {
-                    const uint32_t lo = min(h[b - 1], h[b]), hi = max(h[b - 1], h[b]);
-                    h[b - 1] = lo;
-                    h[b] = hi;
-                }
-            uint32_t run = qstart[L];
-#pragma unroll
-            for (uint32_t t = 0; t < kNarrowHeads; ++t)
-                if (t < m) {
-                    rowoff[h[t]] = run;
-                    run += rowend[h[t]] - rowbeg[h[t]];
-                }
-        }
+// Blocks [0, nbp): sorted particles -- position of each touched particle and
+// the end of each particle run; the rest: the end of each pair run.
+__global__ void __launch_bounds__(kBlock)
+rps_segments_kernel(int64_t n, RpsArrays A, RpsStrides st, unsigned nbp) {
+    const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
+    if (blockIdx.x < nbp) {
+        const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+        if (e >= n) return;
+        const uint32_t* spk = A.spk + ro;
+        const uint32_t L = spk[e];
+        if (L >= (uint64_t)n) return;
+        const uint32_t x = A.ids[ro + e];
+        if (x < (uint64_t)n) A.loc[ro + x] = (uint32_t)e;
+        if (e + 1 == n || spk[e + 1] != L) A.pend[ro + L] = (uint32_t)(e + 1);
         return;
     }
-    const uint32_t cnt = lists[1];
-    for (uint32_t c = blockIdx.x - kCompGrid; c < cnt; c += gridDim.x - kCompGrid) {
-        const uint32_t L = wide[c];
-        const uint32_t hb = hstart[L], m = min(hstart[L + 1] - hb, (uint32_t)n);
-        uint32_t* hs;
-        if (m <= kSortHeads) {
-            uint32_t m2 = 1;
-            while (m2 < m) m2 <<= 1;
-            for (uint32_t t = lane; t < m2; t += 64) heads[t] = t < m ? members[hb + t] : 0xffffffffu;
-            __syncthreads();
-            for (uint32_t size = 2; size <= m2; size <<= 1) {
-                for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                    for (uint32_t t = lane; t < m2; t += 64) {
-                        const uint32_t u = t ^ stride;
-                        if (u > t) {
-                            const uint32_t x = heads[t], y = heads[u];
-                            if ((x > y) == ((t & size) == 0)) {
-                                heads[t] = y;
-                                heads[u] = x;
-                            }
-                        }
-                    }
-                    __syncthreads();
-                }
-            }
-            hs = heads;
-        } else {
-            if (lane == 0) heapsort_u32(members + hb, m);
-            __syncthreads();
-            hs = members + hb;
-        }
-        // row offsets: each lane scans a contiguous run of heads
-        const uint32_t per = (m + 63) / 64;
-        const uint32_t t0 = min(lane * per, m), t1 = min(t0 + per, m);
-        uint32_t local = 0;
-        for (uint32_t t = t0; t < t1; ++t) local += rowend[hs[t]] - rowbeg[hs[t]];
-        uint32_t run = qstart[L] + wave_inclusive_scan(local) - local;
-        for (uint32_t t = t0; t < t1; ++t) {
-            rowoff[hs[t]] = run;
-            run += rowend[hs[t]] - rowbeg[hs[t]];
-        }
-        __syncthreads();
+    const int64_t P = A.plen[s];
+    const uint32_t* sqk = A.sqk + po;
+    for (int64_t e = (int64_t)(blockIdx.x - nbp) * kBlock + threadIdx.x; e < P;
+         e += (int64_t)(gridDim.x - nbp) * kBlock) {
+        const uint32_t L = sqk[e];
+        if (L < (uint64_t)n && (e + 1 == P || sqk[e + 1] != L)) A.qend[ro + L] = (uint32_t)(e + 1);
     }
 }
 
 __global__ void __launch_bounds__(kBlock)
-rps_fill_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __restrict__ pj_all,
-                const int64_t* __restrict__ npairs, RpsArrays A, RpsStrides st) {
-    const int64_t s = blockIdx.y;
-    RPS_STEP_VIEW(A, st, s);
-    const int64_t P = step_pairs(npairs, s, st.pair);
-    const int32_t* pi = pi_all + _po;
-    const int32_t* pj = pj_all + _po;
-    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
-        const uint32_t h = (uint32_t)pi[k], j = (uint32_t)pj[k];
-        if (h >= (uint64_t)n || j >= (uint64_t)n) continue;
-        const uint32_t pos = rowoff[h] + (uint32_t)k - rowbeg[h];
-        if (pos < (uint64_t)P) rec[pos] = make_uint2((loc[h] & kSlotMask) | ((uint32_t)bits[k] << 29), loc[j]);
+rps_fill_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __restrict__ pj_all, RpsArrays A,
+                RpsStrides st) {
+    const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
+    const int64_t P = A.plen[s];
+    const uint32_t* sqk = A.sqk + po;
+    const uint32_t* sqv = A.sqv + po;
+    const uint32_t* loc = A.loc + ro;
+    const int32_t* pi = pi_all + po;
+    const int32_t* pj = pj_all + po;
+    const uint32_t lane = threadIdx.x & 63;
+    // whole wavefronts iterate together (the list append is a wavefront operation)
+    for (int64_t e0 = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63u); e0 < P;
+         e0 += (int64_t)gridDim.x * kBlock) {
+        const int64_t e = e0 + lane;
+        bool head = false;
+        uint4 d = make_uint4(0u, 0u, 0u, 0u);
+        if (e < P) {
+            const uint32_t L = sqk[e];
+            if (L < (uint64_t)n) {
+                const uint32_t k = sqv[e];
+                const uint32_t i = (uint32_t)pi[k], j = (uint32_t)pj[k];
+                const uint32_t pb = loc[L];  // the root is the first particle of its run
+                A.rec[po + e] = make_uint2(((loc[i] - pb) & kSlotMask) | ((uint32_t)A.bits[po + k] << 29),
+                                           loc[j] - pb);
+                if (e == 0 || sqk[e - 1] != L) {
+                    head = true;
+                    d = make_uint4((uint32_t)e, A.qend[ro + L] - (uint32_t)e, pb, A.pend[ro + L] - pb);
+                }
+            }
+        }
+        const bool to_narrow = head && d.y <= kNarrowPairs;
+        wave_append(to_narrow, d, A.narrow + ro, A.lists + 4 * s + 0, (uint32_t)st.row);
+        wave_append(head && !to_narrow, d, A.wide + ro, A.lists + 4 * s + 1, (uint32_t)st.row);
     }
 }
 
@@ -392,13 +274,14 @@ rps_fill_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __
 // pairs, the bitmask of earlier lanes that share a particle slot.
 __global__ void __launch_bounds__(64)
 rps_deps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
-    const int64_t s = blockIdx.y;
-    RPS_STEP_VIEW(A, st, s);
+    const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
     const uint32_t lane = threadIdx.x;
-    const uint32_t cnt = lists[1];
+    const uint32_t cnt = A.lists[4 * s + 1];
+    const uint2* rec = A.rec + po;
+    unsigned long long* deps = A.deps + po;
     for (uint32_t c = blockIdx.x; c < cnt; c += gridDim.x) {
-        const uint32_t L = wide[c];
-        const uint32_t qb = qstart[L], np = qstart[L + 1] - qb;
+        const uint4 dsc = A.wide[ro + c];
+        const uint32_t qb = dsc.x, np = dsc.y;
         for (uint32_t w = 0; w < np; w += 64) {
             const bool live = w + lane < np;
             const uint2 r = live ? rec[qb + w + lane] : make_uint2(kUnset, kUnset - 1u);
@@ -423,7 +306,8 @@ __global__ void __launch_bounds__(kBlock)
 rps_play_narrow_kernel(int64_t n, int64_t s, RpsArrays A, RpsStrides st, const int8_t* sp_in, int8_t* sp_out,
                        unsigned ncopy) {
     __shared__ int8_t slots[kBlock * kNarrowSlots];
-    RPS_STEP_VIEW(A, st, s);
+    const int64_t ro = s * st.row, po = s * st.pair;
+    const uint32_t* loc = A.loc + ro;
     if (blockIdx.x < ncopy) {
         // four particles per thread; only particles without pairs are written
         // here (the others belong to a component and are written by its lane)
@@ -443,12 +327,14 @@ rps_play_narrow_kernel(int64_t n, int64_t s, RpsArrays A, RpsStrides st, const i
         return;
     }
     int8_t* my = slots + threadIdx.x * kNarrowSlots;
-    const uint32_t cnt = lists[0];
+    const uint32_t* ids = A.ids + ro;
+    const uint2* rec = A.rec + po;
+    const uint32_t cnt = A.lists[4 * s + 0];
     const uint32_t nblk = gridDim.x - ncopy;
     for (uint32_t c = (blockIdx.x - ncopy) * kBlock + threadIdx.x; c < cnt; c += nblk * kBlock) {
-        const uint32_t L = narrow[c];
-        const uint32_t pb = pstart[L], psz = min(pstart[L + 1] - pb, kNarrowSlots);
-        const uint32_t qb = qstart[L], np = min(qstart[L + 1] - qb, kNarrowPairs);
+        const uint4 dsc = A.narrow[ro + c];
+        const uint32_t qb = dsc.x, np = min(dsc.y, kNarrowPairs);
+        const uint32_t pb = dsc.z, psz = min(dsc.w, kNarrowSlots);
         uint32_t id[kNarrowSlots];
 #pragma unroll
         for (uint32_t t = 0; t < kNarrowSlots; ++t) id[t] = t < psz ? ids[pb + t] : kUnset;
@@ -476,21 +362,25 @@ rps_play_narrow_kernel(int64_t n, int64_t s, RpsArrays A, RpsStrides st, const i
     }
 }
 
-// 64-lane blocks, one wavefront per wide component.
+// 64-lane blocks, one wavefront per wide component.  The block is a single
+// wavefront, whose LDS operations complete in program order, so a round's
+// writes are seen by the next round's reads without a barrier.
 __global__ void __launch_bounds__(64)
 rps_play_wide_kernel(int64_t n, int64_t s, RpsArrays A, RpsStrides st, const int8_t* sp_in, int8_t* sp_out) {
     __shared__ int8_t slots[kWideSlots];
-    RPS_STEP_VIEW(A, st, s);
+    const int64_t ro = s * st.row, po = s * st.pair;
+    const uint32_t* ids = A.ids + ro;
+    const uint2* rec = A.rec + po;
+    const unsigned long long* deps = A.deps + po;
     const uint32_t lane = threadIdx.x;
-    const uint32_t cnt = lists[1];
+    const uint32_t cnt = A.lists[4 * s + 1];
     for (uint32_t c = blockIdx.x; c < cnt; c += gridDim.x) {
-        const uint32_t L = wide[c];
-        const uint32_t pb = pstart[L], psz = pstart[L + 1] - pb;
-        const uint32_t qb = qstart[L], np = qstart[L + 1] - qb;
+        const uint4 dsc = A.wide[ro + c];
+        const uint32_t qb = dsc.x, np = dsc.y, pb = dsc.z, psz = dsc.w;
         const bool in_lds = psz <= kWideSlots;
         // species by slot: LDS when the component fits, else sp_out itself.
         // Four slots per lane per pass with every load issued before the
-        // stores, so a pass costs one memory round trip per dependent level.
+        // stores, so a pass costs one memory round trip.
         if (in_lds) {
             for (uint32_t t0 = lane; t0 < psz; t0 += 256) {
                 uint32_t x[4];
@@ -525,15 +415,23 @@ rps_play_wide_kernel(int64_t n, int64_t s, RpsArrays A, RpsStrides st, const int
             const unsigned long long all = __ballot(live);
             unsigned long long done = 0ull;
             bool mine = live;
-            while (done != all) {
-                const bool ready = mine && (dep & ~done) == 0ull;
-                if (ready) {
-                    if (in_lds) {
+            if (in_lds) {
+                while (done != all) {
+                    const bool ready = mine && (dep & ~done) == 0ull;
+                    if (ready) {
                         int si = slots[a], sj = slots[b];
                         play_pair(si, sj, fb);
                         slots[a] = (int8_t)si;
                         slots[b] = (int8_t)sj;
-                    } else {
+                        mine = false;
+                    }
+                    done |= __ballot(ready);
+                    __builtin_amdgcn_wave_barrier();
+                }
+            } else {
+                while (done != all) {
+                    const bool ready = mine && (dep & ~done) == 0ull;
+                    if (ready) {
                         const uint32_t xa = ids[pb + a], xb = ids[pb + b];
                         if (xa < (uint64_t)n && xb < (uint64_t)n) {
                             int si = sp_out[xa], sj = sp_out[xb];
@@ -541,11 +439,11 @@ rps_play_wide_kernel(int64_t n, int64_t s, RpsArrays A, RpsStrides st, const int
                             sp_out[xa] = (int8_t)si;
                             sp_out[xb] = (int8_t)sj;
                         }
+                        mine = false;
                     }
-                    mine = false;
+                    done |= __ballot(ready);
+                    __syncthreads();  // this round's stores before the next round's loads
                 }
-                done |= __ballot(ready);
-                __syncthreads();  // this round's writes before the next round's reads
             }
         }
         if (in_lds) {
@@ -565,28 +463,29 @@ rps_play_wide_kernel(int64_t n, int64_t s, RpsArrays A, RpsStrides st, const int
 // Component size statistics of one prepared step (diagnostic, on request):
 // stats = {components, max pairs in one, max particles in one, total pairs}.
 __global__ void __launch_bounds__(kBlock)
-rps_stats_kernel(int64_t n, int64_t s, RpsArrays A, RpsStrides st, int64_t* __restrict__ stats) {
-    RPS_STEP_VIEW(A, st, s);
-    const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    uint32_t active = 0, np = 0, psz = 0;
-    if (r < n && loc[r] != kUnset && label[r] == (uint32_t)r) {
-        active = 1;
-        np = qstart[r + 1] - qstart[r];
-        psz = pstart[r + 1] - pstart[r];
+rps_stats_kernel(int64_t s, RpsArrays A, RpsStrides st, int64_t* __restrict__ stats) {
+    const int64_t ro = s * st.row;
+    const uint32_t c0 = A.lists[4 * s + 0], c1 = A.lists[4 * s + 1];
+    uint32_t active = 0, np_sum = 0, np_max = 0, psz_max = 0;
+    for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < c0 + c1; c += gridDim.x * kBlock) {
+        const uint4 d = c < c0 ? A.narrow[ro + c] : A.wide[ro + (c - c0)];
+        active += 1;
+        np_sum += d.y;
+        np_max = max(np_max, d.y);
+        psz_max = max(psz_max, d.w);
     }
-    uint32_t s_active = active, s_np = np, x_np = np, x_p = psz;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
-        s_active += __shfl_xor(s_active, d, 64);
-        s_np += __shfl_xor(s_np, d, 64);
-        x_np = max(x_np, __shfl_xor(x_np, d, 64));
-        x_p = max(x_p, __shfl_xor(x_p, d, 64));
+        active += __shfl_xor(active, d, 64);
+        np_sum += __shfl_xor(np_sum, d, 64);
+        np_max = max(np_max, __shfl_xor(np_max, d, 64));
+        psz_max = max(psz_max, __shfl_xor(psz_max, d, 64));
     }
-    if ((threadIdx.x & 63) == 0 && s_active) {
-        atomicAdd((unsigned long long*)&stats[0], (unsigned long long)s_active);
-        atomicMax((unsigned long long*)&stats[1], (unsigned long long)x_np);
-        atomicMax((unsigned long long*)&stats[2], (unsigned long long)x_p);
-        atomicAdd((unsigned long long*)&stats[3], (unsigned long long)s_np);
+    if ((threadIdx.x & 63) == 0 && active) {
+        atomicAdd((unsigned long long*)&stats[0], (unsigned long long)active);
+        atomicMax((unsigned long long*)&stats[1], (unsigned long long)np_max);
+        atomicMax((unsigned long long*)&stats[2], (unsigned long long)psz_max);
+        atomicAdd((unsigned long long*)&stats[3], (unsigned long long)np_sum);
     }
 }
 
@@ -602,12 +501,11 @@ struct lm_rps_ws {
     int64_t row = 0;          // per-step stride of per-particle arrays
     int64_t pair_cap = 0;     // per-step stride of per-pair arrays
     lm::RpsArrays A{};
-    uint32_t* scan_tmp = nullptr;
+    lm::SortScratch sort{};
     int64_t bytes = 0, pair_bytes = 0;
     int64_t last_n = 0;       // particles of the last prepare
     int32_t last_steps = 0;   // steps of the last prepare
     int64_t pair_stride_used = 0;  // per-step pair stride of the last prepare
-    bool dirty = false;       // a call stopped midway: counters must be re-zeroed
 };
 
 template <typename T>
@@ -623,13 +521,17 @@ static int rws_alloc(T** p, int64_t count, int64_t* bytes) {
 }
 
 static void rps_free_pairs(lm_rps_ws* w) {
-    (void)hipFree(w->A.bits);
-    (void)hipFree(w->A.rec);
-    (void)hipFree(w->A.deps);
-    w->A.bits = nullptr;
-    w->A.rec = nullptr;
-    w->A.deps = nullptr;
-    w->bytes -= w->pair_bytes;
+    lm::RpsArrays& A = w->A;
+    (void)hipFree(A.bits);
+    (void)hipFree(A.rec);
+    (void)hipFree(A.deps);
+    for (uint32_t* p : {A.qkey, A.qval, A.qtk, A.qtv}) (void)hipFree(p);
+    A.bits = nullptr;
+    A.rec = nullptr;
+    A.deps = nullptr;
+    A.qkey = A.qval = A.qtk = A.qtv = nullptr;
+    w->bytes -= w->pair_bytes + w->sort.bytes;
+    lm::sort_scratch_free(&w->sort);
     w->pair_bytes = 0;
     w->pair_cap = 0;
 }
@@ -637,33 +539,45 @@ static void rps_free_pairs(lm_rps_ws* w) {
 static void rps_ws_free(lm_rps_ws* w) {
     if (!w) return;
     lm::RpsArrays& A = w->A;
-    for (uint32_t* p : {A.parent, A.label, A.loc, A.rowbeg, A.rowend, A.rowoff, A.hcount, A.hstart, A.pcount,
-                        A.pstart, A.qcount, A.qstart, A.members, A.ids, A.narrow, A.wide, A.lists})
-        (void)hipFree(p);
+    for (uint32_t* p : {A.parent, A.loc, A.pend, A.qend, A.pkey, A.pval, A.ptk, A.ptv, A.lists}) (void)hipFree(p);
+    (void)hipFree(A.touched);
+    (void)hipFree(A.narrow);
+    (void)hipFree(A.wide);
+    (void)hipFree(A.plen);
     rps_free_pairs(w);
-    (void)hipFree(w->scan_tmp);
     delete w;
 }
 
-// Per-pair arrays sized for `cap` pairs per step (grown, never shrunk).
+// Per-pair arrays (and the sort scratch) sized for `cap` pairs per step
+// (grown, never shrunk).
 static int rps_ensure_pairs(lm_rps_ws* w, int64_t cap, hipStream_t s) {
-    if (cap <= w->pair_cap) return LM_OK;
+    if (cap <= w->pair_cap && w->sort.hist != nullptr) return LM_OK;
     LM_HIP_CHECK(hipStreamSynchronize(s));
     rps_free_pairs(w);
     const int64_t c = cap + cap / 4 + 1024, S = w->steps_max;
+    lm::RpsArrays& A = w->A;
     int64_t b = 0;
     int rc;
-    if ((rc = rws_alloc(&w->A.bits, S * c, &b)) || (rc = rws_alloc(&w->A.rec, S * c, &b)) ||
-        (rc = rws_alloc(&w->A.deps, S * c, &b))) {
+    if ((rc = rws_alloc(&A.bits, S * c, &b)) || (rc = rws_alloc(&A.rec, S * c, &b)) ||
+        (rc = rws_alloc(&A.deps, S * c, &b)) || (rc = rws_alloc(&A.qkey, S * c, &b)) ||
+        (rc = rws_alloc(&A.qval, S * c, &b)) || (rc = rws_alloc(&A.qtk, S * c, &b)) ||
+        (rc = rws_alloc(&A.qtv, S * c, &b)) ||
+        (rc = lm::sort_scratch_alloc(&w->sort, c > w->n_max ? c : w->n_max, w->steps_max))) {
         w->bytes += b;
         w->pair_bytes = b;
         rps_free_pairs(w);
         return rc;
     }
-    w->bytes += b;
+    w->bytes += b + w->sort.bytes;
     w->pair_bytes = b;
     w->pair_cap = c;
     return LM_OK;
+}
+
+static int key_bits_for(int64_t n) {  // bits holding every value in [0, n]
+    int b = 1;
+    while (b < 32 && ((int64_t)1 << b) <= n) ++b;
+    return b;
 }
 
 extern "C" {
@@ -680,23 +594,17 @@ int lm_rps_ws_create(int64_t n_max, int32_t steps_max, lm_rps_ws** out) {
     const int64_t S = steps_max, R = S * w->row;
     lm::RpsArrays& A = w->A;
     int rc;
-    if ((rc = rws_alloc(&A.parent, R, &w->bytes)) || (rc = rws_alloc(&A.label, R, &w->bytes)) ||
-        (rc = rws_alloc(&A.loc, R, &w->bytes)) || (rc = rws_alloc(&A.rowbeg, R, &w->bytes)) ||
-        (rc = rws_alloc(&A.rowend, R, &w->bytes)) || (rc = rws_alloc(&A.rowoff, R, &w->bytes)) ||
-        (rc = rws_alloc(&A.hcount, R, &w->bytes)) || (rc = rws_alloc(&A.hstart, R, &w->bytes)) ||
-        (rc = rws_alloc(&A.pcount, R, &w->bytes)) || (rc = rws_alloc(&A.pstart, R, &w->bytes)) ||
-        (rc = rws_alloc(&A.qcount, R, &w->bytes)) || (rc = rws_alloc(&A.qstart, R, &w->bytes)) ||
-        (rc = rws_alloc(&A.members, R, &w->bytes)) || (rc = rws_alloc(&A.ids, R, &w->bytes)) ||
-        (rc = rws_alloc(&A.narrow, R, &w->bytes)) || (rc = rws_alloc(&A.wide, R, &w->bytes)) ||
-        (rc = rws_alloc(&A.lists, 4 * S, &w->bytes)) ||
-        (rc = rws_alloc(&w->scan_tmp, S * lm::scan_num_blocks(n_max), &w->bytes))) {
+    if ((rc = rws_alloc(&A.parent, R, &w->bytes)) || (rc = rws_alloc(&A.loc, R, &w->bytes)) ||
+        (rc = rws_alloc(&A.pend, R, &w->bytes)) || (rc = rws_alloc(&A.qend, R, &w->bytes)) ||
+        (rc = rws_alloc(&A.touched, R, &w->bytes)) || (rc = rws_alloc(&A.pkey, R, &w->bytes)) ||
+        (rc = rws_alloc(&A.pval, R, &w->bytes)) || (rc = rws_alloc(&A.ptk, R, &w->bytes)) ||
+        (rc = rws_alloc(&A.ptv, R, &w->bytes)) || (rc = rws_alloc(&A.narrow, R, &w->bytes)) ||
+        (rc = rws_alloc(&A.wide, R, &w->bytes)) || (rc = rws_alloc(&A.lists, 4 * S, &w->bytes)) ||
+        (rc = rws_alloc(&A.plen, S, &w->bytes))) {
         rps_ws_free(w);
         return rc;
     }
-    hipError_t e = hipMemset(A.hcount, 0, sizeof(uint32_t) * (size_t)R);
-    if (e == hipSuccess) e = hipMemset(A.pcount, 0, sizeof(uint32_t) * (size_t)R);
-    if (e == hipSuccess) e = hipMemset(A.qcount, 0, sizeof(uint32_t) * (size_t)R);
-    if (e == hipSuccess) e = hipMemset(A.lists, 0, sizeof(uint32_t) * (size_t)(4 * S));
+    hipError_t e = hipMemset(A.lists, 0, sizeof(uint32_t) * (size_t)(4 * S));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         lm::set_error("workspace init failed: %s", hipGetErrorString(e));
@@ -733,39 +641,36 @@ int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair
     // holds steps_max * pair_cap >= steps_max * pair_stride entries.
     LM_REQUIRE(pair_stride <= ws->pair_cap || pair_stride == 0, LM_ERR_ARG, "internal: pair capacity");
     const lm::RpsStrides sp{ws->row, pair_stride};
-    const lm::RpsArrays& A = ws->A;
-    const int64_t R = (int64_t)ws->steps_max * ws->row;
-    if (ws->dirty) {
-        LM_HIP_CHECK(hipMemsetAsync(A.hcount, 0, sizeof(uint32_t) * (size_t)R, s));
-        LM_HIP_CHECK(hipMemsetAsync(A.pcount, 0, sizeof(uint32_t) * (size_t)R, s));
-        LM_HIP_CHECK(hipMemsetAsync(A.qcount, 0, sizeof(uint32_t) * (size_t)R, s));
-        ws->dirty = false;
-    }
-    ws->dirty = true;  // until every launch below went through
+    lm::RpsArrays& A = ws->A;
     const unsigned S = (unsigned)nsteps;
-    LM_LAUNCH(lm::K_RPS_INIT, lm::rps_init_kernel, dim3(lm::grid_for(n), S), dim3(lm::kBlock), 0, s, n, A, sp);
+    const unsigned nbp = lm::grid_for(n);
+    LM_LAUNCH(lm::K_RPS_INIT, lm::rps_init_kernel, dim3(nbp, S), dim3(lm::kBlock), 0, s, n, A, sp);
     LM_LAUNCH_CHECK();
     if (pair_stride > 0) {
         LM_LAUNCH(lm::K_RPS_UNION, lm::rps_union_kernel, dim3(lm::kPairGrid, S), dim3(lm::kBlock), 0, s, n,
                   pair_i, pair_j, npairs, A, sp, pRS, pPR, pSP, key0, key1, step0);
         LM_LAUNCH_CHECK();
-        LM_LAUNCH(lm::K_RPS_HEADS, lm::rps_claim_kernel, dim3(lm::kPairGrid, S), dim3(lm::kBlock), 0, s, n, pair_i,
-                  pair_j, npairs, A, sp);
+        LM_LAUNCH(lm::K_RPS_KEYS, lm::rps_keys_kernel, dim3(nbp + lm::kPairGrid, S), dim3(lm::kBlock), 0, s, n,
+                  pair_i, pair_j, A, sp, nbp);
         LM_LAUNCH_CHECK();
-        if ((rc = lm::exclusive_scan_u32_batched(A.hcount, n, ws->row, A.hstart, ws->row, nsteps, ws->scan_tmp,
-                                                 nullptr, s)) ||
-            (rc = lm::exclusive_scan_u32_batched(A.pcount, n, ws->row, A.pstart, ws->row, nsteps, ws->scan_tmp,
-                                                 nullptr, s)) ||
-            (rc = lm::exclusive_scan_u32_batched(A.qcount, n, ws->row, A.qstart, ws->row, nsteps, ws->scan_tmp,
-                                                 nullptr, s)))
+        const int kb = key_bits_for(n);
+        int res = 0;
+        uint32_t* pk[2] = {A.pkey, A.ptk};
+        uint32_t* pv[2] = {A.pval, A.ptv};
+        if ((rc = lm::radix_sort_u32(pk, pv, ws->row, nsteps, nullptr, n, n, kb, &ws->sort, s, &res))) return rc;
+        A.spk = pk[res];
+        A.ids = pv[res];
+        uint32_t* qk[2] = {A.qkey, A.qtk};
+        uint32_t* qv[2] = {A.qval, A.qtv};
+        if ((rc = lm::radix_sort_u32(qk, qv, pair_stride, nsteps, A.plen, 0, pair_stride, kb, &ws->sort, s, &res)))
             return rc;
-        LM_LAUNCH(lm::K_RPS_SCATTER, lm::rps_group_kernel, dim3(lm::kPairGrid + lm::grid_for(n) / 4 + 1, S),
-                  dim3(lm::kBlock), 0, s, n, pair_i, npairs, A, sp);
-        LM_LAUNCH_CHECK();
-        LM_LAUNCH(lm::K_RPS_ORDER, lm::rps_order_kernel, dim3(lm::kCompGrid + 1024, S), dim3(64), 0, s, n, A, sp);
+        A.sqk = qk[res];
+        A.sqv = qv[res];
+        LM_LAUNCH(lm::K_RPS_SEGMENTS, lm::rps_segments_kernel, dim3(nbp + lm::kPairGrid, S), dim3(lm::kBlock), 0,
+                  s, n, A, sp, nbp);
         LM_LAUNCH_CHECK();
         LM_LAUNCH(lm::K_RPS_FILL, lm::rps_fill_kernel, dim3(lm::kPairGrid, S), dim3(lm::kBlock), 0, s, n, pair_i,
-                  pair_j, npairs, A, sp);
+                  pair_j, A, sp);
         LM_LAUNCH_CHECK();
         LM_LAUNCH(lm::K_RPS_DEPS, lm::rps_deps_kernel, dim3(1024, S), dim3(64), 0, s, n, A, sp);
         LM_LAUNCH_CHECK();
@@ -775,7 +680,6 @@ int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair
     }
     ws->last_n = n;
     ws->last_steps = nsteps;
-    ws->dirty = false;
     return LM_OK;
 }
 
@@ -793,9 +697,11 @@ int lm_rps_play(lm_rps_ws* ws, int32_t step, int64_t n, const int8_t* species_in
     LM_LAUNCH(lm::K_RPS_PLAY, lm::rps_play_narrow_kernel, dim3(ncopy + 512), dim3(lm::kBlock), 0, s, n,
               (int64_t)step, ws->A, sp, species_in, species_out, ncopy);
     LM_LAUNCH_CHECK();
-    LM_LAUNCH(lm::K_RPS_PLAY_WIDE, lm::rps_play_wide_kernel, dim3(1024), dim3(64), 0, s, n, (int64_t)step, ws->A,
-              sp, species_in, species_out);
-    LM_LAUNCH_CHECK();
+    if (ws->pair_stride_used > 0) {
+        LM_LAUNCH(lm::K_RPS_PLAY_WIDE, lm::rps_play_wide_kernel, dim3(lm::kWideBlocks), dim3(64), 0, s, n,
+                  (int64_t)step, ws->A, sp, species_in, species_out);
+        LM_LAUNCH_CHECK();
+    }
     return LM_OK;
 }
 
@@ -815,8 +721,8 @@ int lm_rps_component_stats(lm_rps_ws* ws, int32_t step, int64_t* out4, void* str
     LM_LAUNCH_CHECK();
     if (ws->last_n > 0 && ws->last_steps > 0 && ws->pair_stride_used > 0) {
         lm::RpsStrides sp{ws->row, ws->pair_stride_used};
-        hipLaunchKernelGGL(lm::rps_stats_kernel, dim3(lm::grid_for(ws->last_n)), dim3(lm::kBlock), 0, s,
-                           ws->last_n, (int64_t)step, ws->A, sp, out4);
+        hipLaunchKernelGGL(lm::rps_stats_kernel, dim3(256), dim3(lm::kBlock), 0, s, (int64_t)step, ws->A, sp,
+                           out4);
         LM_LAUNCH_CHECK();
     }
     return LM_OK;

@@ -120,6 +120,24 @@ def copy_bytes(src, dst, nbytes):
     check(lib().lm_copy_bytes(ptr(src), ptr(dst), int(nbytes), stream_ptr()))
 
 
+def sort_pairs_u32(keys, vals, key_bits=32, seg_len=None):
+    """Stable sort of each row of int32 ``keys`` [nseg, L] (as uint32) with
+    ``vals``, in place; row g sorts its first ``seg_len[g]`` entries."""
+    if not (keys.dtype == torch.int32 and vals.dtype == torch.int32 and keys.shape == vals.shape
+            and keys.is_cuda and vals.is_cuda and keys.is_contiguous() and vals.is_contiguous()):
+        raise ValueError("keys and vals must be contiguous int32 device tensors of one shape")
+    k2 = keys if keys.dim() == 2 else keys.view(1, -1)
+    v2 = vals if vals.dim() == 2 else vals.view(1, -1)
+    nseg, L = k2.shape
+    kt, vt = torch.empty_like(k2), torch.empty_like(v2)
+    sl = None
+    if seg_len is not None:
+        sl = seg_len.to(device=k2.device, dtype=torch.int64).contiguous()
+    check(lib().lm_sort_pairs_u32(nseg, ptr(sl) if sl is not None else None, L, L, L, int(key_bits),
+                                  ptr(k2), ptr(v2), ptr(kt), ptr(vt), stream_ptr()))
+    return keys, vals
+
+
 # ---------------------------------------------------------------------------
 # pair search
 # ---------------------------------------------------------------------------

@@ -266,3 +266,28 @@ def test_rps_workspace_reuse(gpu):
         sp = rng.integers(1, 4, n).astype(np.int8)
         g, _ = gpu_rps(sp, pi, pj, 0.6, 0.5, 0.4, (2, 2), step, updater=up)
         assert np.array_equal(g, oracle.rps_sequential(sp, pi, pj, 0.6, 0.5, 0.4, (2, 2), step))
+
+
+# ---------------------------------------------------------------- radix sort
+@pytest.mark.parametrize("L,bits,maxkey", [(1, 32, 2 ** 32), (2047, 8, 7), (2048, 16, 2 ** 16), (2049, 24, 300),
+                                           (70001, 32, 2 ** 32), (300000, 20, 2 ** 20), (5000, 19, 2 ** 32),
+                                           (100000, 10, 2 ** 32), (9000, 9, 2 ** 9)])
+def test_radix_sort_stable_segments(gpu, L, bits, maxkey):
+    rng = np.random.default_rng(L + bits)
+    nseg = 3
+    keys = rng.integers(0, maxkey, size=(nseg, L), dtype=np.uint64).astype(np.uint32)
+    vals = np.tile(np.arange(L, dtype=np.uint32), (nseg, 1))
+    lens = np.array([L, max(L // 3, 0), L - 1 if L > 1 else 0], dtype=np.int64)
+    k = T(keys.view(np.int32))
+    v = T(vals.view(np.int32))
+    dev.sort_pairs_u32(k, v, key_bits=bits, seg_len=T(lens))
+    gk = k.cpu().numpy().view(np.uint32)
+    gv = v.cpu().numpy().view(np.uint32)
+    mask = np.uint32((1 << bits) - 1) if bits < 32 else np.uint32(0xFFFFFFFF)
+    for g in range(nseg):
+        n = int(lens[g])
+        order = np.argsort(keys[g, :n] & mask, kind="stable")
+        assert np.array_equal(gk[g, :n], keys[g, :n][order])
+        assert np.array_equal(gv[g, :n], vals[g, :n][order])
+        # entries past the segment length are left alone
+        assert np.array_equal(gk[g, n:], keys[g, n:])
