@@ -146,7 +146,9 @@ int64_t lm_rps_ws_bytes(const lm_rps_ws* ws);
  * pRS, paper over rock with pPR, scissors over paper with pSP) wins iff
  * u < p, and the loser takes the winner's species.  Species outside {1,2,3}
  * never interact.  *npairs is read on the device; if it exceeds capacity the
- * call changes nothing.
+ * call changes nothing.  The list must be sorted by (i, j), as lm_query_pairs
+ * returns it (the reference visits sorted(set(pairs))); pairs with an index
+ * outside [0, n) are skipped.
  * Replaces the per-pair loop interaction_simulator.py:104-105 calling
  * rock_paper_scissors_interaction (interactions.py:13-40). */
 int lm_rps_update(lm_rps_ws* ws, int64_t n, int8_t* species, const int32_t* pair_i,

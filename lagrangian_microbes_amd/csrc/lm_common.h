@@ -57,8 +57,8 @@ void set_error(const char* fmt, ...);
 enum KernelId {
     K_ADVECT = 0,
     K_BLEND,
-    K_BIN_COUNT,
-    K_BIN_SCATTER,
+    K_BIN_KEYS,
+    K_BIN_TABLE,
     K_PAIR_COUNT,
     K_PAIR_EMIT,
     K_SCAN_REDUCE,
@@ -67,6 +67,7 @@ enum KernelId {
     K_RPS_UNION,
     K_RPS_KEYS,
     K_RPS_SEGMENTS,
+    K_RPS_COMPS,
     K_RPS_PLAY,
     K_RPS_PLAY_WIDE,
     K_RPS_FILL,

@@ -23,8 +23,8 @@ void set_error(const char* fmt, ...) {
 // per-kernel timing
 // ---------------------------------------------------------------------------
 static const char* const kKernelNames[K_NUM_KERNELS] = {
-    "advect_rk4_f64", "field_blend_f64", "bin_count", "bin_scatter", "pair_count", "pair_emit",
-    "scan_reduce",    "scan_apply",      "rps_init",  "rps_union",   "rps_keys",   "rps_segments",
+    "advect_rk4_f64", "field_blend_f64", "bin_keys", "bin_table", "pair_count", "pair_emit",
+    "scan_reduce",    "scan_apply",      "rps_init",  "rps_union",   "rps_keys",   "rps_segments", "rps_comps",
     "rps_play",       "rps_play_wide",   "rps_fill",  "rps_deps",    "philox",
     "copy16", "sort_hist", "sort_scatter"};
 

@@ -1,12 +1,14 @@
 // Fixed-radius pair search in lon/lat degree space, for one or many steps.
 //
-// Cell list with cells of side h >= radius over a fixed box, stored as a
-// counting sort into T = 2^k buckets (bucket = dense cell id mod T, so the
-// three cells of a stencil row are three consecutive buckets; when the box has
-// fewer than T cells the table is dense and nothing aliases):
-//   1. bin_count    : bucket counts (atomics; the scatter returns them to 0)
-//   2. scan         : bucket starts
-//   3. bin_scatter  : records {x, y, index} grouped by bucket
+// Cell list with cells of side h >= radius over a fixed box, stored as a sort
+// of the particles by bucket over T = 2^k buckets (bucket = dense cell id mod
+// T, so the three cells of a stencil row are three consecutive buckets; when
+// the box has fewer than T cells the table is dense and nothing aliases):
+//   1. bin_keys     : each particle's bucket
+//   2. sort         : stable radix sort of (bucket, index)
+//   3. bin_table    : bucket starts (every bucket between two neighbouring
+//                     sorted keys starts at the later one) and records
+//                     {x, y, index} in bucket order
 //   4. pair_count   : per particle i, the neighbours j > i within radius over
 //                     the 3x3 cell stencil (run in bucket order for locality);
 //                     up to kNbr of them are kept in a scratch row
@@ -14,17 +16,17 @@
 //   6. pair_emit    : per particle, its neighbours sorted ascending, written at
 //                     its row offset; rows with more than kNbr neighbours are
 //                     re-searched kNbr at a time.  The result is the (i, j)
-//                     lexicographic pair list, independent of the atomic order
-//                     inside a bucket.
+//                     lexicographic pair list.
 // A batch of S steps (S position snapshots) runs every phase once with
 // blockIdx.y = step, each step with its own tables and pair region, so a
-// launch carries S times the parallelism of one step.
+// launch carries S times the parallelism of one step.  No global atomics.
 // Distances use the float32 positions widened to double, with the same
 // operations cKDTree applies: dx*dx + dy*dy <= r*r for p = 2.  Since a
 // difference of two nearby floats is exact in double and its square is exact,
 // the test is independent of FMA contraction.
 #include "lm_common.h"
 #include "lm_scan.h"
+#include "lm_sort.h"
 
 #include <cmath>
 
@@ -52,7 +54,7 @@ struct Metric {
 struct Strides {
     int64_t pos;    // between position snapshots
     int64_t tab;    // bucket tables (>= T + 1, multiple of 4)
-    int64_t rec;    // bucket-ordered records
+    int64_t rec;    // bucket-ordered records and sort buffers
     int64_t row;    // per-particle counts / offsets (>= n + 1, multiple of 4)
     int64_t nbr;    // neighbour scratch rows
     int64_t pair;   // pair list capacity per step
@@ -110,28 +112,37 @@ __device__ __forceinline__ void for_stencil(const CellGrid& g, const uint32_t* _
 }
 
 __global__ void __launch_bounds__(kBlock)
-bin_count_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y, CellGrid g, Strides st,
-                 uint32_t* __restrict__ count) {
+bin_keys_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y, CellGrid g, Strides st,
+                uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const int64_t s = blockIdx.y;
     int cx, cy;
     cell_of(g, x[s * st.pos + i], y[s * st.pos + i], &cx, &cy);
-    atomicAdd(&count[s * st.tab + bucket_of(g, cx, cy)], 1u);
+    keys[s * st.rec + i] = bucket_of(g, cx, cy);
+    vals[s * st.rec + i] = (uint32_t)i;
 }
 
+// Over the sorted (bucket, index): bucket starts and records.  Position e
+// starts every bucket in (key[e-1], key[e]]; the last position also ends the
+// buckets after its own, so all T + 1 entries are written.
 __global__ void __launch_bounds__(kBlock)
-bin_scatter_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y, CellGrid g, Strides st,
-                   const uint32_t* __restrict__ start, uint32_t* __restrict__ count, float4* __restrict__ recs) {
-    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+bin_table_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y, CellGrid g, Strides st,
+                 const uint32_t* __restrict__ skeys, const uint32_t* __restrict__ svals,
+                 uint32_t* __restrict__ start_all, float4* __restrict__ recs) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= n) return;
     const int64_t s = blockIdx.y;
-    const float xf = x[s * st.pos + i], yf = y[s * st.pos + i];
-    int cx, cy;
-    cell_of(g, xf, yf, &cx, &cy);
-    const uint32_t b = bucket_of(g, cx, cy);
-    const uint32_t slot = start[s * st.tab + b] + atomicSub(&count[s * st.tab + b], 1u) - 1u;
-    if (slot < (uint64_t)n) recs[s * st.rec + slot] = make_float4(xf, yf, __uint_as_float((uint32_t)i), 0.0f);
+    const uint32_t* key = skeys + s * st.rec;
+    uint32_t* start = start_all + s * st.tab;
+    const uint32_t b = min(key[e], g.T - 1u);
+    const uint32_t from = e == 0 ? 0u : min(key[e - 1], g.T - 1u) + 1u;
+    for (uint32_t q = from; q <= b; ++q) start[q] = (uint32_t)e;
+    if (e + 1 == n)
+        for (uint32_t q = b + 1; q <= g.T; ++q) start[q] = (uint32_t)n;
+    const uint32_t i = svals[s * st.rec + e];
+    if (i < (uint64_t)n)
+        recs[s * st.rec + e] = make_float4(x[s * st.pos + i], y[s * st.pos + i], __uint_as_float(i), 0.0f);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -264,7 +275,10 @@ struct lm_pairs_ws {
     double max_radius = 0.0;
     lm::CellGrid grid{};
     int64_t tab_stride = 0, row_stride = 0;
-    uint32_t* count = nullptr;     // [steps][tab]   zero between calls
+    uint32_t* keys[2] = {nullptr, nullptr};  // [steps][n_max] sort buffers
+    uint32_t* vals[2] = {nullptr, nullptr};
+    lm::SortScratch sort{};
+    int key_bits = 0;              // log2(T)
     uint32_t* start = nullptr;     // [steps][tab]
     uint32_t* scan_tmp = nullptr;  // block sums
     float4* recs = nullptr;        // [steps][n_max]
@@ -272,12 +286,15 @@ struct lm_pairs_ws {
     uint32_t* nbr = nullptr;       // [steps][n_max * kNbr]
     uint32_t* row_off = nullptr;   // [steps][row]
     int64_t bytes = 0;
-    bool dirty = false;            // a call stopped midway: counts must be re-zeroed
 };
 
 static void pairs_ws_free(lm_pairs_ws* w) {
     if (!w) return;
-    (void)hipFree(w->count);
+    for (int q = 0; q < 2; ++q) {
+        (void)hipFree(w->keys[q]);
+        (void)hipFree(w->vals[q]);
+    }
+    lm::sort_scratch_free(&w->sort);
     (void)hipFree(w->start);
     (void)hipFree(w->scan_tmp);
     (void)hipFree(w->recs);
@@ -350,7 +367,9 @@ int lm_pairs_ws_create(int64_t n_max, int32_t steps_max, double lon_min, double 
     const int64_t nb2 = lm::scan_num_blocks(n_max);
     if (nb2 > nb) nb = nb2;
     int rc = LM_OK;
-    if ((rc = ws_alloc(&w->count, S * w->tab_stride, &w->bytes)) ||
+    if ((rc = ws_alloc(&w->keys[0], S * n_max, &w->bytes)) || (rc = ws_alloc(&w->keys[1], S * n_max, &w->bytes)) ||
+        (rc = ws_alloc(&w->vals[0], S * n_max, &w->bytes)) || (rc = ws_alloc(&w->vals[1], S * n_max, &w->bytes)) ||
+        (rc = lm::sort_scratch_alloc(&w->sort, n_max, steps_max)) ||
         (rc = ws_alloc(&w->start, S * w->tab_stride, &w->bytes)) ||
         (rc = ws_alloc(&w->scan_tmp, S * nb, &w->bytes)) || (rc = ws_alloc(&w->recs, S * n_max, &w->bytes)) ||
         (rc = ws_alloc(&w->nbr_cnt, S * w->row_stride, &w->bytes)) ||
@@ -359,13 +378,8 @@ int lm_pairs_ws_create(int64_t n_max, int32_t steps_max, double lon_min, double 
         pairs_ws_free(w);
         return rc;
     }
-    hipError_t e = hipMemset(w->count, 0, sizeof(uint32_t) * (size_t)(S * w->tab_stride));
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e != hipSuccess) {
-        lm::set_error("workspace init failed: %s", hipGetErrorString(e));
-        pairs_ws_free(w);
-        return LM_ERR_HIP;
-    }
+    w->bytes += w->sort.bytes;
+    while (((uint32_t)1 << w->key_bits) < T) ++w->key_bits;
     *out = w;
     return LM_OK;
 }
@@ -423,19 +437,14 @@ int lm_query_pairs_batch(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float
     st.nbr = ws->n_max * lm::kNbr;
     st.pair = pair_stride;
     const dim3 grid(lm::grid_for(n), (unsigned)nsteps);
-    if (ws->dirty) {
-        LM_HIP_CHECK(hipMemsetAsync(ws->count, 0, sizeof(uint32_t) * (size_t)(ws->steps_max * ws->tab_stride), s));
-        ws->dirty = false;
-    }
-    ws->dirty = true;  // until every launch below went through
-
-    LM_LAUNCH(lm::K_BIN_COUNT, lm::bin_count_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, g, st, ws->count);
+    LM_LAUNCH(lm::K_BIN_KEYS, lm::bin_keys_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, g, st, ws->keys[0],
+              ws->vals[0]);
     LM_LAUNCH_CHECK();
-    int rc = lm::exclusive_scan_u32_batched(ws->count, (int64_t)g.T, st.tab, ws->start, st.tab, nsteps,
-                                            ws->scan_tmp, nullptr, s);
+    int res = 0;
+    int rc = lm::radix_sort_u32(ws->keys, ws->vals, st.rec, nsteps, nullptr, n, n, ws->key_bits, &ws->sort, s, &res);
     if (rc) return rc;
-    LM_LAUNCH(lm::K_BIN_SCATTER, lm::bin_scatter_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, g, st,
-              (const uint32_t*)ws->start, ws->count, ws->recs);
+    LM_LAUNCH(lm::K_BIN_TABLE, lm::bin_table_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, g, st,
+              (const uint32_t*)ws->keys[res], (const uint32_t*)ws->vals[res], ws->start, ws->recs);
     LM_LAUNCH_CHECK();
     LM_LAUNCH(lm::K_PAIR_COUNT, lm::pair_count_kernel, grid, dim3(lm::kBlock), 0, s, n, g, m, st,
               (const uint32_t*)ws->start, (const float4*)ws->recs, ws->nbr_cnt, ws->nbr);
@@ -447,7 +456,6 @@ int lm_query_pairs_batch(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float
               (const uint32_t*)ws->start, (const float4*)ws->recs, (const uint32_t*)ws->nbr_cnt,
               (const uint32_t*)ws->nbr, (const uint32_t*)ws->row_off, pair_i, pair_j);
     LM_LAUNCH_CHECK();
-    ws->dirty = false;
     return LM_OK;
 }
 

@@ -7,26 +7,31 @@
 //
 // Everything that depends only on the pairs runs once for a batch of steps
 // (blockIdx.y = step), with the parallelism of all steps at once:
-//   prepare 1. init      parent[v] = v; touched marks and slots cleared
+//   prepare 1. init      parent[v] = v; touched marks, slots and rows cleared
 //           2. union     lock-free union-find, hooking the larger root under the
 //                        smaller, so every root is the smallest index of its
 //                        component; the same pass draws each pair's uniform
 //                        (Philox keyed by pair rank and step), keeps its three
-//                        comparisons with pRS, pPR, pSP as bits and marks both
-//                        particles as touched
-//           3. keys      sort keys: each touched particle's root (untouched
-//                        particles: n, after every root) and each pair's root
-//           4. sorts     stable radix sorts of particles and of pairs by root:
-//                        a component's particles become one run in index
-//                        order -- its root first -- and its pairs one run in
-//                        pair order
-//           5. segments  each particle's position in the sorted particles (its
-//                        slot is that minus its root's position) and the end of
-//                        every component's runs
-//           6. fill      pair records (slot_i | bits << 29, slot_j) in run
-//                        order; each component listed once, as small (<= 32
-//                        pairs) or wide, with its two runs
-//           7. deps      for the wide components, per 64-pair window, the
+//                        comparisons with pRS, pPR, pSP as bits, marks both
+//                        particles as touched and records where each row (the
+//                        run of pairs with one i) begins and ends
+//           3. keys      each touched particle's root (untouched: n, after
+//                        every root)
+//           4. sort      stable radix sort of the particles by root: a
+//                        component's particles become one run in index order,
+//                        its root first
+//           5. segments  each particle's position in that order, the end of
+//                        every run, and the row lengths in that order
+//           6. scan      row offsets in that order.  Pairs are sorted by
+//                        (i, j), so a component's pairs in sequential order are
+//                        the rows of its particles in index order: the scan
+//                        places every component's pairs in one run, in order
+//           7. comps     per particle its slot (position minus its root's) and
+//                        the shift from pair rank to run position; each
+//                        component listed once, as small (<= 32 pairs) or wide
+//           8. fill      pair records (slot_i | bits << 29, slot_j) at their
+//                        run positions
+//           9. deps      for the wide components, per 64-pair window, the
 //                        bitmask of earlier pairs of the window sharing a slot
 //   No global atomics but the union's CAS and one add per wavefront for the
 //   component lists.
@@ -51,6 +56,7 @@ constexpr uint32_t kWideSlots = 16384;               // LDS species slots of the
 constexpr uint32_t kSlotMask = (1u << 29) - 1u;
 constexpr unsigned kPairGrid = 256;                  // blocks per step for pair-parallel passes
 constexpr unsigned kWideBlocks = 4096;               // one-wavefront blocks of the wide replay
+constexpr int kCompItems = 8;                        // sorted positions per thread when listing components
 
 // Per-step strides (elements) of a batch.
 struct RpsStrides {
@@ -62,18 +68,19 @@ struct RpsStrides {
 // per-pair array at s * pair.  A component is described by a uint4
 // {first pair, pairs, first particle, particles} of its runs.
 struct RpsArrays {
-    uint32_t *parent, *loc, *pend, *qend;  // per particle
-    uint8_t* touched;                      // per particle
-    uint32_t *pkey, *pval, *ptk, *ptv;     // particle sort buffers (per particle)
-    uint32_t *qkey, *qval, *qtk, *qtv;     // pair sort buffers (per pair)
-    const uint32_t *spk, *ids;             // sorted particle roots / indices (two of the above)
-    const uint32_t *sqk, *sqv;             // sorted pair roots / ranks
-    uint4 *narrow, *wide;                  // component lists (per particle)
-    uint32_t* lists;                       // [step][4]: narrow count, wide count
-    int64_t* plen;                         // [step]: pairs taken (0 for an overflowed step)
-    uint8_t* bits;                         // per pair (by rank)
-    uint2* rec;                            // per pair (run order)
-    unsigned long long* deps;              // per pair (run order, wide components)
+    uint32_t *parent, *loc, *pend;        // per particle
+    uint32_t *rowbeg, *rowend;            // per particle: its row [rowbeg, rowend) of pairs
+    uint32_t *rcnt, *roff;                // per sorted position: row length, run offset
+    uint2* info;                          // per particle: {run position - pair rank, slot}
+    uint8_t* touched;                     // per particle
+    uint32_t *pkey, *pval, *ptk, *ptv;    // particle sort buffers
+    const uint32_t *spk, *ids;            // sorted roots / particle indices (two of the above)
+    uint4 *narrow, *wide;                 // component lists (per particle)
+    uint32_t* lists;                      // [step][4]: narrow count, wide count
+    int64_t* plen;                        // [step]: pairs taken (0 for an overflowed step)
+    uint8_t* bits;                        // per pair (by rank)
+    uint2* rec;                           // per pair (run order)
+    unsigned long long* deps;             // per pair (run order, wide components)
 };
 
 __device__ __forceinline__ uint32_t find_root(const uint32_t* parent_c, uint32_t v) {
@@ -154,6 +161,8 @@ rps_init_kernel(int64_t n, RpsArrays A, RpsStrides st) {
         A.parent[ro + v] = (uint32_t)v;
         A.loc[ro + v] = kUnset;
         A.touched[ro + v] = 0;
+        A.rowbeg[ro + v] = 0;
+        A.rowend[ro + v] = 0;
     }
     if (v < 4) A.lists[4 * s + v] = 0;
 }
@@ -175,98 +184,111 @@ rps_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* _
         const double u =
             u01_from_u64(philox4x64_10((uint64_t)k, step0 + (uint64_t)s, kStreamPairs, 0, k0, k1).v[0]);
         bits[k] = (uint8_t)((u < pRS ? 1u : 0u) | (u < pPR ? 2u : 0u) | (u < pSP ? 4u : 0u));
-        if (a < (uint64_t)n && b < (uint64_t)n) {
+        if (a < (uint64_t)n) {
+            if (k == 0 || (uint32_t)pi[k - 1] != a) A.rowbeg[ro + a] = (uint32_t)k;
+            if (k + 1 == P || (uint32_t)pi[k + 1] != a) A.rowend[ro + a] = (uint32_t)(k + 1);
             touched[a] = 1;
-            touched[b] = 1;
-            if (a != b) unite(parent, a, b);
+            if (b < (uint64_t)n) {
+                touched[b] = 1;
+                if (a != b) unite(parent, a, b);
+            }
         }
     }
 }
 
-// Blocks [0, nbp): particle keys; the rest: pair keys.
 __global__ void __launch_bounds__(kBlock)
-rps_keys_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __restrict__ pj_all, RpsArrays A,
-                RpsStrides st, unsigned nbp) {
-    const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
-    const uint32_t* parent = A.parent + ro;
-    if (blockIdx.x < nbp) {
-        const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-        if (v < n) {
-            A.pkey[ro + v] = A.touched[ro + v] ? find_root(parent, (uint32_t)v) : (uint32_t)n;
-            A.pval[ro + v] = (uint32_t)v;
-        }
-        return;
-    }
-    const int64_t P = A.plen[s];
-    const int32_t* pi = pi_all + po;
-    const int32_t* pj = pj_all + po;
-    for (int64_t k = (int64_t)(blockIdx.x - nbp) * kBlock + threadIdx.x; k < P;
-         k += (int64_t)(gridDim.x - nbp) * kBlock) {
-        const uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
-        A.qkey[po + k] = (a < (uint64_t)n && b < (uint64_t)n) ? find_root(parent, a) : (uint32_t)n;
-        A.qval[po + k] = (uint32_t)k;
+rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st) {
+    const int64_t s = blockIdx.y, ro = s * st.row;
+    const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (v < n) {
+        A.pkey[ro + v] = A.touched[ro + v] ? find_root(A.parent + ro, (uint32_t)v) : (uint32_t)n;
+        A.pval[ro + v] = (uint32_t)v;
     }
 }
 
-// Blocks [0, nbp): sorted particles -- position of each touched particle and
-// the end of each particle run; the rest: the end of each pair run.
+// Over the sorted particles: position of each touched particle, end of each
+// run, row lengths in sorted order.
 __global__ void __launch_bounds__(kBlock)
-rps_segments_kernel(int64_t n, RpsArrays A, RpsStrides st, unsigned nbp) {
-    const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
-    if (blockIdx.x < nbp) {
-        const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-        if (e >= n) return;
-        const uint32_t* spk = A.spk + ro;
-        const uint32_t L = spk[e];
-        if (L >= (uint64_t)n) return;
-        const uint32_t x = A.ids[ro + e];
-        if (x < (uint64_t)n) A.loc[ro + x] = (uint32_t)e;
+rps_segments_kernel(int64_t n, RpsArrays A, RpsStrides st) {
+    const int64_t s = blockIdx.y, ro = s * st.row;
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e >= n) return;
+    const uint32_t* spk = A.spk + ro;
+    const uint32_t L = spk[e];
+    const uint32_t x = A.ids[ro + e];
+    uint32_t len = 0;
+    if (L < (uint64_t)n && x < (uint64_t)n) {
+        A.loc[ro + x] = (uint32_t)e;
+        const uint32_t rb = A.rowbeg[ro + x], re = A.rowend[ro + x];
+        len = re > rb ? re - rb : 0u;
         if (e + 1 == n || spk[e + 1] != L) A.pend[ro + L] = (uint32_t)(e + 1);
-        return;
     }
-    const int64_t P = A.plen[s];
-    const uint32_t* sqk = A.sqk + po;
-    for (int64_t e = (int64_t)(blockIdx.x - nbp) * kBlock + threadIdx.x; e < P;
-         e += (int64_t)(gridDim.x - nbp) * kBlock) {
-        const uint32_t L = sqk[e];
-        if (L < (uint64_t)n && (e + 1 == P || sqk[e + 1] != L)) A.qend[ro + L] = (uint32_t)(e + 1);
+    A.rcnt[ro + e] = len;
+}
+
+// Over the sorted particles: slot and pair shift per particle; every run start
+// lists its component.  A block takes kCompItems positions per thread and
+// reserves its list entries with one global add per list.
+__global__ void __launch_bounds__(kBlock)
+rps_comps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
+    __shared__ uint32_t cnt[2], base[2];
+    const int64_t s = blockIdx.y, ro = s * st.row;
+    const uint32_t* spk = A.spk + ro;
+    const uint32_t* roff = A.roff + ro;
+    // run bounds are clamped to the step's pairs, so that pairs not sorted by i
+    // (outside the contract) cannot send the replay past them
+    const uint32_t P = (uint32_t)A.plen[s];
+    if (threadIdx.x < 2) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    uint4 d[kCompItems];
+    uint32_t at[kCompItems];  // list (bit 31: wide) and local index, or kUnset
+#pragma unroll
+    for (int r = 0; r < kCompItems; ++r) {
+        const int64_t e = ((int64_t)blockIdx.x * kCompItems + r) * kBlock + threadIdx.x;
+        at[r] = kUnset;
+        d[r] = make_uint4(0u, 0u, 0u, 0u);
+        if (e >= n) continue;
+        const uint32_t L = spk[e];
+        const uint32_t x = A.ids[ro + e];
+        if (L >= (uint64_t)n || x >= (uint64_t)n) continue;
+        const uint32_t pb = A.loc[ro + L];  // the root is the first particle of its run
+        A.info[ro + x] = make_uint2(roff[e] - A.rowbeg[ro + x], (uint32_t)e - pb);
+        if (x == L) {
+            const uint32_t pe = A.pend[ro + L];
+            const uint32_t qb = min(roff[e], P), qe = min(roff[pe], P);
+            d[r] = make_uint4(qb, qe > qb ? qe - qb : 0u, (uint32_t)e, pe - (uint32_t)e);
+            const uint32_t wide = d[r].y > kNarrowPairs ? 1u : 0u;
+            at[r] = (wide << 31) | atomicAdd(&cnt[wide], 1u);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 && cnt[threadIdx.x] > 0) base[threadIdx.x] = atomicAdd(A.lists + 4 * s + threadIdx.x, cnt[threadIdx.x]);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kCompItems; ++r) {
+        if (at[r] == kUnset) continue;
+        const uint32_t wide = at[r] >> 31, pos = base[wide] + (at[r] & 0x7fffffffu);
+        if (pos < (uint64_t)st.row) (wide ? A.wide : A.narrow)[ro + pos] = d[r];
     }
 }
 
+// Over the pairs in rank order: records at their run positions.  A pair whose
+// j is not a particle index replays as a no-op (both ends on slot_i).
 __global__ void __launch_bounds__(kBlock)
 rps_fill_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __restrict__ pj_all, RpsArrays A,
                 RpsStrides st) {
     const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
     const int64_t P = A.plen[s];
-    const uint32_t* sqk = A.sqk + po;
-    const uint32_t* sqv = A.sqv + po;
-    const uint32_t* loc = A.loc + ro;
     const int32_t* pi = pi_all + po;
     const int32_t* pj = pj_all + po;
-    const uint32_t lane = threadIdx.x & 63;
-    // whole wavefronts iterate together (the list append is a wavefront operation)
-    for (int64_t e0 = (int64_t)blockIdx.x * kBlock + (threadIdx.x & ~63u); e0 < P;
-         e0 += (int64_t)gridDim.x * kBlock) {
-        const int64_t e = e0 + lane;
-        bool head = false;
-        uint4 d = make_uint4(0u, 0u, 0u, 0u);
-        if (e < P) {
-            const uint32_t L = sqk[e];
-            if (L < (uint64_t)n) {
-                const uint32_t k = sqv[e];
-                const uint32_t i = (uint32_t)pi[k], j = (uint32_t)pj[k];
-                const uint32_t pb = loc[L];  // the root is the first particle of its run
-                A.rec[po + e] = make_uint2(((loc[i] - pb) & kSlotMask) | ((uint32_t)A.bits[po + k] << 29),
-                                           loc[j] - pb);
-                if (e == 0 || sqk[e - 1] != L) {
-                    head = true;
-                    d = make_uint4((uint32_t)e, A.qend[ro + L] - (uint32_t)e, pb, A.pend[ro + L] - pb);
-                }
-            }
-        }
-        const bool to_narrow = head && d.y <= kNarrowPairs;
-        wave_append(to_narrow, d, A.narrow + ro, A.lists + 4 * s + 0, (uint32_t)st.row);
-        wave_append(head && !to_narrow, d, A.wide + ro, A.lists + 4 * s + 1, (uint32_t)st.row);
+    const uint2* info = A.info + ro;
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
+        const uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
+        if (a >= (uint64_t)n) continue;
+        const uint2 ia = info[a];
+        const uint32_t sb = b < (uint64_t)n ? info[b].y : ia.y;
+        const uint32_t pos = ia.x + (uint32_t)k;
+        if (pos < (uint64_t)P) A.rec[po + pos] = make_uint2((ia.y & kSlotMask) | ((uint32_t)A.bits[po + k] << 29), sb);
     }
 }
 
@@ -502,6 +524,7 @@ struct lm_rps_ws {
     int64_t pair_cap = 0;     // per-step stride of per-pair arrays
     lm::RpsArrays A{};
     lm::SortScratch sort{};
+    uint32_t* scan_tmp = nullptr;
     int64_t bytes = 0, pair_bytes = 0;
     int64_t last_n = 0;       // particles of the last prepare
     int32_t last_steps = 0;   // steps of the last prepare
@@ -525,13 +548,10 @@ static void rps_free_pairs(lm_rps_ws* w) {
     (void)hipFree(A.bits);
     (void)hipFree(A.rec);
     (void)hipFree(A.deps);
-    for (uint32_t* p : {A.qkey, A.qval, A.qtk, A.qtv}) (void)hipFree(p);
     A.bits = nullptr;
     A.rec = nullptr;
     A.deps = nullptr;
-    A.qkey = A.qval = A.qtk = A.qtv = nullptr;
-    w->bytes -= w->pair_bytes + w->sort.bytes;
-    lm::sort_scratch_free(&w->sort);
+    w->bytes -= w->pair_bytes;
     w->pair_bytes = 0;
     w->pair_cap = 0;
 }
@@ -539,19 +559,22 @@ static void rps_free_pairs(lm_rps_ws* w) {
 static void rps_ws_free(lm_rps_ws* w) {
     if (!w) return;
     lm::RpsArrays& A = w->A;
-    for (uint32_t* p : {A.parent, A.loc, A.pend, A.qend, A.pkey, A.pval, A.ptk, A.ptv, A.lists}) (void)hipFree(p);
+    for (uint32_t* p : {A.parent, A.loc, A.pend, A.rowbeg, A.rowend, A.rcnt, A.roff, A.pkey, A.pval, A.ptk, A.ptv,
+                        A.lists, w->scan_tmp})
+        (void)hipFree(p);
+    (void)hipFree(A.info);
     (void)hipFree(A.touched);
     (void)hipFree(A.narrow);
     (void)hipFree(A.wide);
     (void)hipFree(A.plen);
+    lm::sort_scratch_free(&w->sort);
     rps_free_pairs(w);
     delete w;
 }
 
-// Per-pair arrays (and the sort scratch) sized for `cap` pairs per step
-// (grown, never shrunk).
+// Per-pair arrays sized for `cap` pairs per step (grown, never shrunk).
 static int rps_ensure_pairs(lm_rps_ws* w, int64_t cap, hipStream_t s) {
-    if (cap <= w->pair_cap && w->sort.hist != nullptr) return LM_OK;
+    if (cap <= w->pair_cap) return LM_OK;
     LM_HIP_CHECK(hipStreamSynchronize(s));
     rps_free_pairs(w);
     const int64_t c = cap + cap / 4 + 1024, S = w->steps_max;
@@ -559,16 +582,13 @@ static int rps_ensure_pairs(lm_rps_ws* w, int64_t cap, hipStream_t s) {
     int64_t b = 0;
     int rc;
     if ((rc = rws_alloc(&A.bits, S * c, &b)) || (rc = rws_alloc(&A.rec, S * c, &b)) ||
-        (rc = rws_alloc(&A.deps, S * c, &b)) || (rc = rws_alloc(&A.qkey, S * c, &b)) ||
-        (rc = rws_alloc(&A.qval, S * c, &b)) || (rc = rws_alloc(&A.qtk, S * c, &b)) ||
-        (rc = rws_alloc(&A.qtv, S * c, &b)) ||
-        (rc = lm::sort_scratch_alloc(&w->sort, c > w->n_max ? c : w->n_max, w->steps_max))) {
+        (rc = rws_alloc(&A.deps, S * c, &b))) {
         w->bytes += b;
         w->pair_bytes = b;
         rps_free_pairs(w);
         return rc;
     }
-    w->bytes += b + w->sort.bytes;
+    w->bytes += b;
     w->pair_bytes = b;
     w->pair_cap = c;
     return LM_OK;
@@ -595,15 +615,20 @@ int lm_rps_ws_create(int64_t n_max, int32_t steps_max, lm_rps_ws** out) {
     lm::RpsArrays& A = w->A;
     int rc;
     if ((rc = rws_alloc(&A.parent, R, &w->bytes)) || (rc = rws_alloc(&A.loc, R, &w->bytes)) ||
-        (rc = rws_alloc(&A.pend, R, &w->bytes)) || (rc = rws_alloc(&A.qend, R, &w->bytes)) ||
+        (rc = rws_alloc(&A.pend, R, &w->bytes)) || (rc = rws_alloc(&A.rowbeg, R, &w->bytes)) ||
+        (rc = rws_alloc(&A.rowend, R, &w->bytes)) || (rc = rws_alloc(&A.rcnt, R, &w->bytes)) ||
+        (rc = rws_alloc(&A.roff, R, &w->bytes)) || (rc = rws_alloc(&A.info, R, &w->bytes)) ||
         (rc = rws_alloc(&A.touched, R, &w->bytes)) || (rc = rws_alloc(&A.pkey, R, &w->bytes)) ||
         (rc = rws_alloc(&A.pval, R, &w->bytes)) || (rc = rws_alloc(&A.ptk, R, &w->bytes)) ||
         (rc = rws_alloc(&A.ptv, R, &w->bytes)) || (rc = rws_alloc(&A.narrow, R, &w->bytes)) ||
         (rc = rws_alloc(&A.wide, R, &w->bytes)) || (rc = rws_alloc(&A.lists, 4 * S, &w->bytes)) ||
-        (rc = rws_alloc(&A.plen, S, &w->bytes))) {
+        (rc = rws_alloc(&A.plen, S, &w->bytes)) ||
+        (rc = rws_alloc(&w->scan_tmp, S * lm::scan_num_blocks(n_max), &w->bytes)) ||
+        (rc = lm::sort_scratch_alloc(&w->sort, n_max, steps_max))) {
         rps_ws_free(w);
         return rc;
     }
+    w->bytes += w->sort.bytes;
     hipError_t e = hipMemset(A.lists, 0, sizeof(uint32_t) * (size_t)(4 * S));
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
@@ -650,24 +675,22 @@ int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair
         LM_LAUNCH(lm::K_RPS_UNION, lm::rps_union_kernel, dim3(lm::kPairGrid, S), dim3(lm::kBlock), 0, s, n,
                   pair_i, pair_j, npairs, A, sp, pRS, pPR, pSP, key0, key1, step0);
         LM_LAUNCH_CHECK();
-        LM_LAUNCH(lm::K_RPS_KEYS, lm::rps_keys_kernel, dim3(nbp + lm::kPairGrid, S), dim3(lm::kBlock), 0, s, n,
-                  pair_i, pair_j, A, sp, nbp);
+        LM_LAUNCH(lm::K_RPS_KEYS, lm::rps_keys_kernel, dim3(nbp, S), dim3(lm::kBlock), 0, s, n, A, sp);
         LM_LAUNCH_CHECK();
-        const int kb = key_bits_for(n);
         int res = 0;
         uint32_t* pk[2] = {A.pkey, A.ptk};
         uint32_t* pv[2] = {A.pval, A.ptv};
-        if ((rc = lm::radix_sort_u32(pk, pv, ws->row, nsteps, nullptr, n, n, kb, &ws->sort, s, &res))) return rc;
+        if ((rc = lm::radix_sort_u32(pk, pv, ws->row, nsteps, nullptr, n, n, key_bits_for(n), &ws->sort, s, &res)))
+            return rc;
         A.spk = pk[res];
         A.ids = pv[res];
-        uint32_t* qk[2] = {A.qkey, A.qtk};
-        uint32_t* qv[2] = {A.qval, A.qtv};
-        if ((rc = lm::radix_sort_u32(qk, qv, pair_stride, nsteps, A.plen, 0, pair_stride, kb, &ws->sort, s, &res)))
+        LM_LAUNCH(lm::K_RPS_SEGMENTS, lm::rps_segments_kernel, dim3(nbp, S), dim3(lm::kBlock), 0, s, n, A, sp);
+        LM_LAUNCH_CHECK();
+        if ((rc = lm::exclusive_scan_u32_batched(A.rcnt, n, ws->row, A.roff, ws->row, nsteps, ws->scan_tmp, nullptr,
+                                                 s)))
             return rc;
-        A.sqk = qk[res];
-        A.sqv = qv[res];
-        LM_LAUNCH(lm::K_RPS_SEGMENTS, lm::rps_segments_kernel, dim3(nbp + lm::kPairGrid, S), dim3(lm::kBlock), 0,
-                  s, n, A, sp, nbp);
+        LM_LAUNCH(lm::K_RPS_COMPS, lm::rps_comps_kernel, dim3((nbp + lm::kCompItems - 1) / lm::kCompItems, S),
+                  dim3(lm::kBlock), 0, s, n, A, sp);
         LM_LAUNCH_CHECK();
         LM_LAUNCH(lm::K_RPS_FILL, lm::rps_fill_kernel, dim3(lm::kPairGrid, S), dim3(lm::kBlock), 0, s, n, pair_i,
                   pair_j, A, sp);

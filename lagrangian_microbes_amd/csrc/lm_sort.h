@@ -14,6 +14,13 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;  // 2048 elements per tile
 constexpr int kMaxDigitBits = 10;  // digit width of a pass (2 passes sort 20-bit keys)
+constexpr int kSortGroup = 4;      // consecutive tiles per block
+
+// Tiles of max_len elements, rounded up to whole groups.
+inline int64_t sort_padded_tiles(int64_t max_len) {
+    const int64_t t = (max_len + kSortTile - 1) / kSortTile;
+    return (t + kSortGroup - 1) / kSortGroup * kSortGroup;
+}
 
 struct SortScratch {
     uint32_t* hist = nullptr;     // [nseg][row]  digit-major tile counts

@@ -1,20 +1,23 @@
 // Segmented stable LSD radix sort (see lm_sort.h).
 //
-// Per pass over a digit of up to 10 bits (R = 2^bits buckets):
-//   hist    : per (segment, tile of 2048) the digit counts, stored
-//             digit-major: hist[g][d * ntiles + tile]
+// Per pass over a digit of up to 10 bits (R = 2^bits buckets), with blocks of
+// kSortGroup consecutive tiles of 2048 elements:
+//   hist    : per (segment, tile) the digit counts, stored digit-major,
+//             hist[g][d * ntp + tile] (ntp = tiles rounded up to the group),
+//             one 16-byte store per digit and block
 //   scan    : exclusive scan of each segment's histogram -> the output offset
 //             of (digit, tile); digit-major order puts every tile's run of a
 //             digit after the earlier tiles' runs of that digit
-//   scatter : each wavefront ranks its contiguous quarter of the tile, 64
-//             elements at a time: the lanes holding the same digit are found
-//             with one ballot per digit bit, a lane's rank is the number of
-//             such lanes below it, and a per-wavefront running count per digit
-//             (LDS, touched only by that wavefront, so in program order)
-//             carries the rank across rounds.  One scan over the digits then
-//             gives each element its place in the tile sorted by digit; the
-//             tile is staged there in LDS and written out in that order, so
-//             consecutive lanes store to consecutive addresses of a run.
+//   scatter : the block walks its tiles in order.  Each wavefront ranks its
+//             contiguous quarter of the tile, 64 elements at a time: the lanes
+//             holding the same digit are found with one ballot per digit bit,
+//             a lane's rank is the number of such lanes below it, and a
+//             per-wavefront running count per digit (LDS, touched only by that
+//             wavefront, so in program order) carries the rank across rounds.
+//             One scan over the digits then gives each element its place in the
+//             tile sorted by digit; the tile is staged there in LDS and written
+//             out in that order, so consecutive lanes store to consecutive
+//             addresses of a run.  The next tile's keys load meanwhile.
 // Equal digits keep their input order, which makes every pass stable.
 #include "lm_sort.h"
 #include "lm_scan.h"
@@ -47,31 +50,38 @@ template <int BITS>
 __global__ void __launch_bounds__(kSortThreads)
 radix_hist_kernel(const uint32_t* __restrict__ keys, int64_t stride, const int64_t* __restrict__ seg_len,
                   int64_t uniform_len, int64_t max_len, int shift, uint32_t dmask, uint32_t* __restrict__ hist,
-                  int64_t row) {
+                  int64_t row, int64_t ntp) {
     constexpr int R = 1 << BITS;
-    __shared__ uint32_t h[R];
+    static_assert(kSortGroup == 4, "one uint4 of counts per digit");
+    __shared__ uint32_t h[kSortGroup][R];
     const int g = blockIdx.y;
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t len = seg_len_of(seg_len, uniform_len, max_len, g);
-    for (int q = t; q < R; q += kSortThreads) h[q] = 0;
+    for (int q = t; q < kSortGroup * R; q += kSortThreads) (&h[0][0])[q] = 0;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kSortTile + wave * kWaveChunk;
     const uint32_t* k = keys + (int64_t)g * stride;
-    uint32_t kk[kSortItems];
+    const int64_t first = (int64_t)blockIdx.x * kSortGroup * kSortTile + wave * kWaveChunk;
+    uint32_t kk[kSortGroup][kSortItems];
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
-        const int64_t e = base + r * 64 + lane;
-        kk[r] = e < len ? k[e] : 0u;
-    }
+    for (int j = 0; j < kSortGroup; ++j)
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
-        const bool valid = base + r * 64 + lane < len;
-        const uint32_t d = (kk[r] >> shift) & dmask;
-        const unsigned long long m = match_digit<BITS>(valid, d);
-        if (valid && __popcll(m & ((1ull << lane) - 1ull)) == 0) atomicAdd(&h[d], (uint32_t)__popcll(m));
-    }
+        for (int r = 0; r < kSortItems; ++r) {
+            const int64_t e = first + (int64_t)j * kSortTile + r * 64 + lane;
+            kk[j][r] = e < len ? k[e] : 0u;
+        }
+#pragma unroll
+    for (int j = 0; j < kSortGroup; ++j)
+#pragma unroll
+        for (int r = 0; r < kSortItems; ++r) {
+            const bool valid = first + (int64_t)j * kSortTile + r * 64 + lane < len;
+            const uint32_t d = (kk[j][r] >> shift) & dmask;
+            const unsigned long long m = match_digit<BITS>(valid, d);
+            if (valid && __popcll(m & ((1ull << lane) - 1ull)) == 0) atomicAdd(&h[j][d], (uint32_t)__popcll(m));
+        }
     __syncthreads();
-    for (int q = t; q < R; q += kSortThreads) hist[(int64_t)g * row + (int64_t)q * gridDim.x + blockIdx.x] = h[q];
+    for (int q = t; q < R; q += kSortThreads)
+        *reinterpret_cast<uint4*>(hist + (int64_t)g * row + (int64_t)q * ntp + (int64_t)blockIdx.x * kSortGroup) =
+            make_uint4(h[0][q], h[1][q], h[2][q], h[3][q]);
 }
 
 template <int BITS>
@@ -79,7 +89,7 @@ __global__ void __launch_bounds__(kSortThreads)
 radix_scatter_kernel(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
                      uint32_t* __restrict__ vout, int64_t stride, const int64_t* __restrict__ seg_len,
                      int64_t uniform_len, int64_t max_len, int shift, uint32_t dmask,
-                     const uint32_t* __restrict__ offs, int64_t row) {
+                     const uint32_t* __restrict__ offs, int64_t row, int64_t ntp) {
     constexpr int R = 1 << BITS;
     constexpr int PER = R / kSortThreads;  // digits per thread in the digit scan
     static_assert(PER >= 1 && R % kSortThreads == 0, "digit count must be a multiple of the block");
@@ -91,87 +101,114 @@ radix_scatter_kernel(const uint32_t* __restrict__ kin, const uint32_t* __restric
     const int g = blockIdx.y;
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t len = seg_len_of(seg_len, uniform_len, max_len, g);
-    const int64_t tile0 = (int64_t)blockIdx.x * kSortTile;
-    if (tile0 >= len) return;  // a tile past this segment's end (uniform over the block)
-    for (int q = t; q < R; q += kSortThreads)
-#pragma unroll
-        for (int w = 0; w < kSortWaves; ++w) wcnt[w][q] = 0;
-    __syncthreads();
-    const int64_t base = tile0 + wave * kWaveChunk;
+    const int64_t group0 = (int64_t)blockIdx.x * kSortGroup * kSortTile;
+    if (group0 >= len) return;  // uniform over the block
     const uint32_t* ki = kin + (int64_t)g * stride;
     const uint32_t* vi = vin + (int64_t)g * stride;
-    uint32_t key[kSortItems], val[kSortItems], mine[kSortItems];
+    uint32_t* ko = kout + (int64_t)g * stride;
+    uint32_t* vo = vout + (int64_t)g * stride;
+    uint4 gb[PER];  // output offsets of my digits in the group's tiles
+#pragma unroll
+    for (int q = 0; q < PER; ++q)
+        gb[q] = *reinterpret_cast<const uint4*>(offs + (int64_t)g * row + (int64_t)(t * PER + q) * ntp +
+                                                (int64_t)blockIdx.x * kSortGroup);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t key[kSortItems], val[kSortItems];
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
-        const int64_t e = base + r * 64 + lane;
+        const int64_t e = group0 + wave * kWaveChunk + r * 64 + lane;
         key[r] = e < len ? ki[e] : 0u;
         val[r] = e < len ? vi[e] : 0u;
     }
-    const unsigned long long below = (1ull << lane) - 1ull;
-    // rank within the wavefront's chunk (this wavefront alone uses wcnt[wave])
 #pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
-        const bool valid = base + r * 64 + lane < len;
-        const uint32_t d = (key[r] >> shift) & dmask;
-        const unsigned long long m = match_digit<BITS>(valid, d);
-        const uint32_t rank = (uint32_t)__popcll(m & below);
-        const uint32_t run = valid ? wcnt[wave][d] : 0u;
-        mine[r] = run + rank;
-        if (valid && rank == 0) wcnt[wave][d] = run + (uint32_t)__popcll(m);
-    }
-    __syncthreads();
-    // per digit: offsets of the wavefronts' runs, tile-local start, global start
-    uint32_t tot[PER], sum = 0;
+    for (int j = 0; j < kSortGroup; ++j) {
+        const int64_t tile0 = group0 + (int64_t)j * kSortTile;
+        if (tile0 >= len) break;  // uniform over the block
+        const int64_t base = tile0 + wave * kWaveChunk;
+        for (int q = t; q < R; q += kSortThreads)
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int d = t * PER + q;
-        uint32_t acc = 0;
+            for (int w = 0; w < kSortWaves; ++w) wcnt[w][q] = 0;
+        __syncthreads();
+        // rank within the wavefront's chunk (this wavefront alone uses wcnt[wave])
+        uint32_t mine[kSortItems];
 #pragma unroll
-        for (int w = 0; w < kSortWaves; ++w) {
-            const uint32_t c = wcnt[w][d];
-            wcnt[w][d] = acc;
-            acc += c;
-        }
-        tot[q] = acc;
-        sum += acc;
-    }
-    uint32_t all;
-    uint32_t run = block_exclusive_scan(sum, scan_lds, &all);
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int d = t * PER + q;
-        lstart[d] = run;
-        gdelta[d] = offs[(int64_t)g * row + (int64_t)d * gridDim.x + blockIdx.x] - run;
-        run += tot[q];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kSortItems; ++r) {
-        if (base + r * 64 + lane < len) {
+        for (int r = 0; r < kSortItems; ++r) {
+            const bool valid = base + r * 64 + lane < len;
             const uint32_t d = (key[r] >> shift) & dmask;
-            const uint32_t lpos = lstart[d] + wcnt[wave][d] + mine[r];
-            if (lpos < (uint32_t)kSortTile) {
-                skey[lpos] = key[r];
-                sval[lpos] = val[r];
-            }
+            const unsigned long long m = match_digit<BITS>(valid, d);
+            const uint32_t rank = (uint32_t)__popcll(m & below);
+            const uint32_t run = valid ? wcnt[wave][d] : 0u;
+            mine[r] = run + rank;
+            if (valid && rank == 0) wcnt[wave][d] = run + (uint32_t)__popcll(m);
         }
-    }
-    __syncthreads();
-    const int64_t left = len - tile0;
-    const uint32_t cnt = left < kSortTile ? (uint32_t)left : (uint32_t)kSortTile;
-    uint32_t* ko = kout + (int64_t)g * stride;
-    uint32_t* vo = vout + (int64_t)g * stride;
+        // next tile's elements, in flight while this one is placed
+        uint32_t nkey[kSortItems], nval[kSortItems];
 #pragma unroll
-    for (int q = 0; q < kSortItems; ++q) {
-        const uint32_t slot = q * kSortThreads + t;
-        if (slot < cnt) {
-            const uint32_t k = skey[slot];
-            const uint32_t pos = gdelta[(k >> shift) & dmask] + slot;
-            if (pos < (uint64_t)len) {
-                ko[pos] = k;
-                vo[pos] = sval[slot];
+        for (int r = 0; r < kSortItems; ++r) {
+            const int64_t e = base + kSortTile + r * 64 + lane;
+            const bool ok = j + 1 < kSortGroup && e < len;
+            nkey[r] = ok ? ki[e] : 0u;
+            nval[r] = ok ? vi[e] : 0u;
+        }
+        __syncthreads();
+        // per digit: offsets of the wavefronts' runs, tile-local start, global start
+        uint32_t tot[PER], sum = 0;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int d = t * PER + q;
+            uint32_t acc = 0;
+#pragma unroll
+            for (int w = 0; w < kSortWaves; ++w) {
+                const uint32_t c = wcnt[w][d];
+                wcnt[w][d] = acc;
+                acc += c;
+            }
+            tot[q] = acc;
+            sum += acc;
+        }
+        uint32_t all;
+        uint32_t run = block_exclusive_scan(sum, scan_lds, &all);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int d = t * PER + q;
+            const uint32_t gbase = j == 0 ? gb[q].x : j == 1 ? gb[q].y : j == 2 ? gb[q].z : gb[q].w;
+            lstart[d] = run;
+            gdelta[d] = gbase - run;
+            run += tot[q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kSortItems; ++r) {
+            if (base + r * 64 + lane < len) {
+                const uint32_t d = (key[r] >> shift) & dmask;
+                const uint32_t lpos = lstart[d] + wcnt[wave][d] + mine[r];
+                if (lpos < (uint32_t)kSortTile) {
+                    skey[lpos] = key[r];
+                    sval[lpos] = val[r];
+                }
             }
         }
+        __syncthreads();
+        const int64_t left = len - tile0;
+        const uint32_t cnt = left < kSortTile ? (uint32_t)left : (uint32_t)kSortTile;
+#pragma unroll
+        for (int q = 0; q < kSortItems; ++q) {
+            const uint32_t slot = q * kSortThreads + t;
+            if (slot < cnt) {
+                const uint32_t k = skey[slot];
+                const uint32_t pos = gdelta[(k >> shift) & dmask] + slot;
+                if (pos < (uint64_t)len) {
+                    ko[pos] = k;
+                    vo[pos] = sval[slot];
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kSortItems; ++r) {
+            key[r] = nkey[r];
+            val[r] = nval[r];
+        }
+        // the next tile's first barrier orders these LDS reads before its writes
     }
 }
 
@@ -188,8 +225,8 @@ seg_copy_kernel(const uint32_t* __restrict__ ks, const uint32_t* __restrict__ vs
 }
 
 int sort_scratch_alloc(SortScratch* s, int64_t max_len, int32_t nseg_max) {
-    const int64_t ntiles = (max_len + kSortTile - 1) / kSortTile;
-    const int64_t n = (int64_t)(1 << kMaxDigitBits) * (ntiles < 1 ? 1 : ntiles);
+    const int64_t ntp = sort_padded_tiles(max_len);
+    const int64_t n = (int64_t)(1 << kMaxDigitBits) * (ntp < kSortGroup ? kSortGroup : ntp);
     s->row = scan_row_stride(n);
     s->max_len = max_len;
     s->nseg_max = nseg_max;
@@ -220,12 +257,12 @@ int radix_sort_u32(uint32_t* keys[2], uint32_t* vals[2], int64_t stride, int32_t
                "sort request exceeds its scratch");
     LM_REQUIRE(key_bits >= 0 && key_bits <= 32, LM_ERR_ARG, "key_bits out of range");
     int cur = 0;
-    const int64_t ntiles = (max_len + kSortTile - 1) / kSortTile;
-    if (ntiles == 0 || key_bits == 0) {
+    const int64_t ntp = sort_padded_tiles(max_len);
+    if (ntp == 0 || key_bits == 0) {
         *result = 0;
         return LM_OK;
     }
-    const dim3 grid((unsigned)ntiles, (unsigned)nseg);
+    const dim3 grid((unsigned)(ntp / kSortGroup), (unsigned)nseg);
     // fewest passes of at most kMaxDigitBits, the narrow kernel when a pass fits 8 bits
     const int passes = (key_bits + kMaxDigitBits - 1) / kMaxDigitBits;
     const int per = (key_bits + passes - 1) / passes;
@@ -233,15 +270,15 @@ int radix_sort_u32(uint32_t* keys[2], uint32_t* vals[2], int64_t stride, int32_t
         const int b = key_bits - shift < per ? key_bits - shift : per;
         const uint32_t dmask = (uint32_t)((1ull << b) - 1ull);
         const bool wide = b > 8;
-        const int64_t n = (int64_t)(wide ? (1 << kMaxDigitBits) : 256) * ntiles;
+        const int64_t n = (int64_t)(wide ? (1 << kMaxDigitBits) : 256) * ntp;
         if (wide)
             LM_LAUNCH(K_SORT_HIST, radix_hist_kernel<kMaxDigitBits>, grid, dim3(kSortThreads), 0, stream,
                       (const uint32_t*)keys[cur], stride, seg_len, uniform_len, max_len, shift, dmask, sc->hist,
-                      sc->row);
+                      sc->row, ntp);
         else
             LM_LAUNCH(K_SORT_HIST, radix_hist_kernel<8>, grid, dim3(kSortThreads), 0, stream,
                       (const uint32_t*)keys[cur], stride, seg_len, uniform_len, max_len, shift, dmask, sc->hist,
-                      sc->row);
+                      sc->row, ntp);
         LM_LAUNCH_CHECK();
         int rc = exclusive_scan_u32_batched(sc->hist, n, sc->row, sc->offs, sc->row, nseg, sc->bsums, nullptr,
                                             stream);
@@ -249,11 +286,11 @@ int radix_sort_u32(uint32_t* keys[2], uint32_t* vals[2], int64_t stride, int32_t
         if (wide)
             LM_LAUNCH(K_SORT_SCATTER, radix_scatter_kernel<kMaxDigitBits>, grid, dim3(kSortThreads), 0, stream,
                       (const uint32_t*)keys[cur], (const uint32_t*)vals[cur], keys[1 - cur], vals[1 - cur], stride,
-                      seg_len, uniform_len, max_len, shift, dmask, (const uint32_t*)sc->offs, sc->row);
+                      seg_len, uniform_len, max_len, shift, dmask, (const uint32_t*)sc->offs, sc->row, ntp);
         else
             LM_LAUNCH(K_SORT_SCATTER, radix_scatter_kernel<8>, grid, dim3(kSortThreads), 0, stream,
                       (const uint32_t*)keys[cur], (const uint32_t*)vals[cur], keys[1 - cur], vals[1 - cur], stride,
-                      seg_len, uniform_len, max_len, shift, dmask, (const uint32_t*)sc->offs, sc->row);
+                      seg_len, uniform_len, max_len, shift, dmask, (const uint32_t*)sc->offs, sc->row, ntp);
         LM_LAUNCH_CHECK();
         cur = 1 - cur;
     }
