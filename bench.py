@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--workload", default="rps_490k_30day")
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=48)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel on BYTES bytes 10 times (byte-counter calibration)")
     return ap.parse_args()
@@ -67,41 +67,56 @@ def run_steps(runner, k, total):
     return done
 
 
-def algorithmic_bytes(kernel, n, pairs, chunk, T, field_len):
-    """Compulsory bytes one launch of `kernel` must move.
+def algorithmic_bytes(kernel, m):
+    """Algorithmic bytes one launch of `kernel` moves, from the per-unit figures
+    of the path: advection reads and writes the float64 position (32 B per
+    particle), writes the float32 record (8 B per particle and step) and reads
+    two field levels per step (F); the pair search reads each position once
+    (16 B per particle) and writes the pair list (8 B per pair); the RPS update
+    reads the pair list (8 B per pair) and reads and writes the species (2 B
+    per particle).  Kernels that only exist to organise the work (sorts, cell
+    tables, grouping, scans) have none: their HBM traffic is overhead, reported
+    from the byte counters instead.
 
-    Per step: n particles, P pairs, T cell buckets, U = min(n, 2P) particles
-    with a pair.  Batched kernels cover `chunk` steps per launch; the replay
-    kernels (rps_play*) run once per step.  The advection launch reads and
-    writes the float64 state once and writes float32 records every step.
+    Per step: n particles, P pairs, Pw / Uw the pairs / particles of the
+    components replayed one wavefront each (more than 32 pairs).  Batched
+    kernels cover `chunk` steps per launch; rps_play* run once per step.
     """
-    P, U = pairs, min(n, 2 * pairs)
-    per_step = {
-        "bin_count": 8 * n + 8 * min(n, T),                     # positions, bucket counters
-        "bin_scatter": 8 * n + 8 * min(n, T) + 16 * n,          # positions, starts, records
-        "pair_count": 16 * n + 4 * n + 4 * P,                   # records, counts, neighbour rows
-        "pair_emit": 8 * n + 4 * P + 8 * P,                     # counts+offsets, rows, pair list
-        "rps_init": 8 * n,                                      # parent, slot
-        "rps_union": 8 * P + P + 8 * U,                         # pairs, outcome bits, parents
-        "rps_claim": 8 * P + 24 * U,                            # pairs, rows, slots, labels, counts
-        "rps_group": 4 * P + 20 * U,                            # heads, ids, offsets
-        "rps_order": 20 * U,                                    # heads, rows, row offsets
-        "rps_fill": 8 * P + P + 12 * U + 8 * P,                 # pairs, bits, lookups, records
-        "rps_deps": 16 * P,                                     # records in, masks out
-    }
+    n, P, Pw, Uw, chunk, F = (m[k] for k in ("n", "P", "Pw", "Uw", "chunk", "F"))
     per_launch = {
-        "advect_rk4_f64": 32 * n + 8 * n * chunk + 8 * field_len * (2 * chunk + 1),
-        "field_blend_f64": 24 * field_len * (2 * chunk + 1),
-        # untouched particles copied (slot check + species), components: ids,
-        # species in and out, pair records
-        "rps_play": 4 * n + 2 * (n - U) + 6 * U + 8 * P,
-        "rps_play_wide": 6 * U + 16 * P,
+        "advect_rk4_f64": 32 * n + (8 * n + F) * chunk,
+        "pair_count": 16 * n * chunk,
+        "pair_emit": 8 * P * chunk,
+        "rps_play": 8 * (P - Pw) + 2 * (n - Uw),
+        "rps_play_wide": 8 * Pw + 2 * Uw,
     }
-    if kernel in per_launch:
-        return per_launch[kernel]
-    if kernel in per_step:
-        return per_step[kernel] * chunk
-    return None
+    return per_launch.get(kernel)
+
+
+def step_statistics(runner, steps, total, chunk):
+    """Mean per-step pair count and component sizes over `steps` steps."""
+    runner.reset()
+    rows = []
+    while runner.step < min(steps, total):
+        k = runner.advance(min(chunk, total - runner.step))
+        cnt = runner.npairs[:k].cpu().numpy()
+        for s in range(k):
+            st = runner.engine.rps.component_stats(s)
+            rows.append((cnt[s], st.touched, st.wide_pairs, st.wide_particles))
+    P, U, Pw, Uw = np.array(rows, dtype=np.float64).mean(axis=0)
+    return {"P": P, "U": U, "Pw": Pw, "Uw": Uw}
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary of
+    this workload (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy), if any."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else int(k["bytes_per_launch"])
 
 
 def cpu_baseline(w, runner, steps):
@@ -186,38 +201,36 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     runner.check_domain()
-    pairs_per_step = float(runner.npairs[:min(args.chunk, args.steps)].double().mean().item())
     log(f"[bench] timed region {elapsed:.3f}s ({1e3 * elapsed / args.steps:.3f} ms/step)")
 
-    # same steps again with per-kernel events (dispatch-bracketed) for the roofline
+    # same steps again with per-kernel events on each launch's own dispatch
     runner.reset()
     torch.cuda.synchronize()
     with dev.KernelTimer() as timer:
         run_steps(runner, args.steps, w.steps)
         torch.cuda.synchronize()
     kt = timer.summary()
-    all_pairs = []
-    runner.reset()
     kernels = {k: {"launches": c, "avg_us": 1e3 * ms / c, "total_ms": ms} for k, (c, ms) in kt.items()}
     dominant = max(kernels, key=lambda k: kernels[k]["total_ms"])
-    T = runner.engine.search.workspace_bytes  # noqa: F841 (reported below)
-    # average pair count over the whole run, from a pass that keeps every step's count
-    counts = []
-    runner.reset()
-    while runner.step < min(args.steps, w.steps):
-        k = runner.advance(min(args.chunk, w.steps - runner.step))
-        counts.append(runner.npairs[:k].cpu().numpy().copy())
-    P = float(np.concatenate(counts).mean()) if counts else 0.0
-    field_len = runner.dfield.level_len
-    grid_T = 1 << max(10, int(np.ceil(np.log2(max(w.n, 1)))))
-    ab = algorithmic_bytes(dominant, w.n, P, args.chunk, grid_T, field_len)
+    stats = step_statistics(runner, args.steps, w.steps, args.chunk)
+    P = stats["P"]
+    # two levels of u and v per step (float64)
+    m = dict(stats, n=w.n, chunk=args.chunk, F=2 * 8 * runner.dfield.level_len)
+    ab = algorithmic_bytes(dominant, m)
     avg_s = kernels[dominant]["avg_us"] * 1e-6
     roofline = None
     if ab is not None:
         achieved = ab / avg_s / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dominant,
-                    "avg_us": round(kernels[dominant]["avg_us"], 2), "algorithmic_bytes": int(ab)}
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(w.name, dominant),
+                    "kernel": dominant, "avg_us": round(kernels[dominant]["avg_us"], 2),
+                    "algorithmic_bytes": int(ab)}
+    for k, v in kernels.items():
+        b, t = algorithmic_bytes(k, m), measured_traffic(w.name, k)
+        v["gbs"] = None if b is None else round(b / (v["avg_us"] * 1e-6) / 1e9, 1)
+        v["hbm_gbs"] = None if t is None else round(t / (v["avg_us"] * 1e-6) / 1e9, 1)
+    # the whole step against the same per-unit figures (records included)
+    path_bytes = 50 * w.n + 16 * P + m["F"] + 9 * w.n
 
     value = w.n * args.steps / elapsed
     result = {
@@ -235,9 +248,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "particles": w.n, "lattice": "700x700", "dt_s": w.dt,
                    "radius_deg": w.radius, "p": w.p, "field": w.describe()["field"],
-                   "mean_pairs_per_step": round(P, 1), "chunk": args.chunk},
+                   "mean_pairs_per_step": round(P, 1), "mean_particles_with_pairs": round(stats["U"], 1),
+                   "chunk": args.chunk},
         "roofline": roofline,
-        "kernels": {k: {"launches": v["launches"], "avg_us": round(v["avg_us"], 2)} for k, v in kernels.items()},
+        "kernels": {k: {"launches": v["launches"], "avg_us": round(v["avg_us"], 2), "algorithmic_gbs": v["gbs"],
+                        "hbm_gbs": v["hbm_gbs"]} for k, v in kernels.items()},
+        "path": {"algorithmic_bytes_per_step": int(path_bytes),
+                 "achieved_gbs": round(path_bytes / (elapsed / args.steps) / 1e9, 1)},
         "adjoint_forward_ratio": None,
     }
     if not args.no_cpu:

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define LM_ABI_VERSION 2
+#define LM_ABI_VERSION 3
 
 #define LM_OK 0
 #define LM_ERR_ARG (-1)      /* invalid argument                             */
@@ -101,6 +101,8 @@ int lm_pairs_ws_create(int64_t n_max, int32_t steps_max, double lon_min, double 
 void lm_pairs_ws_destroy(lm_pairs_ws* ws);
 /* Device bytes held by the workspace. */
 int64_t lm_pairs_ws_bytes(const lm_pairs_ws* ws);
+/* Number of cell buckets T of the workspace's cell list (a power of two). */
+int64_t lm_pairs_ws_buckets(const lm_pairs_ws* ws);
 
 /* All pairs (i, j), i < j, with ||x_i - x_j||_p <= radius (closed ball) in
  * degree space, where positions are the float32 lon/lat widened to double.
@@ -175,10 +177,11 @@ int lm_rps_play(lm_rps_ws* ws, int32_t step, int64_t n, const int8_t* species_in
                 int8_t* species_out, void* stream);
 
 /* Size statistics of the interaction components of step `step` of the last
- * prepare, written to device int64[4]: components with at least one pair,
- * largest component (pairs), largest component (particles), total pairs.
- * Diagnostic only. */
-int lm_rps_component_stats(lm_rps_ws* ws, int32_t step, int64_t* out4, void* stream);
+ * prepare, written to device int64[8]: components with at least one pair,
+ * largest component (pairs), largest component (particles), total pairs,
+ * components replayed by a wavefront (more than 32 pairs), their pairs, their
+ * particles, particles with at least one pair.  Diagnostic only. */
+int lm_rps_component_stats(lm_rps_ws* ws, int32_t step, int64_t* out8, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Counter-based uniforms                                                   */

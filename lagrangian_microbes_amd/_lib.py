@@ -34,6 +34,7 @@ _SIGNATURES = {
                                           ctypes.POINTER(c_vp)]),
     "lm_pairs_ws_destroy": (None, [c_vp]),
     "lm_pairs_ws_bytes": (c_i64, [c_vp]),
+    "lm_pairs_ws_buckets": (c_i64, [c_vp]),
     "lm_query_pairs": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_dbl, c_dbl, c_vp, c_vp, c_i64, c_vp, c_vp]),
     "lm_query_pairs_batch": (ctypes.c_int, [c_vp, c_i32, c_i64, c_vp, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_vp,
                                             c_i64, c_vp, c_vp]),
@@ -72,7 +73,7 @@ def load(path=LIB_PATH):
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        if L.lm_abi_version() != 2:
+        if L.lm_abi_version() != 3:
             raise RuntimeError("liblm.so ABI version mismatch; rebuild it")
         _lib = L
     return _lib

@@ -388,6 +388,8 @@ void lm_pairs_ws_destroy(lm_pairs_ws* ws) { pairs_ws_free(ws); }
 
 int64_t lm_pairs_ws_bytes(const lm_pairs_ws* ws) { return ws ? ws->bytes : 0; }
 
+int64_t lm_pairs_ws_buckets(const lm_pairs_ws* ws) { return ws ? (int64_t)ws->grid.T : 0; }
+
 int lm_query_pairs_batch(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float* lon, const float* lat,
                          int64_t pos_stride, double radius, double p, int32_t* pair_i, int32_t* pair_j,
                          int64_t pair_stride, int64_t* npairs, void* stream) {

@@ -483,31 +483,40 @@ rps_play_wide_kernel(int64_t n, int64_t s, RpsArrays A, RpsStrides st, const int
 }
 
 // Component size statistics of one prepared step (diagnostic, on request):
-// stats = {components, max pairs in one, max particles in one, total pairs}.
+// stats = {components, max pairs in one, max particles in one, total pairs,
+//          wide components, their pairs, their particles, particles with pairs}.
 __global__ void __launch_bounds__(kBlock)
 rps_stats_kernel(int64_t s, RpsArrays A, RpsStrides st, int64_t* __restrict__ stats) {
     const int64_t ro = s * st.row;
     const uint32_t c0 = A.lists[4 * s + 0], c1 = A.lists[4 * s + 1];
-    uint32_t active = 0, np_sum = 0, np_max = 0, psz_max = 0;
+    uint64_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < c0 + c1; c += gridDim.x * kBlock) {
-        const uint4 d = c < c0 ? A.narrow[ro + c] : A.wide[ro + (c - c0)];
-        active += 1;
-        np_sum += d.y;
-        np_max = max(np_max, d.y);
-        psz_max = max(psz_max, d.w);
+        const bool wide = c >= c0;
+        const uint4 d = wide ? A.wide[ro + (c - c0)] : A.narrow[ro + c];
+        v[0] += 1;
+        v[1] = v[1] > d.y ? v[1] : d.y;
+        v[2] = v[2] > d.w ? v[2] : d.w;
+        v[3] += d.y;
+        v[4] += wide ? 1 : 0;
+        v[5] += wide ? d.y : 0;
+        v[6] += wide ? d.w : 0;
+        v[7] += d.w;
     }
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        active += __shfl_xor(active, d, 64);
-        np_sum += __shfl_xor(np_sum, d, 64);
-        np_max = max(np_max, __shfl_xor(np_max, d, 64));
-        psz_max = max(psz_max, __shfl_xor(psz_max, d, 64));
-    }
-    if ((threadIdx.x & 63) == 0 && active) {
-        atomicAdd((unsigned long long*)&stats[0], (unsigned long long)active);
-        atomicMax((unsigned long long*)&stats[1], (unsigned long long)np_max);
-        atomicMax((unsigned long long*)&stats[2], (unsigned long long)psz_max);
-        atomicAdd((unsigned long long*)&stats[3], (unsigned long long)np_sum);
+    for (int q = 0; q < 8; ++q)
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            const uint64_t o = __shfl_xor(v[q], d, 64);
+            v[q] = (q == 1 || q == 2) ? (v[q] > o ? v[q] : o) : v[q] + o;
+        }
+    if ((threadIdx.x & 63) == 0 && v[0]) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (q == 1 || q == 2)
+                atomicMax((unsigned long long*)&stats[q], (unsigned long long)v[q]);
+            else
+                atomicAdd((unsigned long long*)&stats[q], (unsigned long long)v[q]);
+        }
     }
 }
 
@@ -736,16 +745,16 @@ int lm_rps_update(lm_rps_ws* ws, int64_t n, int8_t* species, const int32_t* pair
     return lm_rps_play(ws, 0, n, species, species, stream);
 }
 
-int lm_rps_component_stats(lm_rps_ws* ws, int32_t step, int64_t* out4, void* stream) {
-    LM_REQUIRE(ws != nullptr && out4 != nullptr, LM_ERR_ARG, "NULL argument");
+int lm_rps_component_stats(lm_rps_ws* ws, int32_t step, int64_t* out8, void* stream) {
+    LM_REQUIRE(ws != nullptr && out8 != nullptr, LM_ERR_ARG, "NULL argument");
     LM_REQUIRE(step >= 0 && (ws->last_steps == 0 || step < ws->last_steps), LM_ERR_ARG, "step out of range");
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(lm::zero_i64_small_kernel, dim3(1), dim3(64), 0, s, out4, 4);
+    hipLaunchKernelGGL(lm::zero_i64_small_kernel, dim3(1), dim3(64), 0, s, out8, 8);
     LM_LAUNCH_CHECK();
     if (ws->last_n > 0 && ws->last_steps > 0 && ws->pair_stride_used > 0) {
         lm::RpsStrides sp{ws->row, ws->pair_stride_used};
         hipLaunchKernelGGL(lm::rps_stats_kernel, dim3(256), dim3(lm::kBlock), 0, s, (int64_t)step, ws->A, sp,
-                           out4);
+                           out8);
         LM_LAUNCH_CHECK();
     }
     return LM_OK;

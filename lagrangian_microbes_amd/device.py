@@ -4,6 +4,7 @@ These are thin: validate on the host, hand raw pointers and the current
 stream to liblm, and return.  Nothing here synchronises with the host except
 where a data-dependent size must come back (PairSearch.query_sync).
 """
+import collections
 import ctypes
 
 import numpy as np
@@ -172,6 +173,11 @@ class PairSearch:
     def workspace_bytes(self):
         return int(lib().lm_pairs_ws_bytes(self._ws))
 
+    @property
+    def buckets(self):
+        """Number of cell buckets of the cell list."""
+        return int(lib().lm_pairs_ws_buckets(self._ws))
+
     def query(self, lon, lat, radius, p, pair_i, pair_j, npairs):
         """Asynchronous: pairs into caller buffers (capacity = their length)."""
         n = lon.numel()
@@ -236,7 +242,7 @@ class RPSUpdater:
         check(lib().lm_rps_ws_create(self.n_max, self.steps_max, ctypes.byref(h)))
         self._ws = h
         self.device = torch.device(device or "cuda")
-        self._stats = torch.zeros(4, dtype=torch.int64, device=self.device)
+        self._stats = torch.zeros(8, dtype=torch.int64, device=self.device)
 
     def __del__(self):
         ws = getattr(self, "_ws", None)
@@ -281,14 +287,18 @@ class RPSUpdater:
         check(lib().lm_rps_play(self._ws, int(step), n, ptr(species_in), ptr(species_out), stream_ptr()))
 
     def component_stats(self, step=0):
-        """(components, max pairs in one, max particles in one, total pairs) of a prepared step."""
+        """Component sizes of step ``step`` of the last prepare (a ComponentStats)."""
         check(lib().lm_rps_component_stats(self._ws, int(step), ptr(self._stats), stream_ptr()))
-        return tuple(int(v) for v in self._stats.cpu().tolist())
+        return ComponentStats(*(int(v) for v in self._stats.cpu().tolist()))
 
 
 # ---------------------------------------------------------------------------
 # per-kernel timing
 # ---------------------------------------------------------------------------
+ComponentStats = collections.namedtuple(
+    "ComponentStats", "components max_pairs max_particles pairs wide_components wide_pairs wide_particles touched")
+
+
 class KernelTimer:
     """Context manager: every liblm launch inside is timed with events on its
     own dispatch; ``summary()`` returns {kernel: (launches, total_ms)}."""

@@ -1,0 +1,133 @@
+"""Short text summaries of rocprofv3 CSV output for liblm kernels.
+
+    python -m lagrangian_microbes_amd.profile_summary trace DIR [--title T] [--command C]
+    python -m lagrangian_microbes_amd.profile_summary traffic FETCH_DIR WRITE_DIR \
+        COPY_FETCH_DIR COPY_WRITE_DIR COPY_BYTES [--json OUT] [--command C]
+
+``trace`` reads the ``*_kernel_stats.csv`` of a ``rocprofv3 --kernel-trace
+--stats --output-format csv`` run and prints a markdown table of the liblm
+kernels (calls, average and total duration).
+
+``traffic`` reads four ``rocprofv3 --pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE``
+runs (``*_counter_collection.csv``): the workload and a plain copy of
+COPY_BYTES bytes each way (``bench.py --copy-calibration COPY_BYTES``).  Each
+counter (KiB per dispatch) is scaled by COPY_BYTES over what it reads for the
+copy, which turns it into HBM bytes per dispatch.  ``--json`` writes the
+per-kernel bytes per launch that bench.py reports as ``roofline.traffic``.
+"""
+import argparse
+import collections
+import csv
+import glob
+import json
+import os
+import sys
+
+# rocprofv3 kernel name -> the name liblm's own launch timer uses
+TIMER_NAMES = {
+    "advect_rk4_f64_kernel": "advect_rk4_f64", "field_blend_f64_kernel": "field_blend_f64",
+    "bin_keys_kernel": "bin_keys", "bin_table_kernel": "bin_table", "pair_count_kernel": "pair_count",
+    "pair_emit_kernel": "pair_emit", "radix_hist_kernel": "sort_hist", "radix_scatter_kernel": "sort_scatter",
+    "scan_reduce_kernel": "scan_reduce", "scan_apply_kernel": "scan_apply", "rps_init_kernel": "rps_init",
+    "rps_union_kernel": "rps_union", "rps_keys_kernel": "rps_keys", "rps_segments_kernel": "rps_segments",
+    "rps_comps_kernel": "rps_comps", "rps_fill_kernel": "rps_fill", "rps_deps_kernel": "rps_deps",
+    "rps_levels_kernel": "rps_levels", "rps_play_narrow_kernel": "rps_play",
+    "rps_play_wide_kernel": "rps_play_wide", "copy16_kernel": "copy16", "philox_kernel": "philox",
+}
+
+
+def short_name(full):
+    """'void lm::radix_scatter_kernel<10>(...)' -> 'radix_scatter_kernel'."""
+    name = full.split("(")[0].replace("void ", "").strip()
+    if not name.startswith("lm::"):
+        return None
+    return name[4:].split("<")[0]
+
+
+def one_file(d, pattern):
+    files = sorted(glob.glob(os.path.join(d, "**", pattern), recursive=True))
+    if not files:
+        sys.exit(f"no {pattern} under {d}")
+    return files[-1]
+
+
+def trace(args):
+    rows = []
+    for r in csv.DictReader(open(one_file(args.dir, "*_kernel_stats.csv"))):
+        k = short_name(r["Name"])
+        if k is None:
+            continue
+        rows.append((float(r["TotalDurationNs"]), k, int(r["Calls"]), float(r["AverageNs"])))
+    total = sum(r[0] for r in rows)
+    out = [f"# {args.title}", ""]
+    if args.command:
+        out += ["Command:", "", "```", args.command, "```", ""]
+    out += ["| kernel | timer name | calls | avg (us) | total (ms) | share |", "|---|---|---:|---:|---:|---:|"]
+    for tot, k, calls, avg in sorted(rows, reverse=True):
+        out.append(f"| {k} | {TIMER_NAMES.get(k, '')} | {calls} | {avg / 1e3:.2f} | {tot / 1e6:.2f} | "
+                   f"{100 * tot / total:.1f}% |")
+    print("\n".join(out))
+
+
+def counters(d, name):
+    acc = collections.defaultdict(list)
+    for r in csv.DictReader(open(one_file(d, "*_counter_collection.csv"))):
+        if r["Counter_Name"] != name:
+            continue
+        k = short_name(r["Kernel_Name"])
+        if k is not None:
+            acc[k].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}, {k: len(v) for k, v in acc.items()}
+
+
+def traffic(args):
+    fetch, calls = counters(args.fetch_dir, "FETCH_SIZE")
+    write, _ = counters(args.write_dir, "WRITE_SIZE")
+    cf, _ = counters(args.copy_fetch_dir, "FETCH_SIZE")
+    cw, _ = counters(args.copy_write_dir, "WRITE_SIZE")
+    nb = args.copy_bytes
+    sf = nb / (1024.0 * cf["copy16_kernel"])
+    sw = nb / (1024.0 * cw["copy16_kernel"])
+    out = ["# HBM traffic per dispatch (FETCH_SIZE + WRITE_SIZE)", ""]
+    if args.command:
+        out += ["Commands:", "", "```", args.command, "```", ""]
+    out += [f"Copy calibration: a plain copy of {nb} bytes each way reads FETCH_SIZE "
+            f"{cf['copy16_kernel']:.0f} KiB and WRITE_SIZE {cw['copy16_kernel']:.0f} KiB per dispatch, so "
+            f"bytes = FETCH_SIZE x 1024 x {sf:.4f} and WRITE_SIZE x 1024 x {sw:.4f}.", "",
+            "| kernel | timer name | dispatches | FETCH_SIZE (KiB) | WRITE_SIZE (KiB) | HBM bytes / dispatch |",
+            "|---|---|---:|---:|---:|---:|"]
+    res = {}
+    for k in sorted(set(fetch) | set(write)):
+        if k == "copy16_kernel":
+            continue
+        f, w = fetch.get(k, 0.0), write.get(k, 0.0)
+        b = f * 1024 * sf + w * 1024 * sw
+        tn = TIMER_NAMES.get(k, k)
+        res[tn] = {"dispatches": calls.get(k, 0), "fetch_kib": round(f, 1), "write_kib": round(w, 1),
+                   "bytes_per_launch": int(round(b))}
+        out.append(f"| {k} | {tn} | {calls.get(k, 0)} | {f:.1f} | {w:.1f} | {b:,.0f} |")
+    print("\n".join(out))
+    if args.json:
+        with open(args.json, "w") as fh:
+            json.dump({"copy_bytes": nb, "fetch_scale": sf, "write_scale": sw, "kernels": res}, fh, indent=1)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("trace")
+    t.add_argument("dir")
+    t.add_argument("--title", default="Kernel trace")
+    t.add_argument("--command", default="")
+    c = sub.add_parser("traffic")
+    for a in ("fetch_dir", "write_dir", "copy_fetch_dir", "copy_write_dir"):
+        c.add_argument(a)
+    c.add_argument("copy_bytes", type=int)
+    c.add_argument("--json")
+    c.add_argument("--command", default="")
+    args = ap.parse_args()
+    (trace if args.cmd == "trace" else traffic)(args)
+
+
+if __name__ == "__main__":
+    main()

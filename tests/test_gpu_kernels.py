@@ -193,8 +193,8 @@ def test_rps_matches_sequential_oracle(gpu, seed, n, m, clustered):
     g, up = gpu_rps(sp, pi, pj, *args)
     o = oracle.rps_sequential(sp, pi, pj, *args)
     assert np.array_equal(g, o)
-    comps, max_pairs, max_heads, total = up.component_stats()
-    assert total == m and comps >= 1
+    st = up.component_stats()
+    assert st.pairs == m and st.components >= 1 and st.touched <= n
 
 
 def test_rps_giant_chain_component(gpu):
@@ -211,7 +211,8 @@ def test_rps_giant_chain_component(gpu):
     g, up = gpu_rps(sp, arr[:, 0], arr[:, 1], 0.5, 0.5, 0.5, (1, 1), 3)
     o = oracle.rps_sequential(sp, arr[:, 0], arr[:, 1], 0.5, 0.5, 0.5, (1, 1), 3)
     assert np.array_equal(g, o)
-    assert up.component_stats()[1] == 20000
+    st = up.component_stats()
+    assert st.max_pairs == 20000 and st.wide_components == 1 and st.wide_pairs == 20000 and st.touched == n
 
 
 @pytest.mark.parametrize("seed", [0, 1])
@@ -241,8 +242,9 @@ def test_rps_every_replay_tier(gpu, seed):
     g, up = gpu_rps(sp, arr[:, 0], arr[:, 1], *args)
     o = oracle.rps_sequential(sp, arr[:, 0], arr[:, 1], *args)
     assert np.array_equal(g, o)
-    comps, max_pairs, max_heads, total = up.component_stats()
-    assert comps == len(sizes) and total == len(arr)
+    st = up.component_stats()
+    assert st.components == len(sizes) and st.pairs == len(arr) and st.touched == n
+    assert st.max_particles == max(sizes)
 
 
 def test_rps_empty_and_overflow(gpu):
