@@ -69,9 +69,9 @@ enum KernelId {
     K_RPS_SEGMENTS,
     K_RPS_COMPS,
     K_RPS_PLAY,
-    K_RPS_PLAY_WIDE,
     K_RPS_FILL,
     K_RPS_DEPS,
+    K_RPS_LEVELS,
     K_PHILOX,
     K_COPY,
     K_SORT_HIST,

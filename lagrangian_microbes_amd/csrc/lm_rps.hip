@@ -33,16 +33,18 @@
 //                        run positions
 //           9. deps      for the wide components, per 64-pair window, the
 //                        bitmask of earlier pairs of the window sharing a slot
-//   No global atomics but the union's CAS and one add per wavefront for the
+//          10. levels    for wide components of up to kLevelPairs pairs: each
+//                        pair's level (longest chain of earlier pairs sharing a
+//                        particle), pairs counting-sorted by level
+//   No global atomics but the union's CAS and one add per block for the
 //   component lists.
-// Only the replay itself runs per step (two launches):
-//   play    narrow       one lane per small component: species into LDS by
-//                        slot, pairs replayed in order; particles without pairs
-//                        copied
-//           wide         one wavefront per larger component: species into LDS,
-//                        each window replayed in rounds -- a pair fires once
-//                        every earlier pair of the window it depends on has
-//                        fired, which is exactly the sequential order
+// Only the replay itself runs per step, in one launch of one-wavefront blocks:
+//   wide     one wavefront per larger component, species in LDS: level by
+//            level (a level's pairs share no particle, so they play at once),
+//            or for components past the level limits window by window in
+//            dependency rounds
+//   small    one lane per component of <= 32 pairs, replayed in order
+//   copy     particles without pairs keep their species
 #include "lm_common.h"
 #include "lm_scan.h"
 #include "lm_sort.h"
@@ -52,10 +54,10 @@ namespace lm {
 constexpr uint32_t kUnset = 0xffffffffu;
 constexpr uint32_t kNarrowPairs = 32;
 constexpr uint32_t kNarrowSlots = kNarrowPairs + 1;  // a component of P pairs has <= P + 1 particles
-constexpr uint32_t kWideSlots = 16384;               // LDS species slots of the wide replay
+constexpr uint32_t kLevelPairs = 4096;               // components of up to this many pairs replay by levels
+constexpr uint32_t kLevelSlots = kLevelPairs + 1;    // ... and so have at most this many particles
 constexpr uint32_t kSlotMask = (1u << 29) - 1u;
 constexpr unsigned kPairGrid = 256;                  // blocks per step for pair-parallel passes
-constexpr unsigned kWideBlocks = 4096;               // one-wavefront blocks of the wide replay
 constexpr int kCompItems = 8;                        // sorted positions per thread when listing components
 
 // Per-step strides (elements) of a batch.
@@ -81,6 +83,9 @@ struct RpsArrays {
     uint8_t* bits;                        // per pair (by rank)
     uint2* rec;                           // per pair (run order)
     unsigned long long* deps;             // per pair (run order, wide components)
+    uint2* lrec;                          // per pair: wide components' records in level order
+    uint32_t* lend;                       // per pair: end of each level, from a component's first pair
+    uint32_t* depth;                      // per wide component: levels, 0 = windowed replay
 };
 
 __device__ __forceinline__ uint32_t find_root(const uint32_t* parent_c, uint32_t v) {
@@ -124,15 +129,22 @@ __device__ __forceinline__ int64_t step_pairs(const int64_t* npairs, int64_t s, 
 // Cyclic dominance: a beats b iff a == b % 3 + 1 (rock 1 > scissors 3, paper
 // 2 > rock 1, scissors 3 > paper 2); the dominant species of the unordered pair
 // wins with that pair's forward probability; the loser takes the winner's code.
+// Written without branches: with u = s - 1 in {0, 1, 2}, the pair's
+// probability bit is k(ui + uj) = {1: pPR (bit 1), 2: pRS (bit 0), 3: pSP
+// (bit 2)} (a like pair also lands on 2, and any outcome leaves it unchanged),
+// and i beats j iff ui - uj is 1 or -2.  Species outside {1, 2, 3} never
+// interact.
 __device__ __forceinline__ void play_pair(int& si, int& sj, uint32_t bits) {
-    if (si == sj || (unsigned)(si - 1) > 2u || (unsigned)(sj - 1) > 2u) return;
-    const int sum = si + sj;  // 3: rock-paper, 4: rock-scissors, 5: paper-scissors
-    const uint32_t forward = (bits >> (sum == 4 ? 0 : sum == 3 ? 1 : 2)) & 1u;
-    const bool i_beats_j = (si == sj % 3 + 1);
-    if ((forward != 0) == i_beats_j)
-        sj = si;
-    else
-        si = sj;
+    const uint32_t ui = (uint32_t)(si - 1), uj = (uint32_t)(sj - 1);
+    const bool valid = (ui < 3u) & (uj < 3u);
+    const uint32_t k = (0x2010u >> (4u * ((ui + uj) & 7u))) & 3u;
+    const uint32_t forward = (bits >> k) & 1u;
+    const uint32_t beats = (0x9u >> ((ui - uj + 2u) & 7u)) & 1u;
+    const bool i_wins = forward == beats;
+    const int ni = valid && !i_wins ? sj : si;
+    const int nj = valid && i_wins ? si : sj;
+    si = ni;
+    sj = nj;
 }
 
 // Wavefront-aggregated append of v to list (one atomic per wavefront); every
@@ -319,166 +331,348 @@ rps_deps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// play (one step)
-// ---------------------------------------------------------------------------
-// Blocks [0, ncopy): particles without pairs keep their species.  The rest:
-// one lane per narrow component.
-__global__ void __launch_bounds__(kBlock)
-rps_play_narrow_kernel(int64_t n, int64_t s, RpsArrays A, RpsStrides st, const int8_t* sp_in, int8_t* sp_out,
-                       unsigned ncopy) {
-    __shared__ int8_t slots[kBlock * kNarrowSlots];
-    const int64_t ro = s * st.row, po = s * st.pair;
-    const uint32_t* loc = A.loc + ro;
-    if (blockIdx.x < ncopy) {
-        // four particles per thread; only particles without pairs are written
-        // here (the others belong to a component and are written by its lane)
-        if (sp_in == sp_out) return;
-        const int64_t x0 = 4 * ((int64_t)blockIdx.x * kBlock + threadIdx.x);
-        if (x0 + 4 <= n) {
-            const uint4 l4 = *reinterpret_cast<const uint4*>(loc + x0);  // rows are 16-byte aligned
-            const int8_t a0 = sp_in[x0], a1 = sp_in[x0 + 1], a2 = sp_in[x0 + 2], a3 = sp_in[x0 + 3];
-            if (l4.x == kUnset) sp_out[x0] = a0;
-            if (l4.y == kUnset) sp_out[x0 + 1] = a1;
-            if (l4.z == kUnset) sp_out[x0 + 2] = a2;
-            if (l4.w == kUnset) sp_out[x0 + 3] = a3;
-        } else {
-            for (int64_t x = x0; x < n; ++x)
-                if (loc[x] == kUnset) sp_out[x] = sp_in[x];
-        }
-        return;
-    }
-    int8_t* my = slots + threadIdx.x * kNarrowSlots;
-    const uint32_t* ids = A.ids + ro;
-    const uint2* rec = A.rec + po;
-    const uint32_t cnt = A.lists[4 * s + 0];
-    const uint32_t nblk = gridDim.x - ncopy;
-    for (uint32_t c = (blockIdx.x - ncopy) * kBlock + threadIdx.x; c < cnt; c += nblk * kBlock) {
-        const uint4 dsc = A.narrow[ro + c];
-        const uint32_t qb = dsc.x, np = min(dsc.y, kNarrowPairs);
-        const uint32_t pb = dsc.z, psz = min(dsc.w, kNarrowSlots);
-        uint32_t id[kNarrowSlots];
-#pragma unroll
-        for (uint32_t t = 0; t < kNarrowSlots; ++t) id[t] = t < psz ? ids[pb + t] : kUnset;
-#pragma unroll
-        for (uint32_t t = 0; t < kNarrowSlots; ++t)
-            if (t < psz) my[t] = id[t] < (uint64_t)n ? sp_in[id[t]] : (int8_t)0;
-        for (uint32_t q0 = 0; q0 < np; q0 += 8) {
-            uint2 r[8];
-#pragma unroll
-            for (uint32_t q = 0; q < 8; ++q) r[q] = q0 + q < np ? rec[qb + q0 + q] : make_uint2(0u, 0u);
-#pragma unroll
-            for (uint32_t q = 0; q < 8; ++q) {
-                if (q0 + q < np) {
-                    const uint32_t a = min(r[q].x & kSlotMask, kNarrowSlots - 1), b = min(r[q].y, kNarrowSlots - 1);
-                    int si = my[a], sj = my[b];
-                    play_pair(si, sj, r[q].x >> 29);
-                    my[a] = (int8_t)si;
-                    my[b] = (int8_t)sj;
-                }
-            }
-        }
-#pragma unroll
-        for (uint32_t t = 0; t < kNarrowSlots; ++t)
-            if (t < psz && id[t] < (uint64_t)n) sp_out[id[t]] = my[t];
-    }
-}
-
-// 64-lane blocks, one wavefront per wide component.  The block is a single
-// wavefront, whose LDS operations complete in program order, so a round's
-// writes are seen by the next round's reads without a barrier.
+// 64-lane blocks, one wavefront per wide component of at most kLevelPairs
+// pairs: the level of every pair in the dependency order (1 + the largest
+// level of an earlier pair sharing a particle; replayed window by window with
+// the masks of rps_deps, so it is exactly the longest-path depth), then the
+// pairs counting-sorted by level.  Pairs of one level share no particle, so a
+// replay may run each level in parallel, levels in order.
 __global__ void __launch_bounds__(64)
-rps_play_wide_kernel(int64_t n, int64_t s, RpsArrays A, RpsStrides st, const int8_t* sp_in, int8_t* sp_out) {
-    __shared__ int8_t slots[kWideSlots];
-    const int64_t ro = s * st.row, po = s * st.pair;
-    const uint32_t* ids = A.ids + ro;
+rps_levels_kernel(int64_t n, RpsArrays A, RpsStrides st) {
+    __shared__ uint16_t last[kLevelSlots];
+    __shared__ uint16_t lvl[kLevelPairs];
+    __shared__ uint32_t cnt[kLevelPairs];
+    const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t ncomp = A.lists[4 * s + 1];
     const uint2* rec = A.rec + po;
     const unsigned long long* deps = A.deps + po;
-    const uint32_t lane = threadIdx.x;
-    const uint32_t cnt = A.lists[4 * s + 1];
-    for (uint32_t c = blockIdx.x; c < cnt; c += gridDim.x) {
+    for (uint32_t c = blockIdx.x; c < ncomp; c += gridDim.x) {
         const uint4 dsc = A.wide[ro + c];
-        const uint32_t qb = dsc.x, np = dsc.y, pb = dsc.z, psz = dsc.w;
-        const bool in_lds = psz <= kWideSlots;
-        // species by slot: LDS when the component fits, else sp_out itself.
-        // Four slots per lane per pass with every load issued before the
-        // stores, so a pass costs one memory round trip.
-        if (in_lds) {
-            for (uint32_t t0 = lane; t0 < psz; t0 += 256) {
-                uint32_t x[4];
-                int8_t v[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) x[q] = t0 + 64 * q < psz ? ids[pb + t0 + 64 * q] : kUnset;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) v[q] = x[q] < (uint64_t)n ? sp_in[x[q]] : (int8_t)0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (t0 + 64 * q < psz) slots[t0 + 64 * q] = v[q];
-            }
-        } else if (sp_in != sp_out) {
-            for (uint32_t t = lane; t < psz; t += 64) {
-                const uint32_t x = ids[pb + t];
-                if (x < (uint64_t)n) sp_out[x] = sp_in[x];
-            }
+        const uint32_t qb = dsc.x, np = dsc.y, psz = dsc.w;
+        if (np > kLevelPairs || psz > kLevelSlots || psz == 0) {
+            if (lane == 0) A.depth[ro + c] = 0;  // windowed replay
+            continue;
         }
+        for (uint32_t t = lane; t < psz; t += 64) last[t] = 0;
+        for (uint32_t t = lane; t < np; t += 64) cnt[t] = 0;
         __syncthreads();
-        uint2 r_next = lane < np ? rec[qb + lane] : make_uint2(0u, 0u);
-        unsigned long long d_next = lane < np ? deps[qb + lane] : 0ull;
+        uint32_t dmax = 0;
         for (uint32_t w = 0; w < np; w += 64) {
             const bool live = w + lane < np;
-            const uint2 r = r_next;
-            const unsigned long long dep = d_next;
-            if (w + 64 + lane < np) {  // next window's records, while this one replays
-                r_next = rec[qb + w + 64 + lane];
-                d_next = deps[qb + w + 64 + lane];
-            }
-            const uint32_t fb = r.x >> 29;
-            const uint32_t a = min(r.x & kSlotMask, psz - 1), b = min(r.y, psz - 1);  // slots are < psz
+            const uint2 r = live ? rec[qb + w + lane] : make_uint2(0u, 0u);
+            const unsigned long long dep = live ? deps[qb + w + lane] : 0ull;
+            const uint32_t a = min(r.x & kSlotMask, psz - 1), b = min(r.y, psz - 1);
             const unsigned long long all = __ballot(live);
             unsigned long long done = 0ull;
             bool mine = live;
-            if (in_lds) {
-                while (done != all) {
-                    const bool ready = mine && (dep & ~done) == 0ull;
-                    if (ready) {
-                        int si = slots[a], sj = slots[b];
-                        play_pair(si, sj, fb);
-                        slots[a] = (int8_t)si;
-                        slots[b] = (int8_t)sj;
-                        mine = false;
-                    }
-                    done |= __ballot(ready);
-                    __builtin_amdgcn_wave_barrier();
+            while (done != all) {
+                const bool ready = mine && (dep & ~done) == 0ull;
+                if (ready) {
+                    const uint32_t lv = (uint32_t)max(last[a], last[b]) + 1u;
+                    last[a] = (uint16_t)lv;
+                    last[b] = (uint16_t)lv;
+                    lvl[w + lane] = (uint16_t)lv;
+                    dmax = max(dmax, lv);
+                    mine = false;
                 }
-            } else {
-                while (done != all) {
-                    const bool ready = mine && (dep & ~done) == 0ull;
-                    if (ready) {
-                        const uint32_t xa = ids[pb + a], xb = ids[pb + b];
-                        if (xa < (uint64_t)n && xb < (uint64_t)n) {
-                            int si = sp_out[xa], sj = sp_out[xb];
-                            play_pair(si, sj, fb);
-                            sp_out[xa] = (int8_t)si;
-                            sp_out[xb] = (int8_t)sj;
-                        }
-                        mine = false;
-                    }
-                    done |= __ballot(ready);
-                    __syncthreads();  // this round's stores before the next round's loads
-                }
+                done |= __ballot(ready);
+                __builtin_amdgcn_wave_barrier();
             }
         }
-        if (in_lds) {
-            for (uint32_t t0 = lane; t0 < psz; t0 += 256) {
-                uint32_t x[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) x[q] = t0 + 64 * q < psz ? ids[pb + t0 + 64 * q] : kUnset;
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (x[q] < (uint64_t)n) sp_out[x[q]] = slots[t0 + 64 * q];
+        for (int d = 32; d >= 1; d >>= 1) dmax = max(dmax, (uint32_t)__shfl_xor(dmax, d, 64));
+        __syncthreads();
+        for (uint32_t q = lane; q < np; q += 64) atomicAdd(&cnt[lvl[q] - 1u], 1u);
+        __syncthreads();
+        // exclusive offsets per level (kept in cnt as cursors) and level ends
+        uint32_t carry = 0;
+        for (uint32_t l0 = 0; l0 < dmax; l0 += 64) {
+            const bool in = l0 + lane < dmax;
+            const uint32_t v = in ? cnt[l0 + lane] : 0u;
+            const uint32_t inc = wave_inclusive_scan(v);
+            if (in) {
+                cnt[l0 + lane] = carry + inc - v;
+                A.lend[po + qb + l0 + lane] = carry + inc;
             }
+            carry += __shfl(inc, 63, 64);
         }
         __syncthreads();
+        for (uint32_t q = lane; q < np; q += 64) {
+            const uint32_t pos = atomicAdd(&cnt[lvl[q] - 1u], 1u);
+            if (pos < np) A.lrec[po + qb + pos] = rec[qb + q];
+        }
+        if (lane == 0) A.depth[ro + c] = dmax;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// play (one step)
+// ---------------------------------------------------------------------------
+// One launch per step, blocks of one wavefront in three roles, ordered so
+// the longest work starts first:
+//   [0, nwide)                 one wavefront per wide component (levels, or
+//                              windows of dependency rounds past kLevelPairs)
+//   [nwide, nwide + nnarrow)   one lane per small component
+//   the rest                   particles without pairs keep their species
+// A wavefront's LDS operations complete in program order, so no role needs
+// a barrier between its LDS writes and later reads by other lanes.
+constexpr uint32_t kWaveSlots = (kLevelSlots + 15u) & ~15u;   // LDS species slots of a wide component
+constexpr unsigned kPlayWideBlocks = 2048;
+constexpr unsigned kPlayNarrowBlocks = 2048;
+constexpr int kCopyItems = 8;                                 // particles per thread in the copy role
+static_assert(64 * kNarrowSlots <= kWaveSlots, "small components share the slot buffer");
+
+__device__ __forceinline__ void play_slots(int8_t* slots, uint2 r, uint32_t top) {
+    const uint32_t a = min(r.x & kSlotMask, top), b = min(r.y, top);
+    int si = slots[a], sj = slots[b];
+    play_pair(si, sj, r.x >> 29);
+    slots[a] = (int8_t)si;
+    slots[b] = (int8_t)sj;
+}
+
+// Levelled component: species in LDS by slot; the pairs of a level share no
+// particle, so a level plays at once, in rows of at most 64 pairs (a wider
+// level is several rows, which are independent of each other).  The records
+// of kRowBatch rows go straight from memory into registers, the next batch
+// loading while the current one plays, so a row costs one LDS round trip
+// (its slots) on the critical path; the wavefront's LDS operations complete
+// in program order, so a row reads what the rows before it wrote.
+constexpr int kRowBatch = 16;
+
+struct RowCursor {
+    uint32_t l, lb, lstart, off;  // level, first level held in `ends`, level start, next row within it
+};
+
+// Row descriptors of the next kRowBatch rows (count 0 past the end) and their
+// records, one per lane.  `ends` holds the ends of levels lb .. lb + 63; it is
+// refilled, if needed, before any record load of the batch is issued (a batch
+// advances at most kRowBatch levels), and the loads are unconditional
+// (clamped), so no wait for them is placed before the batch plays.
+__device__ __forceinline__ void next_rows(RowCursor& rc, uint32_t& ends, const uint2* __restrict__ lrec,
+                                          const uint32_t* __restrict__ lend, uint32_t np, uint32_t depth,
+                                          uint32_t (&cnt)[kRowBatch], uint2 (&rr)[kRowBatch]) {
+    const uint32_t lane = threadIdx.x;
+    if (rc.l - rc.lb + kRowBatch > 64) {
+        rc.lb = rc.l;
+        ends = rc.lb + lane < depth ? lend[rc.lb + lane] : np;
+    }
+    uint32_t pos[kRowBatch];
+#pragma unroll
+    for (int q = 0; q < kRowBatch; ++q) {
+        const uint32_t at = rc.lstart + rc.off;
+        const uint32_t le = max(min((uint32_t)__builtin_amdgcn_readlane((int)ends, (int)(rc.l - rc.lb)), np), at + 1u);
+        const bool live = at < np;
+        pos[q] = live ? at : 0u;
+        cnt[q] = live ? min(64u, le - at) : 0u;
+        const bool level_done = live && at + cnt[q] >= le;
+        rc.off = level_done ? 0u : (live ? rc.off + 64u : rc.off);
+        rc.lstart = level_done ? le : rc.lstart;
+        rc.l += level_done ? 1u : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < kRowBatch; ++q) rr[q] = lrec[min(pos[q] + lane, np - 1u)];
+}
+
+__device__ __forceinline__ void play_rows(int8_t* slots, const uint32_t (&cnt)[kRowBatch],
+                                          const uint2 (&rr)[kRowBatch], uint32_t top) {
+    const uint32_t lane = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < kRowBatch; ++q) {
+        if (lane < cnt[q]) play_slots(slots, rr[q], top);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+__device__ void play_levels(int64_t n, const uint32_t* __restrict__ ids, uint32_t psz,
+                            const uint2* __restrict__ lrec, const uint32_t* __restrict__ lend, uint32_t np,
+                            uint32_t depth, const int8_t* __restrict__ sp_in, int8_t* __restrict__ sp_out,
+                            int8_t* slots) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t top = psz - 1;  // 1 <= psz <= kLevelSlots
+    uint32_t ends = lane < depth ? lend[lane] : np;  // ends of levels lb .. lb + 63
+    uint32_t x[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) x[q] = lane + 64 * q < psz ? ids[lane + 64 * q] : kUnset;
+    RowCursor rc{0u, 0u, 0u, 0u};
+    uint32_t ca[kRowBatch], cb[kRowBatch];
+    uint2 ra[kRowBatch], rb[kRowBatch];
+    next_rows(rc, ends, lrec, lend, np, depth, ca, ra);
+    int8_t v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = x[q] < (uint64_t)n ? sp_in[x[q]] : (int8_t)0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (lane + 64 * q < psz) slots[lane + 64 * q] = v[q];
+    for (uint32_t t0 = 256 + lane; t0 < psz; t0 += 64) {  // more than 256 particles
+        const uint32_t xt = ids[t0];
+        slots[t0] = xt < (uint64_t)n ? sp_in[xt] : (int8_t)0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    while (ca[0] > 0) {
+        next_rows(rc, ends, lrec, lend, np, depth, cb, rb);
+        play_rows(slots, ca, ra, top);
+        if (cb[0] == 0) break;
+        next_rows(rc, ends, lrec, lend, np, depth, ca, ra);
+        play_rows(slots, cb, rb, top);
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        if (x[q] < (uint64_t)n) sp_out[x[q]] = slots[lane + 64 * q];
+    for (uint32_t t0 = 256 + lane; t0 < psz; t0 += 64) {
+        const uint32_t xt = ids[t0];
+        if (xt < (uint64_t)n) sp_out[xt] = slots[t0];
+    }
+}
+
+// Component past the level limits: windows of 64 pairs, each replayed in
+// dependency rounds (a pair fires once every earlier pair of its window that
+// shares a slot has fired); species in LDS when they fit, else in sp_out.
+__device__ void play_windows(int64_t n, const uint32_t* __restrict__ ids, uint32_t psz,
+                             const uint2* __restrict__ rec, const unsigned long long* __restrict__ deps,
+                             uint32_t np, const int8_t* __restrict__ sp_in, int8_t* __restrict__ sp_out,
+                             int8_t* slots) {
+    const uint32_t lane = threadIdx.x;
+    const bool in_lds = psz <= kWaveSlots;
+    for (uint32_t t = lane; t < psz; t += 64) {
+        const uint32_t x = ids[t];
+        const int8_t v = x < (uint64_t)n ? sp_in[x] : (int8_t)0;
+        if (in_lds)
+            slots[t] = v;
+        else if (x < (uint64_t)n)
+            sp_out[x] = v;
+    }
+    __syncthreads();  // (one wavefront) the copies above are visible to every lane
+    uint2 r_next = lane < np ? rec[lane] : make_uint2(0u, 0u);
+    unsigned long long d_next = lane < np ? deps[lane] : 0ull;
+    for (uint32_t w = 0; w < np; w += 64) {
+        const bool live = w + lane < np;
+        const uint2 r = r_next;
+        const unsigned long long dep = d_next;
+        if (w + 64 + lane < np) {  // next window's records, while this one replays
+            r_next = rec[w + 64 + lane];
+            d_next = deps[w + 64 + lane];
+        }
+        const uint32_t fb = r.x >> 29;
+        const uint32_t a = min(r.x & kSlotMask, psz - 1), b = min(r.y, psz - 1);  // slots are < psz
+        const unsigned long long all = __ballot(live);
+        unsigned long long done = 0ull;
+        bool mine = live;
+        while (done != all) {
+            const bool ready = mine && (dep & ~done) == 0ull;
+            if (ready) {
+                if (in_lds) {
+                    int si = slots[a], sj = slots[b];
+                    play_pair(si, sj, fb);
+                    slots[a] = (int8_t)si;
+                    slots[b] = (int8_t)sj;
+                } else {
+                    const uint32_t xa = ids[a], xb = ids[b];
+                    if (xa < (uint64_t)n && xb < (uint64_t)n) {
+                        int si = sp_out[xa], sj = sp_out[xb];
+                        play_pair(si, sj, fb);
+                        sp_out[xa] = (int8_t)si;
+                        sp_out[xb] = (int8_t)sj;
+                    }
+                }
+                mine = false;
+            }
+            done |= __ballot(ready);
+            if (in_lds)
+                __builtin_amdgcn_wave_barrier();
+            else
+                __syncthreads();  // this round's global stores before the next round's loads
+        }
+    }
+    if (in_lds) {
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t t = lane; t < psz; t += 64) {
+            const uint32_t x = ids[t];
+            if (x < (uint64_t)n) sp_out[x] = slots[t];
+        }
+    }
+}
+
+// Small component, one lane: its slots in LDS, its records 8 at a time with
+// the next 8 loading while the current ones play.
+__device__ __forceinline__ void play_narrow(int64_t n, const uint32_t* __restrict__ ids, const uint2* __restrict__ rec,
+                                            uint4 dsc, const int8_t* __restrict__ sp_in,
+                                            int8_t* __restrict__ sp_out, int8_t* my) {
+    const uint32_t qb = dsc.x, np = min(dsc.y, kNarrowPairs);
+    const uint32_t pb = dsc.z, psz = min(dsc.w, kNarrowSlots);
+    uint32_t id[kNarrowSlots];
+#pragma unroll
+    for (uint32_t t = 0; t < kNarrowSlots; ++t) id[t] = t < psz ? ids[pb + t] : kUnset;
+    uint2 r[8];
+#pragma unroll
+    for (uint32_t q = 0; q < 8; ++q) r[q] = q < np ? rec[qb + q] : make_uint2(0u, 0u);
+#pragma unroll
+    for (uint32_t t = 0; t < kNarrowSlots; ++t)
+        if (t < psz) my[t] = id[t] < (uint64_t)n ? sp_in[id[t]] : (int8_t)0;
+    for (uint32_t q0 = 0; q0 < np; q0 += 8) {
+        uint2 rn[8];
+#pragma unroll
+        for (uint32_t q = 0; q < 8; ++q) rn[q] = q0 + 8 + q < np ? rec[qb + q0 + 8 + q] : make_uint2(0u, 0u);
+#pragma unroll
+        for (uint32_t q = 0; q < 8; ++q)
+            if (q0 + q < np) play_slots(my, r[q], kNarrowSlots - 1);
+#pragma unroll
+        for (uint32_t q = 0; q < 8; ++q) r[q] = rn[q];
+    }
+#pragma unroll
+    for (uint32_t t = 0; t < kNarrowSlots; ++t)
+        if (t < psz && id[t] < (uint64_t)n) sp_out[id[t]] = my[t];
+}
+
+__global__ void __launch_bounds__(64)
+rps_play_kernel(int64_t n, int64_t s, RpsArrays A, RpsStrides st, const int8_t* sp_in, int8_t* sp_out,
+                unsigned nwide, unsigned nnarrow) {
+    __shared__ int8_t slots[kWaveSlots];
+    const int64_t ro = s * st.row, po = s * st.pair;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t bid = blockIdx.x;
+    if (bid < nwide) {
+        const uint32_t cnt = A.lists[4 * s + 1];
+        for (uint32_t c = bid; c < cnt; c += nwide) {
+            const uint4 dsc = A.wide[ro + c];
+            const uint32_t qb = dsc.x, np = dsc.y, pb = dsc.z, psz = dsc.w;
+            const uint32_t depth = A.depth[ro + c];
+            if (depth > 0 && psz >= 1 && psz <= kLevelSlots)
+                play_levels(n, A.ids + ro + pb, psz, A.lrec + po + qb, A.lend + po + qb, np, depth, sp_in, sp_out,
+                            slots);
+            else if (psz >= 1)
+                play_windows(n, A.ids + ro + pb, psz, A.rec + po + qb, A.deps + po + qb, np, sp_in, sp_out, slots);
+            __syncthreads();  // (one wavefront) LDS free for the next component
+        }
+        return;
+    }
+    if (bid < nwide + nnarrow) {
+        const uint32_t cnt = A.lists[4 * s + 0];
+        int8_t* my = slots + lane * kNarrowSlots;
+        for (uint32_t c = (bid - nwide) * 64 + lane; c < cnt; c += nnarrow * 64)
+            play_narrow(n, A.ids + ro, A.rec + po, A.narrow[ro + c], sp_in, sp_out, my);
+        return;
+    }
+    // particles without pairs (loc unset) keep their species; the others are
+    // written by their component
+    if (sp_in == sp_out) return;
+    const uint32_t* loc = A.loc + ro;
+    const int64_t x0 = (int64_t)kCopyItems * ((int64_t)(bid - nwide - nnarrow) * 64 + lane);
+    if (x0 + kCopyItems <= n) {
+        const uint4 l0 = *reinterpret_cast<const uint4*>(loc + x0);  // rows are 16-byte aligned
+        const uint4 l1 = *reinterpret_cast<const uint4*>(loc + x0 + 4);
+        int8_t v[kCopyItems];
+#pragma unroll
+        for (int q = 0; q < kCopyItems; ++q) v[q] = sp_in[x0 + q];
+        const uint32_t lv[kCopyItems] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+#pragma unroll
+        for (int q = 0; q < kCopyItems; ++q)
+            if (lv[q] == kUnset) sp_out[x0 + q] = v[q];
+    } else {
+        for (int64_t x = x0; x < n; ++x)
+            if (loc[x] == kUnset) sp_out[x] = sp_in[x];
     }
 }
 
@@ -557,9 +751,13 @@ static void rps_free_pairs(lm_rps_ws* w) {
     (void)hipFree(A.bits);
     (void)hipFree(A.rec);
     (void)hipFree(A.deps);
+    (void)hipFree(A.lrec);
+    (void)hipFree(A.lend);
     A.bits = nullptr;
     A.rec = nullptr;
     A.deps = nullptr;
+    A.lrec = nullptr;
+    A.lend = nullptr;
     w->bytes -= w->pair_bytes;
     w->pair_bytes = 0;
     w->pair_cap = 0;
@@ -576,6 +774,7 @@ static void rps_ws_free(lm_rps_ws* w) {
     (void)hipFree(A.narrow);
     (void)hipFree(A.wide);
     (void)hipFree(A.plen);
+    (void)hipFree(A.depth);
     lm::sort_scratch_free(&w->sort);
     rps_free_pairs(w);
     delete w;
@@ -591,7 +790,8 @@ static int rps_ensure_pairs(lm_rps_ws* w, int64_t cap, hipStream_t s) {
     int64_t b = 0;
     int rc;
     if ((rc = rws_alloc(&A.bits, S * c, &b)) || (rc = rws_alloc(&A.rec, S * c, &b)) ||
-        (rc = rws_alloc(&A.deps, S * c, &b))) {
+        (rc = rws_alloc(&A.deps, S * c, &b)) || (rc = rws_alloc(&A.lrec, S * c, &b)) ||
+        (rc = rws_alloc(&A.lend, S * c, &b))) {
         w->bytes += b;
         w->pair_bytes = b;
         rps_free_pairs(w);
@@ -631,7 +831,7 @@ int lm_rps_ws_create(int64_t n_max, int32_t steps_max, lm_rps_ws** out) {
         (rc = rws_alloc(&A.pval, R, &w->bytes)) || (rc = rws_alloc(&A.ptk, R, &w->bytes)) ||
         (rc = rws_alloc(&A.ptv, R, &w->bytes)) || (rc = rws_alloc(&A.narrow, R, &w->bytes)) ||
         (rc = rws_alloc(&A.wide, R, &w->bytes)) || (rc = rws_alloc(&A.lists, 4 * S, &w->bytes)) ||
-        (rc = rws_alloc(&A.plen, S, &w->bytes)) ||
+        (rc = rws_alloc(&A.plen, S, &w->bytes)) || (rc = rws_alloc(&A.depth, R, &w->bytes)) ||
         (rc = rws_alloc(&w->scan_tmp, S * lm::scan_num_blocks(n_max), &w->bytes)) ||
         (rc = lm::sort_scratch_alloc(&w->sort, n_max, steps_max))) {
         rps_ws_free(w);
@@ -706,6 +906,8 @@ int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair
         LM_LAUNCH_CHECK();
         LM_LAUNCH(lm::K_RPS_DEPS, lm::rps_deps_kernel, dim3(1024, S), dim3(64), 0, s, n, A, sp);
         LM_LAUNCH_CHECK();
+        LM_LAUNCH(lm::K_RPS_LEVELS, lm::rps_levels_kernel, dim3(1024, S), dim3(64), 0, s, n, A, sp);
+        LM_LAUNCH_CHECK();
         ws->pair_stride_used = pair_stride;
     } else {
         ws->pair_stride_used = 0;
@@ -725,15 +927,12 @@ int lm_rps_play(lm_rps_ws* ws, int32_t step, int64_t n, const int8_t* species_in
     LM_REQUIRE(species_in && species_out, LM_ERR_ARG, "NULL species");
     hipStream_t s = (hipStream_t)stream;
     lm::RpsStrides sp{ws->row, ws->pair_stride_used};
-    const unsigned ncopy = lm::grid_for((n + 3) / 4);  // one thread per four particles
-    LM_LAUNCH(lm::K_RPS_PLAY, lm::rps_play_narrow_kernel, dim3(ncopy + 512), dim3(lm::kBlock), 0, s, n,
-              (int64_t)step, ws->A, sp, species_in, species_out, ncopy);
+    const unsigned nwide = ws->pair_stride_used > 0 ? lm::kPlayWideBlocks : 0u;
+    const unsigned nnarrow = ws->pair_stride_used > 0 ? lm::kPlayNarrowBlocks : 0u;
+    const unsigned ncopy = (unsigned)((n + 64 * lm::kCopyItems - 1) / (64 * lm::kCopyItems));
+    LM_LAUNCH(lm::K_RPS_PLAY, lm::rps_play_kernel, dim3(nwide + nnarrow + ncopy), dim3(64), 0, s, n,
+              (int64_t)step, ws->A, sp, species_in, species_out, nwide, nnarrow);
     LM_LAUNCH_CHECK();
-    if (ws->pair_stride_used > 0) {
-        LM_LAUNCH(lm::K_RPS_PLAY_WIDE, lm::rps_play_wide_kernel, dim3(lm::kWideBlocks), dim3(64), 0, s, n,
-                  (int64_t)step, ws->A, sp, species_in, species_out);
-        LM_LAUNCH_CHECK();
-    }
     return LM_OK;
 }
 

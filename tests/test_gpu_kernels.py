@@ -197,9 +197,10 @@ def test_rps_matches_sequential_oracle(gpu, seed, n, m, clustered):
     assert st.pairs == m and st.components >= 1 and st.touched <= n
 
 
-def test_rps_giant_chain_component(gpu):
-    # one component of 20,000 pairs: a chain plus random chords
-    n = 10000
+@pytest.mark.parametrize("n", [10000, 20000])
+def test_rps_giant_chain_component(gpu, n):
+    # one component of 20,000 pairs: a chain plus random chords (n = 20000:
+    # more particles than the replay keeps in LDS, so it runs in global memory)
     rng = np.random.default_rng(9)
     pairs = {(k, k + 1) for k in range(n - 1)}
     while len(pairs) < 20000:
