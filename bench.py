@@ -58,12 +58,13 @@ def copy_calibration(nbytes):
 
 
 def run_steps(runner, k, total):
-    """Run k steps from the runner's state, restarting the workload at its end."""
+    """Run k steps from the runner's state (chunks pipelined), restarting the
+    workload at its end."""
     done = 0
     while done < k:
         if runner.step >= total:
             runner.reset()
-        done += runner.advance(min(k - done, total - runner.step))
+        done += runner.run(min(k - done, total - runner.step))
     return done
 
 

@@ -23,26 +23,85 @@ from . import device as dev
 
 class InteractionEngine:
     """Pair search + rock-paper-scissors for n particles on the device, run
-    over batches of up to steps_max steps."""
+    over batches of up to steps_max steps.
 
-    def __init__(self, n, bbox, radius, norm=2.0, device=None, steps_max=1, initial_capacity=None):
+    ``nsets`` independent sets of pair buffers and grouping workspaces let a
+    caller prepare one batch (pair search and grouping, which need positions
+    only) while another batch replays (which needs the species): set b is
+    addressed by the ``*_set`` methods; ``run_chunk`` uses set 0.
+    """
+
+    def __init__(self, n, bbox, radius, norm=2.0, device=None, steps_max=1, initial_capacity=None, nsets=1):
         self.n = int(n)
         self.radius = float(radius)
         self.norm = float(norm)
         self.steps_max = int(steps_max)
         self.device = torch.device(device or "cuda")
         self.search = dev.PairSearch(self.n, bbox, self.radius, self.device, self.steps_max)
-        self.rps = dev.RPSUpdater(self.n, self.device, self.steps_max)
-        self.npairs = torch.zeros(self.steps_max, dtype=torch.int64, device=self.device)
+        self.nsets = int(nsets)
+        self.rps_sets = [dev.RPSUpdater(self.n, self.device, self.steps_max) for _ in range(self.nsets)]
+        self.npairs_sets = [torch.zeros(self.steps_max, dtype=torch.int64, device=self.device)
+                            for _ in range(self.nsets)]
+        self.pair_sets = [None] * self.nsets
         self.capacity = 0
         self._grow(initial_capacity or max(2 * self.n, 1024))
+
+    # set 0 under the single-batch names
+    @property
+    def rps(self):
+        return self.rps_sets[0]
+
+    @property
+    def npairs(self):
+        return self.npairs_sets[0]
+
+    @property
+    def pair_i(self):
+        return self.pair_sets[0][0]
+
+    @property
+    def pair_j(self):
+        return self.pair_sets[0][1]
 
     def _grow(self, cap):
         cap = int(cap)
         if cap > self.capacity:
             self.capacity = cap
-            self.pair_i = torch.empty((self.steps_max, cap), dtype=torch.int32, device=self.device)
-            self.pair_j = torch.empty((self.steps_max, cap), dtype=torch.int32, device=self.device)
+            self.pair_sets = [(torch.empty((self.steps_max, cap), dtype=torch.int32, device=self.device),
+                               torch.empty((self.steps_max, cap), dtype=torch.int32, device=self.device))
+                              for _ in range(self.nsets)]
+
+    def prepare_set(self, b, rec_x, rec_y, params, key, step0):
+        """Pair search and grouping of S steps into set b (asynchronous)."""
+        S = rec_x.shape[0]
+        if S > self.steps_max:
+            raise ValueError(f"{S} steps exceed the engine's batch of {self.steps_max}")
+        pi, pj = (t[:S] for t in self.pair_sets[b])
+        cnt = self.npairs_sets[b][:S]
+        self.search.query_batch(rec_x, rec_y, self.radius, self.norm, pi, pj, cnt)
+        self.rps_sets[b].prepare(self.n, pi, pj, cnt, params["pRS"], params["pPR"], params["pSP"], key, step0)
+
+    def worst_count(self, b, S):
+        """Largest pair count of set b's S steps (synchronises the stream)."""
+        return int(self.npairs_sets[b][:S].max().item()) if S > 0 else 0
+
+    def fits(self, worst):
+        """True if the pair buffers held every step; otherwise they are grown
+        (the caller prepares the affected sets again)."""
+        if worst <= self.capacity:
+            return True
+        self._grow(worst + worst // 4 + 1024)
+        return False
+
+    def play_set(self, b, S, species, rec_species):
+        """Replay set b's S steps: rec_species[s] = species after step s, and
+        species ends as the state after the last step (asynchronous)."""
+        sp_in = species
+        for s in range(S):
+            self.rps_sets[b].play(s, sp_in, rec_species[s])
+            sp_in = rec_species[s]
+        if S > 0:
+            species.copy_(rec_species[S - 1])
 
     def run_chunk(self, rec_x, rec_y, species, params, key, step0, rec_species=None, npairs=None):
         """Steps s = 0..S-1 over recorded float32 positions rec_x[s], rec_y[s] ([S, n]).
@@ -61,22 +120,19 @@ class InteractionEngine:
             raise ValueError(f"{S} steps exceed the engine's batch of {self.steps_max}")
         start = species.clone() if rec_species is None else None
         while True:
-            pi, pj, cnt = self.pair_i[:S], self.pair_j[:S], self.npairs[:S]
-            self.search.query_batch(rec_x, rec_y, self.radius, self.norm, pi, pj, cnt)
-            self.rps.prepare(self.n, pi, pj, cnt, params["pRS"], params["pPR"], params["pSP"], key, step0)
+            self.prepare_set(0, rec_x, rec_y, params, key, step0)
+            cnt = self.npairs[:S]
             sp_in = species
             for s in range(S):
                 out = rec_species[s] if rec_species is not None else species
                 self.rps.play(s, sp_in, out)
                 sp_in = out
-            worst = int(cnt.max().item())
-            if worst <= self.capacity:
+            if self.fits(self.worst_count(0, S)):
                 if rec_species is not None:
                     species.copy_(rec_species[S - 1])
                 if npairs is not None:
                     npairs.copy_(cnt)
                 return cnt
-            self._grow(worst + worst // 4 + 1024)
             if start is not None:
                 species.copy_(start)
 

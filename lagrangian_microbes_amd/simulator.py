@@ -51,7 +51,7 @@ class FusedRunner:
         self.walk_key = tuple(walk_key)
         self.dfield = dev.DeviceField(field, self.device)
         self.engine = InteractionEngine(self.n, bbox or field.bounds, radius, norm, self.device,
-                                        steps_max=self.chunk)
+                                        steps_max=self.chunk, nsets=2)
         self.rec_x = torch.empty((self.chunk, self.n), dtype=torch.float32, device=self.device)
         self.rec_y = torch.empty((self.chunk, self.n), dtype=torch.float32, device=self.device)
         self.rec_s = torch.empty((self.chunk, self.n), dtype=torch.int8, device=self.device)
@@ -86,6 +86,87 @@ class FusedRunner:
         self.step += k
         self.clock_steps += k
         return k
+
+    def run(self, nsteps, on_chunk=None):
+        """Run nsteps steps with consecutive chunks overlapped on the device.
+
+        Advection, pair search and grouping need positions only, so chunk
+        c + 1 is prepared on a side stream while chunk c replays the species
+        on the caller's stream (two sets of chunk buffers alternate).  After
+        each chunk, ``on_chunk(k, rec_x, rec_y, rec_s, npairs)`` receives that
+        chunk's records (device views, valid until the callback returns).
+        The state and step counters end as after ``advance`` calls over the
+        same steps; returns the number of steps done.
+        """
+        total = max(0, int(nsteps))
+        if total == 0:
+            return 0
+        self._ensure_pipeline()
+        eng = self.engine
+        main = torch.cuda.current_stream(self.device)
+        side = self._side
+        sizes = []
+        while sum(sizes) < total:
+            sizes.append(min(self.chunk, total - sum(sizes)))
+        step0 = [self.step + sum(sizes[:c]) for c in range(len(sizes))]
+        clock0 = [self.clock_steps + sum(sizes[:c]) for c in range(len(sizes))]
+        prepared = [None] * len(sizes)   # event after chunk c's preparation
+        played = [None, None]            # event after the replay that last used set b
+        side.wait_stream(main)           # the state as the caller left it
+
+        def group(c):
+            b, k = c % 2, sizes[c]
+            with torch.cuda.stream(side):
+                eng.prepare_set(b, self._rec_sets[b][0][:k], self._rec_sets[b][1][:k], self.params, self.key,
+                                step0[c])
+                ev = torch.cuda.Event()
+                ev.record(side)
+                prepared[c] = ev
+
+        def prepare(c):
+            b, k = c % 2, sizes[c]
+            with torch.cuda.stream(side):
+                if played[b] is not None:
+                    side.wait_event(played[b])
+                stage, stride = self.dfield.stage_fields(clock0[c] * self.dt_s, self.dt_s, k)
+                dev.advect_rk4_f64(self.lon, self.lat, stage, stride, self.dfield.grid, k, self.dt_s,
+                                   self._rec_sets[b][0], self._rec_sets[b][1], self.n, self.walk_amp,
+                                   self.walk_key, step0[c], self.oob)
+            group(c)
+
+        prepare(0)
+        for c, k in enumerate(sizes):
+            b = c % 2
+            if c + 1 < len(sizes):
+                prepare(c + 1)
+            main.wait_event(prepared[c])
+            worst = eng.worst_count(b, k)
+            if worst > eng.capacity:
+                # larger pair buffers for every set, then the prepared chunks
+                # are grouped again from their records (positions are kept)
+                torch.cuda.synchronize(self.device)
+                eng.fits(worst)
+                group(c)
+                if c + 1 < len(sizes):
+                    group(c + 1)
+                main.wait_event(prepared[c])
+            rs = self._rec_sets[b][2][:k]
+            eng.play_set(b, k, self.species, rs)
+            if on_chunk is not None:
+                on_chunk(k, self._rec_sets[b][0][:k], self._rec_sets[b][1][:k], rs, eng.npairs_sets[b][:k])
+            ev = torch.cuda.Event()  # after the callback's own work on this stream
+            ev.record(main)
+            played[b] = ev
+            self.step += k
+            self.clock_steps += k
+        main.wait_stream(side)
+        return total
+
+    def _ensure_pipeline(self):
+        if getattr(self, "_side", None) is None:
+            second = tuple(torch.empty_like(t) for t in (self.rec_x, self.rec_y, self.rec_s))
+            self._rec_sets = [(self.rec_x, self.rec_y, self.rec_s), second]
+            self._side = torch.cuda.Stream(self.device)
 
     def check_domain(self):
         n_oob = int(self.oob.item())
@@ -144,14 +225,17 @@ class MicrobeSimulator:
         self.pairs_per_step = np.zeros(iters, dtype=np.int64)
         tic = time()
         done = 0
-        while done < iters:
-            k = runner.advance(iters - done)
-            runner.check_domain()
-            lon[:, done:done + k] = runner.rec_x[:k].cpu().numpy().T
-            lat[:, done:done + k] = runner.rec_y[:k].cpu().numpy().T
-            spc[:, done:done + k] = runner.rec_s[:k].cpu().numpy().T
-            self.pairs_per_step[done:done + k] = runner.npairs[:k].cpu().numpy()
+
+        def keep(k, rx, ry, rs, cnt):
+            nonlocal done
+            lon[:, done:done + k] = rx.cpu().numpy().T
+            lat[:, done:done + k] = ry.cpu().numpy().T
+            spc[:, done:done + k] = rs.cpu().numpy().T
+            self.pairs_per_step[done:done + k] = cnt.cpu().numpy()
             done += k
+
+        runner.run(iters, on_chunk=keep)
+        runner.check_domain()
         logger.info("Fused advection + interactions, {:d} steps: {:s}".format(iters, pretty_time(time() - tic)))
         self.microbe_properties["species"][:] = runner.species.cpu().numpy()
         self.final_lons = runner.lon.cpu().numpy()

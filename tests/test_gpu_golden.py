@@ -62,3 +62,35 @@ def test_c1_fused_runner_fixture(gpu):
     r.check_domain()
     assert np.array_equal(r.lon.cpu().numpy(), g["lon"]) and np.array_equal(r.lat.cpu().numpy(), g["lat"])
     assert np.array_equal(r.species.cpu().numpy(), g["species"])
+
+
+@pytest.mark.parametrize("chunk,grow", [(7, False), (10, True), (24, False)])
+def test_c1_pipelined_run_fixture(gpu, chunk, grow):
+    """Chunks overlapped on two streams give the same records and end state,
+    also when the pair buffers must grow in the middle of the run."""
+    g = load("c1_24h")
+    w = configs.rps_10k_steady()
+    lon, lat = w.particles()
+    steps = len(g["npairs"])
+    r = FusedRunner(lon, lat, w.species(), w.velocity_field(), w.radius, w.params, w.key, w.dt, chunk=chunk)
+    if grow:
+        r.engine.capacity = 0
+        r.engine._grow(64)  # far too small: most steps after the first hours overflow
+    seen = []
+
+    def check(k, rx, ry, rs, cnt):
+        base = len(seen)
+        for s in range(k):
+            S = base + s
+            assert digest(rx[s].cpu().numpy(), ry[s].cpu().numpy()) == g["digest_records"][S], S
+            assert int(cnt[s].item()) == g["npairs"][S], S
+            assert digest(rs[s].cpu().numpy()) == g["digest_species"][S], S
+            seen.append(S)
+
+    assert r.run(steps, on_chunk=check) == steps
+    assert seen == list(range(steps)) and r.step == steps
+    r.check_domain()
+    assert np.array_equal(r.lon.cpu().numpy(), g["lon"]) and np.array_equal(r.lat.cpu().numpy(), g["lat"])
+    assert np.array_equal(r.species.cpu().numpy(), g["species"])
+    if grow:
+        assert r.engine.capacity >= int(g["npairs"].max())
