@@ -170,6 +170,23 @@ constexpr uint64_t kStreamWalk = 1;
 // with explicit fma.  Written out so CPU and GPU evaluate the same operations.
 // ----------------------------------------------------------------------------
 #define LM_DEG2RAD 0.017453292519943295
+// XCD-aware block order.  MI355X has 8 XCDs, each with its own L2, and hands
+// workgroups to them round-robin by linear workgroup id.  This maps the
+// dispatch order so that XCD x runs the x-th contiguous eighth of the
+// logical (blockIdx.x fastest, then blockIdx.y) block range: neighbouring
+// logical blocks, which touch neighbouring data, share an L2.
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ void xcd_block(uint32_t* bx, uint32_t* by) {
+    const uint32_t gx = gridDim.x, total = gridDim.x * gridDim.y;
+    const uint32_t id = blockIdx.x + blockIdx.y * gx;
+    const uint32_t x = id % kXcds, k = id / kXcds;
+    const uint32_t base = total / kXcds, extra = total % kXcds;
+    const uint32_t logical = x * base + (x < extra ? x : extra) + k;
+    *by = logical / gx;
+    *bx = logical % gx;
+}
+
 __host__ __device__ inline double cos_deg(double lat_deg) {
     const double x = lat_deg * LM_DEG2RAD;
     const double z = x * x;

@@ -48,6 +48,7 @@ struct Metric {
     double r;    // radius
     double r2;   // radius*radius (p = 2) or radius^p (general p)
     double p;
+    float box;   // float32 bound above the radius: |dx| or |dy| beyond it rules a pair out
 };
 
 // Per-step strides of a batch (elements).
@@ -145,13 +146,54 @@ bin_table_kernel(int64_t n, const float* __restrict__ x, const float* __restrict
         recs[s * st.rec + e] = make_float4(x[s * st.pos + i], y[s * st.pos + i], __uint_as_float(i), 0.0f);
 }
 
+// The (up to) six record ranges of the 3x3 stencil around (cx, cy): a row of
+// three cells is one bucket range, or two when it wraps around the table.
+// All bucket starts are loaded at once.  Ranges are clamped to nrec.
+__device__ __forceinline__ void stencil_ranges(const CellGrid& g, const uint32_t* __restrict__ start, uint32_t nrec,
+                                               int cx, int cy, uint32_t (&rb)[6], uint32_t (&re)[6]) {
+    const int c0 = cx > 0 ? cx - 1 : 0;
+    const int c1 = cx < g.nx - 1 ? cx + 1 : g.nx - 1;
+    uint32_t lo[6], hi[6];
+    bool on[6];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int row = cy + r - 1;
+        const bool valid = row >= 0 && row < g.ny;
+        const uint32_t b0 = valid ? bucket_of(g, c0, row) : 0u;
+        const uint32_t b1 = valid ? bucket_of(g, c1, row) : 0u;
+        const bool wrap = b0 > b1;
+        // first segment [b0, wrap ? T : b1 + 1), second [0, b1 + 1) if wrapped
+        lo[2 * r] = b0;
+        hi[2 * r] = wrap ? g.T : b1 + 1u;
+        on[2 * r] = valid;
+        lo[2 * r + 1] = 0u;
+        hi[2 * r + 1] = b1 + 1u;
+        on[2 * r + 1] = valid && wrap;
+    }
+    uint32_t sb[6], se[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+        sb[r] = on[r] ? start[lo[r]] : 0u;
+        se[r] = on[r] ? start[hi[r]] : 0u;
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+        re[r] = min(se[r], nrec);
+        rb[r] = min(sb[r], re[r]);
+    }
+}
+
+constexpr int kCand = 8;  // candidate records in flight per thread
+
 __global__ void __launch_bounds__(kBlock)
 pair_count_kernel(int64_t n, CellGrid g, Metric m, Strides st, const uint32_t* __restrict__ start_all,
                   const float4* __restrict__ recs_all, uint32_t* __restrict__ nbr_cnt,
                   uint32_t* __restrict__ nbr) {
-    const int64_t s0 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t bx, by;
+    xcd_block(&bx, &by);  // a particle's stencil rows lie a few blocks away in bucket order
+    const int64_t s0 = (int64_t)bx * kBlock + threadIdx.x;
     if (s0 >= n) return;
-    const int64_t s = blockIdx.y;
+    const int64_t s = by;
     const uint32_t* start = start_all + s * st.tab;
     const float4* recs = recs_all + s * st.rec;
     const float4 me = recs[s0];
@@ -159,16 +201,40 @@ pair_count_kernel(int64_t n, CellGrid g, Metric m, Strides st, const uint32_t* _
     if (i >= (uint64_t)n) return;
     int cx, cy;
     cell_of(g, me.x, me.y, &cx, &cy);
+    uint32_t rb[6], re[6];
+    stencil_ranges(g, start, (uint32_t)n, cx, cy, rb, re);
+    // the ranges as one sequence: candidate t lies in range r when
+    // pre[r] <= t < pre[r + 1]
+    uint32_t pre[7];
+    pre[0] = 0;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) pre[r + 1] = pre[r] + (re[r] - rb[r]);
+    const uint32_t total = pre[6];
     uint32_t cnt = 0;
     uint32_t* row = nbr + s * st.nbr + (int64_t)i * kNbr;
-    for_stencil(g, start, (uint32_t)n, cx, cy, [&](uint32_t q) {
-        const float4 o = recs[q];
-        const uint32_t j = __float_as_uint(o.z);
-        if (j > i && within(m, me.x, me.y, o.x, o.y)) {
-            if (cnt < kNbr) row[cnt] = j;
-            ++cnt;
+    for (uint32_t t0 = 0; t0 < total; t0 += kCand) {
+        float4 o[kCand];
+#pragma unroll
+        for (int u = 0; u < kCand; ++u) {
+            const uint32_t t = t0 + u;
+            uint32_t q = rb[0] + t;
+#pragma unroll
+            for (int r = 1; r < 6; ++r) q = t >= pre[r] ? rb[r] + (t - pre[r]) : q;
+            o[u] = t < total ? recs[q] : make_float4(0.f, 0.f, __uint_as_float(0u), 0.f);
         }
-    });
+#pragma unroll
+        for (int u = 0; u < kCand; ++u) {
+            const uint32_t j = __float_as_uint(o[u].z);
+            // |dx| <= r and |dy| <= r hold for every p >= 1; the float32
+            // differences of nearby positions are exact, and m.box is above
+            // r, so the prefilter only drops pairs the exact test drops
+            if (t0 + u < total && j > i && fabsf(o[u].x - me.x) <= m.box && fabsf(o[u].y - me.y) <= m.box &&
+                within(m, me.x, me.y, o[u].x, o[u].y)) {
+                if (cnt < kNbr) row[cnt] = j;
+                ++cnt;
+            }
+        }
+    }
     nbr_cnt[s * st.row + i] = cnt;
 }
 
@@ -351,7 +417,8 @@ int lm_pairs_ws_create(int64_t n_max, int32_t steps_max, double lon_min, double 
     g.nx = (int32_t)std::floor((lon_max - lon_min) * g.inv_h) + 3;
     g.ny = (int32_t)std::floor((lat_max - lat_min) * g.inv_h) + 3;
     const uint64_t cells = (uint64_t)g.nx * (uint64_t)g.ny;
-    uint64_t want = (uint64_t)n_max;
+    // about two buckets per particle: fewer cells alias into a stencil's buckets
+    uint64_t want = 2ull * (uint64_t)n_max;
     if (want < 4ull * (uint64_t)g.nx + 4) want = 4ull * (uint64_t)g.nx + 4;
     if (want < 1024) want = 1024;
     uint32_t T = next_pow2_u32(want);
@@ -417,6 +484,8 @@ int lm_query_pairs_batch(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float
     lm::Metric m;
     m.r = radius;
     m.p = p;
+    // above r by more than float32 rounding of a difference (2^-24 relative)
+    m.box = std::nextafter((float)(radius * (1.0 + 1e-6)), INFINITY);
     if (p == 2.0) {
         m.kind = lm::kL2;
         m.r2 = radius * radius;
