@@ -124,26 +124,62 @@ bin_keys_kernel(int64_t n, const float* __restrict__ x, const float* __restrict_
     vals[s * st.rec + i] = (uint32_t)i;
 }
 
-// Over the sorted (bucket, index): bucket starts and records.  Position e
-// starts every bucket in (key[e-1], key[e]]; the last position also ends the
-// buckets after its own, so all T + 1 entries are written.
+// Second level of the binning: one block per super-bucket (the particles
+// whose bucket has the same high bits, made contiguous by one radix pass),
+// counting its `fine` buckets in LDS.  It writes their starts (a contiguous
+// slice of the table; the last super-bucket also the end, start[T] = n) and
+// places the records {x, y, index} of its particles in bucket order (the
+// order inside a bucket is free: the pair list is sorted later).
+constexpr uint32_t kFineMax = 4096;
+
 __global__ void __launch_bounds__(kBlock)
 bin_table_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y, CellGrid g, Strides st,
-                 const uint32_t* __restrict__ skeys, const uint32_t* __restrict__ svals,
-                 uint32_t* __restrict__ start_all, float4* __restrict__ recs) {
-    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (e >= n) return;
+                 const uint32_t* __restrict__ keys_all, const uint32_t* __restrict__ vals_all,
+                 const uint32_t* __restrict__ dstart, int64_t drow, int64_t dstride, uint32_t fine,
+                 uint32_t* __restrict__ start_all, float4* __restrict__ recs_all) {
+    __shared__ uint32_t cnt[kFineMax];
+    __shared__ uint32_t scan_lds[kBlock / 64];
+    const uint32_t d = blockIdx.x, nsuper = gridDim.x, t = threadIdx.x;
     const int64_t s = blockIdx.y;
-    const uint32_t* key = skeys + s * st.rec;
+    const uint32_t* keys = keys_all + s * st.rec;
+    const uint32_t* vals = vals_all + s * st.rec;
     uint32_t* start = start_all + s * st.tab;
-    const uint32_t b = min(key[e], g.T - 1u);
-    const uint32_t from = e == 0 ? 0u : min(key[e - 1], g.T - 1u) + 1u;
-    for (uint32_t q = from; q <= b; ++q) start[q] = (uint32_t)e;
-    if (e + 1 == n)
-        for (uint32_t q = b + 1; q <= g.T; ++q) start[q] = (uint32_t)n;
-    const uint32_t i = svals[s * st.rec + e];
-    if (i < (uint64_t)n)
-        recs[s * st.rec + e] = make_float4(x[s * st.pos + i], y[s * st.pos + i], __uint_as_float(i), 0.0f);
+    const uint32_t nn = (uint32_t)n;
+    uint32_t a = 0, b = nn;
+    if (dstart != nullptr) {
+        a = min(dstart[s * drow + (int64_t)d * dstride], nn);
+        b = d + 1 < nsuper ? min(dstart[s * drow + (int64_t)(d + 1) * dstride], nn) : nn;
+        b = max(a, b);
+    }
+    const uint32_t fmask = fine - 1u;
+    for (uint32_t f = t; f < fine; f += kBlock) cnt[f] = 0;
+    __syncthreads();
+    for (uint32_t e = a + t; e < b; e += kBlock) atomicAdd(&cnt[keys[e] & fmask], 1u);
+    __syncthreads();
+    // exclusive offsets: thread t owns `per` consecutive buckets
+    const uint32_t per = (fine + kBlock - 1) / kBlock;
+    const uint32_t f0 = min(t * per, fine), f1 = min(f0 + per, fine);
+    uint32_t sum = 0;
+    for (uint32_t f = f0; f < f1; ++f) sum += cnt[f];
+    uint32_t all;
+    uint32_t run = block_exclusive_scan(sum, scan_lds, &all);
+    for (uint32_t f = f0; f < f1; ++f) {
+        const uint32_t c = cnt[f];
+        cnt[f] = run;
+        run += c;
+    }
+    __syncthreads();
+    const uint64_t base = (uint64_t)d * fine;
+    for (uint32_t f = t; f < fine; f += kBlock)
+        if (base + f < (uint64_t)g.T) start[base + f] = a + cnt[f];
+    if (d + 1 == nsuper && t == 0) start[g.T] = nn;
+    __syncthreads();
+    for (uint32_t e = a + t; e < b; e += kBlock) {
+        const uint32_t i = vals[e];
+        const uint32_t pos = a + atomicAdd(&cnt[keys[e] & fmask], 1u);
+        if (i < nn && pos < b)
+            recs_all[s * st.rec + pos] = make_float4(x[s * st.pos + i], y[s * st.pos + i], __uint_as_float(i), 0.0f);
+    }
 }
 
 // The (up to) six record ranges of the 3x3 stencil around (cx, cy): a row of
@@ -511,11 +547,23 @@ int lm_query_pairs_batch(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float
     LM_LAUNCH(lm::K_BIN_KEYS, lm::bin_keys_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, g, st, ws->keys[0],
               ws->vals[0]);
     LM_LAUNCH_CHECK();
+    // two-level binning: one radix pass on the high bits of the bucket, then
+    // a block per super-bucket sorts its fine buckets in LDS
+    const int fine_bits = ws->key_bits < 12 ? ws->key_bits : 12;
+    const int super_bits = ws->key_bits - fine_bits;
+    const uint32_t* dstart = nullptr;
+    int64_t dstride = 0;
+    int rc = LM_OK;
     int res = 0;
-    int rc = lm::radix_sort_u32(ws->keys, ws->vals, st.rec, nsteps, nullptr, n, n, ws->key_bits, &ws->sort, s, &res);
-    if (rc) return rc;
-    LM_LAUNCH(lm::K_BIN_TABLE, lm::bin_table_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, g, st,
-              (const uint32_t*)ws->keys[res], (const uint32_t*)ws->vals[res], ws->start, ws->recs);
+    if (super_bits > 0) {
+        rc = lm::radix_partition_u32(ws->keys, ws->vals, st.rec, nsteps, n, fine_bits, super_bits, &ws->sort, s,
+                                     &dstart, &dstride);
+        if (rc) return rc;
+        res = 1;
+    }
+    LM_LAUNCH(lm::K_BIN_TABLE, lm::bin_table_kernel, dim3(1u << super_bits, (unsigned)nsteps), dim3(lm::kBlock), 0,
+              s, n, lon, lat, g, st, (const uint32_t*)ws->keys[res], (const uint32_t*)ws->vals[res], dstart,
+              ws->sort.row, dstride, 1u << fine_bits, ws->start, ws->recs);
     LM_LAUNCH_CHECK();
     LM_LAUNCH(lm::K_PAIR_COUNT, lm::pair_count_kernel, grid, dim3(lm::kBlock), 0, s, n, g, m, st,
               (const uint32_t*)ws->start, (const float4*)ws->recs, ws->nbr_cnt, ws->nbr);

@@ -250,6 +250,38 @@ void sort_scratch_free(SortScratch* s) {
     s->bytes = 0;
 }
 
+// One stable pass over the digit (key >> shift) & dmask (at most
+// kMaxDigitBits wide) from keys/vals[cur] into keys/vals[1 - cur].
+static int radix_pass(uint32_t* keys[2], uint32_t* vals[2], int cur, int64_t stride, int32_t nseg,
+                      const int64_t* seg_len, int64_t uniform_len, int64_t max_len, int shift, int bits,
+                      SortScratch* sc, hipStream_t stream) {
+    const int64_t ntp = sort_padded_tiles(max_len);
+    const dim3 grid((unsigned)(ntp / kSortGroup), (unsigned)nseg);
+    const uint32_t dmask = (uint32_t)((1ull << bits) - 1ull);
+    const bool wide = bits > 8;
+    const int64_t n = (int64_t)(wide ? (1 << kMaxDigitBits) : 256) * ntp;
+    if (wide)
+        LM_LAUNCH(K_SORT_HIST, radix_hist_kernel<kMaxDigitBits>, grid, dim3(kSortThreads), 0, stream,
+                  (const uint32_t*)keys[cur], stride, seg_len, uniform_len, max_len, shift, dmask, sc->hist, sc->row,
+                  ntp);
+    else
+        LM_LAUNCH(K_SORT_HIST, radix_hist_kernel<8>, grid, dim3(kSortThreads), 0, stream, (const uint32_t*)keys[cur],
+                  stride, seg_len, uniform_len, max_len, shift, dmask, sc->hist, sc->row, ntp);
+    LM_LAUNCH_CHECK();
+    int rc = exclusive_scan_u32_batched(sc->hist, n, sc->row, sc->offs, sc->row, nseg, sc->bsums, nullptr, stream);
+    if (rc) return rc;
+    if (wide)
+        LM_LAUNCH(K_SORT_SCATTER, radix_scatter_kernel<kMaxDigitBits>, grid, dim3(kSortThreads), 0, stream,
+                  (const uint32_t*)keys[cur], (const uint32_t*)vals[cur], keys[1 - cur], vals[1 - cur], stride,
+                  seg_len, uniform_len, max_len, shift, dmask, (const uint32_t*)sc->offs, sc->row, ntp);
+    else
+        LM_LAUNCH(K_SORT_SCATTER, radix_scatter_kernel<8>, grid, dim3(kSortThreads), 0, stream,
+                  (const uint32_t*)keys[cur], (const uint32_t*)vals[cur], keys[1 - cur], vals[1 - cur], stride,
+                  seg_len, uniform_len, max_len, shift, dmask, (const uint32_t*)sc->offs, sc->row, ntp);
+    LM_LAUNCH_CHECK();
+    return LM_OK;
+}
+
 int radix_sort_u32(uint32_t* keys[2], uint32_t* vals[2], int64_t stride, int32_t nseg, const int64_t* seg_len,
                    int64_t uniform_len, int64_t max_len, int key_bits, SortScratch* sc, hipStream_t stream,
                    int* result) {
@@ -257,44 +289,35 @@ int radix_sort_u32(uint32_t* keys[2], uint32_t* vals[2], int64_t stride, int32_t
                "sort request exceeds its scratch");
     LM_REQUIRE(key_bits >= 0 && key_bits <= 32, LM_ERR_ARG, "key_bits out of range");
     int cur = 0;
-    const int64_t ntp = sort_padded_tiles(max_len);
-    if (ntp == 0 || key_bits == 0) {
+    if (sort_padded_tiles(max_len) == 0 || key_bits == 0) {
         *result = 0;
         return LM_OK;
     }
-    const dim3 grid((unsigned)(ntp / kSortGroup), (unsigned)nseg);
     // fewest passes of at most kMaxDigitBits, the narrow kernel when a pass fits 8 bits
     const int passes = (key_bits + kMaxDigitBits - 1) / kMaxDigitBits;
     const int per = (key_bits + passes - 1) / passes;
     for (int shift = 0; shift < key_bits; shift += per) {
         const int b = key_bits - shift < per ? key_bits - shift : per;
-        const uint32_t dmask = (uint32_t)((1ull << b) - 1ull);
-        const bool wide = b > 8;
-        const int64_t n = (int64_t)(wide ? (1 << kMaxDigitBits) : 256) * ntp;
-        if (wide)
-            LM_LAUNCH(K_SORT_HIST, radix_hist_kernel<kMaxDigitBits>, grid, dim3(kSortThreads), 0, stream,
-                      (const uint32_t*)keys[cur], stride, seg_len, uniform_len, max_len, shift, dmask, sc->hist,
-                      sc->row, ntp);
-        else
-            LM_LAUNCH(K_SORT_HIST, radix_hist_kernel<8>, grid, dim3(kSortThreads), 0, stream,
-                      (const uint32_t*)keys[cur], stride, seg_len, uniform_len, max_len, shift, dmask, sc->hist,
-                      sc->row, ntp);
-        LM_LAUNCH_CHECK();
-        int rc = exclusive_scan_u32_batched(sc->hist, n, sc->row, sc->offs, sc->row, nseg, sc->bsums, nullptr,
-                                            stream);
+        int rc = radix_pass(keys, vals, cur, stride, nseg, seg_len, uniform_len, max_len, shift, b, sc, stream);
         if (rc) return rc;
-        if (wide)
-            LM_LAUNCH(K_SORT_SCATTER, radix_scatter_kernel<kMaxDigitBits>, grid, dim3(kSortThreads), 0, stream,
-                      (const uint32_t*)keys[cur], (const uint32_t*)vals[cur], keys[1 - cur], vals[1 - cur], stride,
-                      seg_len, uniform_len, max_len, shift, dmask, (const uint32_t*)sc->offs, sc->row, ntp);
-        else
-            LM_LAUNCH(K_SORT_SCATTER, radix_scatter_kernel<8>, grid, dim3(kSortThreads), 0, stream,
-                      (const uint32_t*)keys[cur], (const uint32_t*)vals[cur], keys[1 - cur], vals[1 - cur], stride,
-                      seg_len, uniform_len, max_len, shift, dmask, (const uint32_t*)sc->offs, sc->row, ntp);
-        LM_LAUNCH_CHECK();
         cur = 1 - cur;
     }
     *result = cur;
+    return LM_OK;
+}
+
+int radix_partition_u32(uint32_t* keys[2], uint32_t* vals[2], int64_t stride, int32_t nseg, int64_t len,
+                        int shift, int bits, SortScratch* sc, hipStream_t stream, const uint32_t** digit_start,
+                        int64_t* digit_stride) {
+    LM_REQUIRE(nseg >= 1 && nseg <= sc->nseg_max && len <= sc->max_len && stride >= len, LM_ERR_ARG,
+               "partition request exceeds its scratch");
+    LM_REQUIRE(bits >= 1 && bits <= kMaxDigitBits && shift >= 0 && shift + bits <= 32, LM_ERR_ARG,
+               "partition digit out of range");
+    LM_REQUIRE(len >= 1, LM_ERR_ARG, "empty partition");
+    int rc = radix_pass(keys, vals, 0, stride, nseg, nullptr, len, len, shift, bits, sc, stream);
+    if (rc) return rc;
+    *digit_start = sc->offs;
+    *digit_stride = sort_padded_tiles(len);
     return LM_OK;
 }
 
