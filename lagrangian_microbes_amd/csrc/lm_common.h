@@ -66,7 +66,7 @@ enum KernelId {
     K_RPS_INIT,
     K_RPS_UNION,
     K_RPS_KEYS,
-    K_RPS_SEGMENTS,
+    K_RPS_GROUP,
     K_RPS_COMPS,
     K_RPS_PLAY,
     K_RPS_FILL,

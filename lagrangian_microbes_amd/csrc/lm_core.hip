@@ -24,7 +24,7 @@ void set_error(const char* fmt, ...) {
 // ---------------------------------------------------------------------------
 static const char* const kKernelNames[K_NUM_KERNELS] = {
     "advect_rk4_f64", "field_blend_f64", "bin_keys", "bin_table", "pair_count", "pair_emit",
-    "scan_reduce",    "scan_apply",      "rps_init",  "rps_union",   "rps_keys",   "rps_segments", "rps_comps",
+    "scan_reduce",    "scan_apply",      "rps_init",  "rps_union",   "rps_keys",   "rps_group", "rps_comps",
     "rps_play",       "rps_fill",  "rps_deps",    "rps_levels",    "philox",
     "copy16", "sort_hist", "sort_scatter"};
 

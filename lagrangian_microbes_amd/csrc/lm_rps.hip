@@ -15,14 +15,15 @@
 //                        comparisons with pRS, pPR, pSP as bits, marks both
 //                        particles as touched and records where each row (the
 //                        run of pairs with one i) begins and ends
-//           3. keys      each touched particle's root (untouched: n, after
-//                        every root)
-//           4. sort      stable radix sort of the particles by root: a
-//                        component's particles become one run in index order,
-//                        its root first
-//           5. segments  each particle's position in that order, the end of
-//                        every run, and the row lengths in that order
-//           6. scan      row offsets in that order.  Pairs are sorted by
+//           3. keys      each touched particle's root (an untouched particle:
+//                        its own index, a run of one that is never listed)
+//           4. group     particles grouped by root, stably, in two levels (a
+//                        radix pass on the root's high bits, then counting in
+//                        LDS per super-bucket): each component's particles one
+//                        run [pstart, pend) in index order, each particle's
+//                        position in that order, and the row lengths in that
+//                        order
+//           5. scan      row offsets in that order.  Pairs are sorted by
 //                        (i, j), so a component's pairs in sequential order are
 //                        the rows of its particles in index order: the scan
 //                        places every component's pairs in one run, in order
@@ -70,7 +71,8 @@ struct RpsStrides {
 // per-pair array at s * pair.  A component is described by a uint4
 // {first pair, pairs, first particle, particles} of its runs.
 struct RpsArrays {
-    uint32_t *parent, *loc, *pend;        // per particle
+    uint32_t *parent, *loc;               // per particle
+    uint32_t *pstart, *pend;              // per root: its run of particles [pstart, pend)
     uint32_t *rowbeg, *rowend;            // per particle: its row [rowbeg, rowend) of pairs
     uint32_t *rcnt, *roff;                // per sorted position: row length, run offset
     uint2* info;                          // per particle: {run position - pair rank, slot}
@@ -213,29 +215,125 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st) {
     const int64_t s = blockIdx.y, ro = s * st.row;
     const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (v < n) {
-        A.pkey[ro + v] = A.touched[ro + v] ? find_root(A.parent + ro, (uint32_t)v) : (uint32_t)n;
+        // an untouched particle is keyed by itself: a run of one, never listed
+        A.pkey[ro + v] = A.touched[ro + v] ? find_root(A.parent + ro, (uint32_t)v) : (uint32_t)v;
         A.pval[ro + v] = (uint32_t)v;
     }
 }
 
-// Over the sorted particles: position of each touched particle, end of each
-// run, row lengths in sorted order.
-__global__ void __launch_bounds__(kBlock)
-rps_segments_kernel(int64_t n, RpsArrays A, RpsStrides st) {
+// Second level of the grouping by root: one wavefront per super-bucket (the
+// particles whose root has the same high bits, made contiguous, in index
+// order, by one stable radix pass) counts its roots' particles in LDS and
+// writes each root's run [pstart, pend), every touched particle's position in
+// the run order (loc) and the row lengths in that order.  The placement is
+// stable -- a run keeps index order, which the pair order relies on: 64
+// particles at a time in order, a lane's rank among equal roots from one
+// ballot per root bit, a running count per root in LDS.  Loads go kGroupRounds
+// rounds at a time.
+constexpr uint32_t kGroupFineBits = 9;
+constexpr uint32_t kGroupFine = 1u << kGroupFineBits;
+constexpr uint32_t kGroupRounds = 8;
+
+__global__ void __launch_bounds__(64)
+rps_group_kernel(int64_t n, RpsArrays A, RpsStrides st, const uint32_t* __restrict__ kin,
+                 const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                 const uint32_t* __restrict__ dstart, int64_t drow, int64_t dstride, uint32_t fine) {
+    __shared__ uint32_t cnt[kGroupFine];
+    const uint32_t d = blockIdx.x, nsuper = gridDim.x, lane = threadIdx.x;
     const int64_t s = blockIdx.y, ro = s * st.row;
-    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (e >= n) return;
-    const uint32_t* spk = A.spk + ro;
-    const uint32_t L = spk[e];
-    const uint32_t x = A.ids[ro + e];
-    uint32_t len = 0;
-    if (L < (uint64_t)n && x < (uint64_t)n) {
-        A.loc[ro + x] = (uint32_t)e;
-        const uint32_t rb = A.rowbeg[ro + x], re = A.rowend[ro + x];
-        len = re > rb ? re - rb : 0u;
-        if (e + 1 == n || spk[e + 1] != L) A.pend[ro + L] = (uint32_t)(e + 1);
+    const uint32_t nn = (uint32_t)n;
+    kin += ro;
+    vin += ro;
+    kout += ro;
+    vout += ro;
+    uint32_t a = 0, b = nn;
+    if (dstart != nullptr) {
+        a = min(dstart[s * drow + (int64_t)d * dstride], nn);
+        b = d + 1 < nsuper ? min(dstart[s * drow + (int64_t)(d + 1) * dstride], nn) : nn;
+        b = max(a, b);
     }
-    A.rcnt[ro + e] = len;
+    const uint32_t fmask = fine - 1u;
+    for (uint32_t f = lane; f < fine; f += 64) cnt[f] = 0;
+    __syncthreads();
+    for (uint32_t c0 = a; c0 < b; c0 += 64 * kGroupRounds) {
+        uint32_t k[kGroupRounds];
+#pragma unroll
+        for (uint32_t r = 0; r < kGroupRounds; ++r) {
+            const uint32_t e = c0 + 64 * r + lane;
+            k[r] = e < b ? kin[e] : 0u;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kGroupRounds; ++r)
+            if (c0 + 64 * r + lane < b) atomicAdd(&cnt[k[r] & fmask], 1u);
+    }
+    __syncthreads();
+    // exclusive offsets, 64 roots at a time; runs and their ends
+    const uint64_t base = (uint64_t)d * fine;
+    uint32_t carry = 0;
+    for (uint32_t f0 = 0; f0 < fine; f0 += 64) {
+        const uint32_t f = f0 + lane;
+        const uint32_t c = f < fine ? cnt[f] : 0u;
+        const uint32_t inc = wave_inclusive_scan(c);
+        const uint32_t off = carry + inc - c;
+        if (f < fine) {
+            cnt[f] = off;
+            if (base + f < (uint64_t)nn) {
+                A.pstart[ro + base + f] = a + off;
+                A.pend[ro + base + f] = a + off + c;
+            }
+        }
+        carry += __shfl(inc, 63, 64);
+    }
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (uint32_t c0 = a; c0 < b; c0 += 64 * kGroupRounds) {
+        uint32_t k[kGroupRounds], v[kGroupRounds];
+#pragma unroll
+        for (uint32_t r = 0; r < kGroupRounds; ++r) {
+            const uint32_t e = c0 + 64 * r + lane;
+            k[r] = e < b ? kin[e] : 0u;
+            v[r] = e < b ? vin[e] : kUnset;
+        }
+        // everything the placement writes, loaded before the ranking rounds
+        uint8_t tch[kGroupRounds];
+        uint32_t rb[kGroupRounds], re[kGroupRounds];
+#pragma unroll
+        for (uint32_t r = 0; r < kGroupRounds; ++r) {
+            const bool ok = v[r] < nn;
+            tch[r] = ok ? A.touched[ro + v[r]] : (uint8_t)0;
+            rb[r] = ok ? A.rowbeg[ro + v[r]] : 0u;
+            re[r] = ok ? A.rowend[ro + v[r]] : 0u;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kGroupRounds; ++r) {
+            const bool valid = c0 + 64 * r + lane < b;
+            if (__ballot(valid) == 0ull) break;  // uniform
+            const uint32_t f = k[r] & fmask;
+            unsigned long long m = __ballot(valid);
+#pragma unroll
+            for (uint32_t bit = 0; bit < kGroupFineBits; ++bit) {
+                const bool on = (f >> bit) & 1u;
+                const unsigned long long bm = __ballot(valid && on);
+                m &= on ? bm : ~bm;
+            }
+            const uint32_t rank = (uint32_t)__popcll(m & below);
+            const uint32_t start = valid ? cnt[f] : 0u;
+            if (valid && rank == 0) cnt[f] = start + (uint32_t)__popcll(m);
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t pos = a + start + rank;
+            if (valid && pos < b) {
+                const uint32_t x = v[r];
+                kout[pos] = k[r];
+                vout[pos] = x;
+                uint32_t len = 0;
+                if (tch[r] && k[r] < nn && x < nn) {
+                    A.loc[ro + x] = pos;
+                    len = re[r] > rb[r] ? re[r] - rb[r] : 0u;
+                }
+                A.rcnt[ro + pos] = len;
+            }
+        }
+    }
 }
 
 // Over the sorted particles: slot and pair shift per particle; every run start
@@ -262,10 +360,10 @@ rps_comps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
         if (e >= n) continue;
         const uint32_t L = spk[e];
         const uint32_t x = A.ids[ro + e];
-        if (L >= (uint64_t)n || x >= (uint64_t)n) continue;
-        const uint32_t pb = A.loc[ro + L];  // the root is the first particle of its run
+        if (L >= (uint64_t)n || x >= (uint64_t)n || A.loc[ro + x] == kUnset) continue;  // untouched
+        const uint32_t pb = A.pstart[ro + L];
         A.info[ro + x] = make_uint2(roff[e] - A.rowbeg[ro + x], (uint32_t)e - pb);
-        if (x == L) {
+        if ((uint32_t)e == pb) {
             const uint32_t pe = A.pend[ro + L];
             const uint32_t qb = min(roff[e], P), qe = min(roff[pe], P);
             d[r] = make_uint4(qb, qe > qb ? qe - qb : 0u, (uint32_t)e, pe - (uint32_t)e);
@@ -766,7 +864,7 @@ static void rps_free_pairs(lm_rps_ws* w) {
 static void rps_ws_free(lm_rps_ws* w) {
     if (!w) return;
     lm::RpsArrays& A = w->A;
-    for (uint32_t* p : {A.parent, A.loc, A.pend, A.rowbeg, A.rowend, A.rcnt, A.roff, A.pkey, A.pval, A.ptk, A.ptv,
+    for (uint32_t* p : {A.parent, A.loc, A.pstart, A.pend, A.rowbeg, A.rowend, A.rcnt, A.roff, A.pkey, A.pval, A.ptk, A.ptv,
                         A.lists, w->scan_tmp})
         (void)hipFree(p);
     (void)hipFree(A.info);
@@ -824,7 +922,8 @@ int lm_rps_ws_create(int64_t n_max, int32_t steps_max, lm_rps_ws** out) {
     lm::RpsArrays& A = w->A;
     int rc;
     if ((rc = rws_alloc(&A.parent, R, &w->bytes)) || (rc = rws_alloc(&A.loc, R, &w->bytes)) ||
-        (rc = rws_alloc(&A.pend, R, &w->bytes)) || (rc = rws_alloc(&A.rowbeg, R, &w->bytes)) ||
+        (rc = rws_alloc(&A.pend, R, &w->bytes)) || (rc = rws_alloc(&A.pstart, R, &w->bytes)) ||
+        (rc = rws_alloc(&A.rowbeg, R, &w->bytes)) ||
         (rc = rws_alloc(&A.rowend, R, &w->bytes)) || (rc = rws_alloc(&A.rcnt, R, &w->bytes)) ||
         (rc = rws_alloc(&A.roff, R, &w->bytes)) || (rc = rws_alloc(&A.info, R, &w->bytes)) ||
         (rc = rws_alloc(&A.touched, R, &w->bytes)) || (rc = rws_alloc(&A.pkey, R, &w->bytes)) ||
@@ -886,15 +985,28 @@ int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair
         LM_LAUNCH_CHECK();
         LM_LAUNCH(lm::K_RPS_KEYS, lm::rps_keys_kernel, dim3(nbp, S), dim3(lm::kBlock), 0, s, n, A, sp);
         LM_LAUNCH_CHECK();
-        int res = 0;
+        // grouping by root in two levels: a radix pass on the root's high
+        // bits, then a block per super-bucket counts its roots in LDS
+        const int kb = key_bits_for(n);
+        const int fine_bits = kb < (int)lm::kGroupFineBits ? kb : (int)lm::kGroupFineBits;
+        const int super_bits = kb - fine_bits;
         uint32_t* pk[2] = {A.pkey, A.ptk};
         uint32_t* pv[2] = {A.pval, A.ptv};
-        if ((rc = lm::radix_sort_u32(pk, pv, ws->row, nsteps, nullptr, n, n, key_bits_for(n), &ws->sort, s, &res)))
-            return rc;
-        A.spk = pk[res];
-        A.ids = pv[res];
-        LM_LAUNCH(lm::K_RPS_SEGMENTS, lm::rps_segments_kernel, dim3(nbp, S), dim3(lm::kBlock), 0, s, n, A, sp);
+        const uint32_t* dstart = nullptr;
+        int64_t dstride = 0;
+        int in = 0;
+        if (super_bits > 0) {
+            if ((rc = lm::radix_partition_u32(pk, pv, ws->row, nsteps, n, fine_bits, super_bits, &ws->sort, s, &dstart,
+                                              &dstride)))
+                return rc;
+            in = 1;
+        }
+        LM_LAUNCH(lm::K_RPS_GROUP, lm::rps_group_kernel, dim3(1u << super_bits, S), dim3(64), 0, s, n, A, sp,
+                  (const uint32_t*)pk[in], (const uint32_t*)pv[in], pk[1 - in], pv[1 - in], dstart, ws->sort.row,
+                  dstride, 1u << fine_bits);
         LM_LAUNCH_CHECK();
+        A.spk = pk[1 - in];
+        A.ids = pv[1 - in];
         if ((rc = lm::exclusive_scan_u32_batched(A.rcnt, n, ws->row, A.roff, ws->row, nsteps, ws->scan_tmp, nullptr,
                                                  s)))
             return rc;
