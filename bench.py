@@ -79,31 +79,35 @@ def algorithmic_bytes(kernel, m):
     tables, grouping, scans) have none: their HBM traffic is overhead, reported
     from the byte counters instead.
 
-    Per step: n particles, P pairs, Pw / Uw the pairs / particles of the
-    components replayed one wavefront each (more than 32 pairs).  Batched
-    kernels cover `chunk` steps per launch; rps_play* run once per step.
+    Per step: n particles, P pairs.  Batched kernels cover `chunk` steps per
+    launch; rps_play runs once per step.
     """
-    n, P, Pw, Uw, chunk, F = (m[k] for k in ("n", "P", "Pw", "Uw", "chunk", "F"))
+    n, P, chunk, F = (m[k] for k in ("n", "P", "chunk", "F"))
     per_launch = {
         "advect_rk4_f64": 32 * n + (8 * n + F) * chunk,
         "pair_count": 16 * n * chunk,
         "pair_emit": 8 * P * chunk,
-        "rps_play": 8 * (P - Pw) + 2 * (n - Uw),
-        "rps_play_wide": 8 * Pw + 2 * Uw,
+        "rps_play": 8 * P + 2 * n,
     }
     return per_launch.get(kernel)
 
 
 def step_statistics(runner, steps, total, chunk):
-    """Mean per-step pair count and component sizes over `steps` steps."""
+    """Mean per-step pair count and component sizes over `steps` steps, from a
+    pass run like the timed one (chunks pipelined)."""
     runner.reset()
     rows = []
-    while runner.step < min(steps, total):
-        k = runner.advance(min(chunk, total - runner.step))
-        cnt = runner.npairs[:k].cpu().numpy()
+    chunks = [0]
+
+    def collect(k, rx, ry, rs, cnt):
+        counts = cnt.cpu().numpy()
+        rps = runner.engine.rps_sets[chunks[0] % 2]
         for s in range(k):
-            st = runner.engine.rps.component_stats(s)
-            rows.append((cnt[s], st.touched, st.wide_pairs, st.wide_particles))
+            st = rps.component_stats(s)
+            rows.append((counts[s], st.touched, st.wide_pairs, st.wide_particles))
+        chunks[0] += 1
+
+    runner.run(min(steps, total), on_chunk=collect)
     P, U, Pw, Uw = np.array(rows, dtype=np.float64).mean(axis=0)
     return {"P": P, "U": U, "Pw": Pw, "Uw": Uw}
 

@@ -58,7 +58,6 @@ constexpr uint32_t kNarrowSlots = kNarrowPairs + 1;  // a component of P pairs h
 constexpr uint32_t kLevelPairs = 4096;               // components of up to this many pairs replay by levels
 constexpr uint32_t kLevelSlots = kLevelPairs + 1;    // ... and so have at most this many particles
 constexpr uint32_t kSlotMask = (1u << 29) - 1u;
-constexpr unsigned kPairGrid = 256;                  // blocks per step for pair-parallel passes
 constexpr int kCompItems = 8;                        // sorted positions per thread when listing components
 
 // Per-step strides (elements) of a batch.
@@ -980,7 +979,9 @@ int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair
     LM_LAUNCH(lm::K_RPS_INIT, lm::rps_init_kernel, dim3(nbp, S), dim3(lm::kBlock), 0, s, n, A, sp);
     LM_LAUNCH_CHECK();
     if (pair_stride > 0) {
-        LM_LAUNCH(lm::K_RPS_UNION, lm::rps_union_kernel, dim3(lm::kPairGrid, S), dim3(lm::kBlock), 0, s, n,
+        // one thread per pair slot of the capacity (threads past a step's count exit)
+        const unsigned npb = lm::grid_for(pair_stride);
+        LM_LAUNCH(lm::K_RPS_UNION, lm::rps_union_kernel, dim3(npb, S), dim3(lm::kBlock), 0, s, n,
                   pair_i, pair_j, npairs, A, sp, pRS, pPR, pSP, key0, key1, step0);
         LM_LAUNCH_CHECK();
         LM_LAUNCH(lm::K_RPS_KEYS, lm::rps_keys_kernel, dim3(nbp, S), dim3(lm::kBlock), 0, s, n, A, sp);
@@ -1013,7 +1014,7 @@ int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair
         LM_LAUNCH(lm::K_RPS_COMPS, lm::rps_comps_kernel, dim3((nbp + lm::kCompItems - 1) / lm::kCompItems, S),
                   dim3(lm::kBlock), 0, s, n, A, sp);
         LM_LAUNCH_CHECK();
-        LM_LAUNCH(lm::K_RPS_FILL, lm::rps_fill_kernel, dim3(lm::kPairGrid, S), dim3(lm::kBlock), 0, s, n, pair_i,
+        LM_LAUNCH(lm::K_RPS_FILL, lm::rps_fill_kernel, dim3(npb, S), dim3(lm::kBlock), 0, s, n, pair_i,
                   pair_j, A, sp);
         LM_LAUNCH_CHECK();
         LM_LAUNCH(lm::K_RPS_DEPS, lm::rps_deps_kernel, dim3(1024, S), dim3(64), 0, s, n, A, sp);

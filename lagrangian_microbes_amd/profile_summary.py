@@ -29,10 +29,10 @@ TIMER_NAMES = {
     "bin_keys_kernel": "bin_keys", "bin_table_kernel": "bin_table", "pair_count_kernel": "pair_count",
     "pair_emit_kernel": "pair_emit", "radix_hist_kernel": "sort_hist", "radix_scatter_kernel": "sort_scatter",
     "scan_reduce_kernel": "scan_reduce", "scan_apply_kernel": "scan_apply", "rps_init_kernel": "rps_init",
-    "rps_union_kernel": "rps_union", "rps_keys_kernel": "rps_keys", "rps_segments_kernel": "rps_segments",
+    "rps_union_kernel": "rps_union", "rps_keys_kernel": "rps_keys", "rps_group_kernel": "rps_group",
     "rps_comps_kernel": "rps_comps", "rps_fill_kernel": "rps_fill", "rps_deps_kernel": "rps_deps",
-    "rps_levels_kernel": "rps_levels", "rps_play_narrow_kernel": "rps_play",
-    "rps_play_wide_kernel": "rps_play_wide", "copy16_kernel": "copy16", "philox_kernel": "philox",
+    "rps_levels_kernel": "rps_levels", "rps_play_kernel": "rps_play", "copy16_kernel": "copy16",
+    "philox_kernel": "philox", "seg_copy_kernel": "copy16",
 }
 
 
@@ -52,12 +52,14 @@ def one_file(d, pattern):
 
 
 def trace(args):
-    rows = []
+    acc = {}
     for r in csv.DictReader(open(one_file(args.dir, "*_kernel_stats.csv"))):
         k = short_name(r["Name"])
         if k is None:
             continue
-        rows.append((float(r["TotalDurationNs"]), k, int(r["Calls"]), float(r["AverageNs"])))
+        tot, calls = acc.get(k, (0.0, 0))
+        acc[k] = (tot + float(r["TotalDurationNs"]), calls + int(r["Calls"]))
+    rows = [(tot, k, calls, tot / calls) for k, (tot, calls) in acc.items()]
     total = sum(r[0] for r in rows)
     out = [f"# {args.title}", ""]
     if args.command:
