@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define LM_ABI_VERSION 3
+#define LM_ABI_VERSION 4
 
 #define LM_OK 0
 #define LM_ERR_ARG (-1)      /* invalid argument                             */
@@ -194,6 +194,47 @@ int lm_rps_component_stats(lm_rps_ws* ws, int32_t step, int64_t* out8, void* str
  * reference draws from numpy's global generator (interactions.py:20). */
 int lm_philox_uniforms(int64_t count, uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3,
                        uint64_t key0, uint64_t key1, int32_t word, double* out, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint of the RK4 advection with respect to a float32 gridded field and */
+/* the initial positions.  No reference counterpart: the reference only     */
+/* advects forward (AdvectionRK4 through parcels, particle_advecter.py:     */
+/* 177-187,222-223); the forward here is that advection, and the backward   */
+/* is its hand-written reverse, for callers that fit u, v to trajectories.  */
+/* ------------------------------------------------------------------------ */
+
+/* The field is float32 [nlev][ny][nx][2] (u, v interleaved, m/s).  Stage
+ * time k (k = 0 .. 2*nsteps, at t0 + k*dt/2) reads levels lev0[k] and
+ * lev0[k] + 1 (level 0 only when nlev == 1) blended with weight w[k]:
+ * fma(w, f1 - f0, f0) on the values widened to double, so the positions are
+ * bit-identical to lm_advect_rk4_f64 over the widened levels blended by
+ * lm_field_blend_f64.  lev0 and w are device arrays of 2*nsteps + 1 entries;
+ * lev0_host is a host copy of lev0, checked against nlev before any launch.
+ * A step with a stage outside the grid keeps its position and adds one to
+ * *oob_count. */
+
+/* Advances lon/lat (in place) by nsteps RK4 steps of dt seconds and stores
+ * the positions at the start of steps 0, ckpt, 2*ckpt, ... as interleaved
+ * (lon, lat) doubles: checkpoints[(s/ckpt)*n + i][2], ceil(nsteps/ckpt)*n*2
+ * doubles, 16-byte aligned. */
+int lm_adjoint_forward(int64_t n, double* lon, double* lat, const float* field, int32_t nlev, const lm_grid* grid,
+                       const int32_t* lev0, const double* w, const int32_t* lev0_host, int32_t nsteps, double dt,
+                       int32_t ckpt, double* checkpoints, uint32_t* oob_count, void* stream);
+
+/* Bytes of device scratch lm_adjoint_backward needs for n particles and
+ * checkpoint interval ckpt (-1 on invalid arguments). */
+int64_t lm_adjoint_scratch_bytes(int64_t n, int32_t ckpt);
+
+/* Reverse pass over the run that lm_adjoint_forward made with the same
+ * arguments.  On entry glon/glat hold dL/d(final lon, lat); on return
+ * dL/d(initial lon, lat).  dL/d(field) is ADDED into gfield (float32, the
+ * field's layout; the caller zeroes it).  Each segment of ckpt steps is
+ * recomputed from its checkpoint into scratch (16-byte aligned, at least
+ * lm_adjoint_scratch_bytes(n, ckpt) bytes), then swept in reverse. */
+int lm_adjoint_backward(int64_t n, const double* checkpoints, const float* field, int32_t nlev, const lm_grid* grid,
+                        const int32_t* lev0, const double* w, const int32_t* lev0_host, int32_t nsteps, double dt,
+                        int32_t ckpt, void* scratch, int64_t scratch_bytes, double* glon, double* glat, float* gfield,
+                        void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Plain streaming copy, used to calibrate hardware byte counters.          */

@@ -48,6 +48,11 @@ _SIGNATURES = {
     "lm_rps_play": (ctypes.c_int, [c_vp, c_i32, c_i64, c_vp, c_vp, c_vp]),
     "lm_rps_component_stats": (ctypes.c_int, [c_vp, c_i32, c_vp, c_vp]),
     "lm_philox_uniforms": (ctypes.c_int, [c_i64, c_u64, c_u64, c_u64, c_u64, c_u64, c_u64, c_i32, c_vp, c_vp]),
+    "lm_adjoint_forward": (ctypes.c_int, [c_i64, c_vp, c_vp, c_vp, c_i32, ctypes.POINTER(LmGrid), c_vp, c_vp, c_vp,
+                                          c_i32, c_dbl, c_i32, c_vp, c_vp, c_vp]),
+    "lm_adjoint_scratch_bytes": (c_i64, [c_i64, c_i32]),
+    "lm_adjoint_backward": (ctypes.c_int, [c_i64, c_vp, c_vp, c_i32, ctypes.POINTER(LmGrid), c_vp, c_vp, c_vp,
+                                           c_i32, c_dbl, c_i32, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "lm_copy_bytes": (ctypes.c_int, [c_vp, c_vp, c_i64, c_vp]),
     "lm_sort_pairs_u32": (ctypes.c_int, [c_i32, c_vp, c_i64, c_i64, c_i64, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "lm_timing_enable": (ctypes.c_int, [ctypes.c_int]),
@@ -73,7 +78,7 @@ def load(path=LIB_PATH):
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        if L.lm_abi_version() != 3:
+        if L.lm_abi_version() != 4:
             raise RuntimeError("liblm.so ABI version mismatch; rebuild it")
         _lib = L
     return _lib

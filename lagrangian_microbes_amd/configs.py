@@ -70,6 +70,13 @@ class Workload:
                                        self.lat_range[1] + m, self.field_spacing, self.field_modes, self.field_rms,
                                        self.field_seed, self.field_levels, self.field_level_interval)
 
+    def target_velocity_field(self):
+        """The second seeded field whose trajectories are an adjoint run's targets."""
+        m = self.field_margin
+        return synthetic_fourier_field(self.lon_range[0] - m, self.lon_range[1] + m, self.lat_range[0] - m,
+                                       self.lat_range[1] + m, self.field_spacing, self.field_modes, self.field_rms,
+                                       self.extra["target_seed"], self.field_levels, self.field_level_interval)
+
     def describe(self):
         return {"workload": self.name, "particles": self.n, "steps": self.steps, "dt_s": self.dt,
                 "radius_deg": self.radius, "p": self.p, "precision": self.precision,
@@ -100,10 +107,21 @@ def rps_4m_60day():
 
 
 def adjoint_1m_10day():
-    """1,000,000 particles on 1000x1000 over 10x10 degrees, 20 field levels 13 h apart, 240 h."""
+    """1,000,000 particles on 1000x1000 over 10x10 degrees, 20 float32 field levels 13 h apart, 240 h;
+    targets from a second field seeded 105."""
     return Workload(name="adjoint_1m_10day", n=1_000_000, lat_range=(25.0, 35.0), lon_range=(205.0, 215.0),
                     field_spacing=0.25, field_margin=5.0, field_levels=20, field_level_interval=13 * HOUR,
-                    field_modes=16, field_rms=0.25, field_seed=5, dt=HOUR, steps=240, p=0.5, species_seed=5)
+                    field_modes=16, field_rms=0.25, field_seed=5, dt=HOUR, steps=240, p=0.5, species_seed=5,
+                    precision="f32 field, f64 positions", extra={"target_seed": 105})
 
 
-WORKLOADS = {w().name: w for w in (rps_10k_steady, rps_490k_30day, rps_4m_60day, adjoint_1m_10day)}
+def adjoint_10k_24h():
+    """The same field and patch with 10,000 particles (100x100) and 24 steps: the size of the
+    autograd cross-check."""
+    w = adjoint_1m_10day()
+    w.name, w.n, w.steps = "adjoint_10k_24h", 10_000, 24
+    return w
+
+
+WORKLOADS = {w().name: w for w in (rps_10k_steady, rps_490k_30day, rps_4m_60day, adjoint_1m_10day,
+                                   adjoint_10k_24h)}

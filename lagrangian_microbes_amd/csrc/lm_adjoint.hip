@@ -1,0 +1,599 @@
+// Adjoint of the RK4 advection through a float32 gridded velocity field:
+// gradients of a loss on the final positions with respect to the initial
+// positions and to every u, v value of every time level.
+//
+// Field: float32 levels [nlev][ny][nx] of interleaved (u, v); a stage at
+// schedule entry s reads level lev0[s] and lev0[s] + 1 (the same level for a
+// steady field, nlev == 1) and blends each corner node as
+// fma(w[s], (double)f1 - (double)f0, (double)f0): the FP64 path's operations
+// on the field widened to double, so forward positions are bit-identical to
+// the FP64 advection of the widened field.
+//
+// Forward (lm_adjoint_forward): RK4 steps as the FP64 path, keeping the
+// positions at the start of every segment of `ckpt` steps.
+//
+// Backward (lm_adjoint_backward), segment by segment from the last:
+//   recompute  from the segment's checkpoint, the start position of every step
+//              into scratch, and per block the bounding box of the cells its
+//              stage samples touch
+//   backward   steps in reverse.  Each step first re-samples its four stages
+//              from the stored start position (same operations as the forward,
+//              so the same trajectory) keeping each stage's velocity Jacobian
+//              and interpolation weights in registers, then sweeps back with
+//              lambda = dL/dx at the step's end:
+//                a4 = h/6 l, a3 = a2 = h/3 l, a1 = h/6 l, dL/dx0 = l
+//                stage 4 (x4 = x0 + h k3):   g = J4^T a4; dx0 += g; a3 += h g
+//                stage 3 (x3 = x0 + h/2 k2): g = J3^T a3; dx0 += g; a2 += h/2 g
+//                stage 2 (x2 = x0 + h/2 k1): g = J2^T a2; dx0 += g; a1 += h/2 g
+//                stage 1 (x0):               g = J1^T a1; dx0 += g
+//              and adds a^T dk/df to the field gradient (bilinear corner
+//              weights times 1 - w, w for the two levels).  A step with a stage
+//              outside the grid left the position unchanged: identity.
+// Velocity in degrees per second: ud = u K / cos(lat), vd = v K, K = 1/111120,
+// so dud/dlat = K (du/dlat / c + u sin(lat) (pi/180) / c^2).
+//
+// Field-gradient accumulation: per lane in registers while successive stage
+// samples stay in one cell and level (a particle crosses a 0.25 degree cell
+// in several steps), then per block in an LDS window (the segment's levels
+// over the block's bounding box), then one global atomic per touched window
+// entry.  A block whose window does not fit the LDS it was launched with adds
+// straight to global memory.
+#include "lm_common.h"
+
+namespace lm {
+
+constexpr int kAdjBlock = 256;
+
+// sin(lat * pi / 180) for |lat| <= 90, used only in derivatives: Taylor
+// series through x^17 (truncation below 5e-14 at x = pi/2), Horner in x^2.
+__device__ __forceinline__ double sin_deg(double deg) {
+    const double x = deg * LM_DEG2RAD;
+    const double z = x * x;
+    // s_k = (-1)^k / (2k+1)!, k = 8 .. 0, as exactly rounded hex literals.
+    double p = 0x1.952c77030ad4ap-49;
+    p = fma(p, z, -0x1.ae7f3e733b81fp-41);
+    p = fma(p, z, 0x1.6124613a86d09p-33);
+    p = fma(p, z, -0x1.ae64567f544e4p-26);
+    p = fma(p, z, 0x1.71de3a556c734p-19);
+    p = fma(p, z, -0x1.a01a01a01a01ap-13);
+    p = fma(p, z, 0x1.1111111111111p-7);
+    p = fma(p, z, -0x1.5555555555555p-3);
+    p = fma(p, z, 1.0);
+    return p * x;
+}
+
+struct AdjField {
+    const float2* f;
+    const int32_t* lev0;
+    const double* w;
+    int64_t level_len;  // nodes per level
+    int64_t lstride;    // level_len, or 0 for a steady field
+    lm_grid g;
+};
+
+// Schedule entry s: first level and weight.
+struct AdjStage {
+    int l;
+    double w;
+};
+
+__device__ __forceinline__ AdjStage adj_stage(const AdjField& F, int s) {
+    return AdjStage{F.lev0[s], F.w[s]};
+}
+
+// Cell and fractional position as the FP64 path computes them; false outside.
+__device__ __forceinline__ bool adj_locate(const lm_grid& g, double x, double y, int* i, int* j, double* xi,
+                                           double* eta) {
+    const double fx = (x - g.x0) * g.inv_dx;
+    const double fy = (y - g.y0) * g.inv_dy;
+    if (!(fx >= 0.0 && fx <= (double)(g.nx - 1) && fy >= 0.0 && fy <= (double)(g.ny - 1))) return false;
+    int ii = (int)fx, jj = (int)fy;
+    ii = ii > g.nx - 2 ? g.nx - 2 : ii;
+    jj = jj > g.ny - 2 ? g.ny - 2 : jj;
+    *i = ii;
+    *j = jj;
+    *xi = fx - (double)ii;
+    *eta = fy - (double)jj;
+    return true;
+}
+
+// Corners (j,i), (j,i+1), (j+1,i), (j+1,i+1), blended in time.
+__device__ __forceinline__ void adj_corners(const AdjField& F, const AdjStage& st, int i, int j, double2 q[4]) {
+    const int64_t o = (int64_t)st.l * F.level_len + (int64_t)j * F.g.nx + i;
+    const int64_t off[4] = {o, o + 1, o + F.g.nx, o + F.g.nx + 1};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float2 a = F.f[off[c]], b = F.f[off[c] + F.lstride];
+        q[c] = make_double2(fma(st.w, (double)b.x - (double)a.x, (double)a.x),
+                            fma(st.w, (double)b.y - (double)a.y, (double)a.y));
+    }
+}
+
+__device__ __forceinline__ bool adj_sample(const AdjField& F, const AdjStage& st, double x, double y, double* ud,
+                                           double* vd, int* ci, int* cj) {
+    int i, j;
+    double xi, eta;
+    if (!adj_locate(F.g, x, y, &i, &j, &xi, &eta)) return false;
+    double2 q[4];
+    adj_corners(F, st, i, j, q);
+    const double ub = fma(xi, q[1].x - q[0].x, q[0].x);
+    const double ut = fma(xi, q[3].x - q[2].x, q[2].x);
+    const double u = fma(eta, ut - ub, ub);
+    const double vb = fma(xi, q[1].y - q[0].y, q[0].y);
+    const double vt = fma(xi, q[3].y - q[2].y, q[2].y);
+    const double v = fma(eta, vt - vb, vb);
+    *ud = (u * LM_INV_M_PER_DEG) / cos_deg(y);
+    *vd = v * LM_INV_M_PER_DEG;
+    *ci = i;
+    *cj = j;
+    return true;
+}
+
+// Cell bounding box of one lane's stage samples.
+struct CellBox {
+    int jmin, jmax, imin, imax;
+    __device__ void reset() {
+        jmin = imin = 0x7fffffff;
+        jmax = imax = -1;
+    }
+    __device__ void add(int j, int i) {
+        jmin = min(jmin, j);
+        jmax = max(jmax, j);
+        imin = min(imin, i);
+        imax = max(imax, i);
+    }
+};
+
+// One RK4 step in the FP64 path's operation order; false (position kept) if
+// a stage leaves the grid.  With box != NULL the stage cells of a completed
+// step are added to it.
+__device__ __forceinline__ bool adj_rk4(const AdjField& F, int s, double hh, double h, double h6, double* x,
+                                       double* y, CellBox* box) {
+    const AdjStage a = adj_stage(F, 2 * s), b = adj_stage(F, 2 * s + 1), c = adj_stage(F, 2 * s + 2);
+    double u1, v1, u2, v2, u3, v3, u4, v4;
+    int i1, j1, i2, j2, i3, j3, i4, j4;
+    const double x0 = *x, y0 = *y;
+    if (!adj_sample(F, a, x0, y0, &u1, &v1, &i1, &j1)) return false;
+    if (!adj_sample(F, b, fma(u1, hh, x0), fma(v1, hh, y0), &u2, &v2, &i2, &j2)) return false;
+    if (!adj_sample(F, b, fma(u2, hh, x0), fma(v2, hh, y0), &u3, &v3, &i3, &j3)) return false;
+    if (!adj_sample(F, c, fma(u3, h, x0), fma(v3, h, y0), &u4, &v4, &i4, &j4)) return false;
+    const double su = fma(2.0, u2 + u3, u1 + u4);
+    const double sv = fma(2.0, v2 + v3, v1 + v4);
+    *x = fma(su, h6, x0);
+    *y = fma(sv, h6, y0);
+    if (box) {
+        box->add(j1, i1);
+        box->add(j2, i2);
+        box->add(j3, i3);
+        box->add(j4, i4);
+    }
+    return true;
+}
+
+// A stage linearised for the reverse sweep: the velocity Jacobian (FP64, it
+// carries lambda) and, in float32 like the field gradient it feeds, the
+// interpolation weights and K / cos(lat).
+struct Lin {
+    double dux, duy, dvx, dvy;  // d(ud)/dx, d(ud)/dy, d(vd)/dx, d(vd)/dy
+    float xi, eta, kc;
+    int cell;  // (j << 16) | i
+};
+
+// 1 / c by two Newton steps from the hardware reciprocal estimate (the
+// quotient the trajectory uses stays an IEEE division).
+__device__ __forceinline__ double recip(double c) {
+    double r = __builtin_amdgcn_rcp(c);
+    r = fma(r, fma(-c, r, 1.0), r);
+    return fma(r, fma(-c, r, 1.0), r);
+}
+
+// Samples stage st at (x, y) like adj_sample (so ud, vd are bit-identical)
+// and fills L; false outside the grid.
+__device__ __forceinline__ bool adj_linearise(const AdjField& F, const AdjStage& st, double x, double y, double* ud,
+                                              double* vd, Lin* L) {
+    int i, j;
+    double xi, eta;
+    if (!adj_locate(F.g, x, y, &i, &j, &xi, &eta)) return false;
+    double2 q[4];
+    adj_corners(F, st, i, j, q);
+    const double ub = fma(xi, q[1].x - q[0].x, q[0].x);
+    const double ut = fma(xi, q[3].x - q[2].x, q[2].x);
+    const double u = fma(eta, ut - ub, ub);
+    const double vb = fma(xi, q[1].y - q[0].y, q[0].y);
+    const double vt = fma(xi, q[3].y - q[2].y, q[2].y);
+    const double v = fma(eta, vt - vb, vb);
+    const double c = cos_deg(y);
+    *ud = (u * LM_INV_M_PER_DEG) / c;
+    *vd = v * LM_INV_M_PER_DEG;
+    const double rc = recip(c);
+    const double kc = LM_INV_M_PER_DEG * rc;
+    const double du_dx = fma(eta, (q[3].x - q[2].x) - (q[1].x - q[0].x), q[1].x - q[0].x) * F.g.inv_dx;
+    const double dv_dx = fma(eta, (q[3].y - q[2].y) - (q[1].y - q[0].y), q[1].y - q[0].y) * F.g.inv_dx;
+    const double du_dy = (ut - ub) * F.g.inv_dy;
+    const double dv_dy = (vt - vb) * F.g.inv_dy;
+    // d(1/cos(lat))/dlat = (pi/180) sin(lat) / cos(lat)^2
+    L->dux = kc * du_dx;
+    L->duy = kc * fma(u * LM_DEG2RAD * rc, sin_deg(y), du_dy);
+    L->dvx = LM_INV_M_PER_DEG * dv_dx;
+    L->dvy = LM_INV_M_PER_DEG * dv_dy;
+    L->xi = (float)xi;
+    L->eta = (float)eta;
+    L->kc = (float)kc;
+    L->cell = (j << 16) | i;
+    return true;
+}
+
+// Field-gradient accumulation in moment form.  Samples in one cell (j, i)
+// at one level pair (l, l + 1) with velocity adjoint g and time weight w
+// contribute g (1 - w) b(xi, eta) to level l and g w b(xi, eta) to level
+// l + 1, b being the four bilinear corner weights.  Being linear, they are
+// summed as the moments S = sum g (1, xi, eta, xi eta) and W = sum g w (...)
+// and turned into corner values only once: corner (0,0) of m is
+// m0 - m1 - m2 + m3, (0,1) is m1 - m3, (1,0) is m2 - m3, (1,1) is m3, with
+// m = S - W for level l and m = W for level l + 1.
+//
+// LDS window: one slot of 16 floats, [comp][S, W][4 moments], per level
+// pair p in [lo, lo + npair) and cell of the block's box (ch x cw cells from
+// (j0, i0)); the write-out turns slots into node sums.
+struct GradSink {
+    float* lds;     // NULL: add to global memory
+    float* global;  // [nlev][ny][nx][2]
+    int nlev, nx;
+    int64_t level_len;
+    int lo, npair, j0, i0, ch, cw;
+};
+
+struct GradCache {
+    int l, cell;  // l < 0: empty; cell = (j << 16) | i
+    float m[16];  // [comp][S, W][moment]
+};
+
+__device__ __forceinline__ float corner_of(const float* m, int dj, int di) {
+    const float m3 = m[3];
+    if (dj == 0 && di == 0) return (m[0] - m[1]) - (m[2] - m3);
+    if (dj == 0) return m[1] - m3;
+    if (di == 0) return m[2] - m3;
+    return m3;
+}
+
+__device__ __forceinline__ void cache_flush(GradCache& gc, const GradSink& sk) {
+    if (gc.l < 0) return;
+    const int cj = gc.cell >> 16, ci = gc.cell & 0xffff;
+    const int wl = gc.l - sk.lo, wj = cj - sk.j0, wi = ci - sk.i0;
+    // the recompute's box covers every sample; the test keeps LDS writes in the window regardless
+    if (sk.lds && (unsigned)wl < (unsigned)sk.npair && (unsigned)wj < (unsigned)sk.ch && (unsigned)wi < (unsigned)sk.cw) {
+        float* slot = sk.lds + ((wl * sk.ch + wj) * sk.cw + wi) * 16;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) atomicAdd(slot + k, gc.m[k]);
+    } else {
+#pragma unroll
+        for (int comp = 0; comp < 2; ++comp) {
+            const float* S = gc.m + comp * 8;
+            const float* W = S + 4;
+            const float lower[4] = {S[0] - W[0], S[1] - W[1], S[2] - W[2], S[3] - W[3]};
+#pragma unroll
+            for (int lev = 0; lev < 2; ++lev) {
+                const int lv = min(gc.l + lev, sk.nlev - 1);  // a steady field's one level takes both parts
+                const float* m = lev == 0 ? lower : W;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float v = corner_of(m, c >> 1, c & 1);
+                    if (v == 0.0f) continue;
+                    const int64_t at =
+                        ((int64_t)lv * sk.level_len + (int64_t)(cj + (c >> 1)) * sk.nx + (ci + (c & 1))) * 2 + comp;
+                    atomicAdd(sk.global + at, v);
+                }
+            }
+        }
+    }
+    gc.l = -1;
+}
+
+// Field gradient of one stage sample with adjoint (au, av) of its velocity.
+__device__ __forceinline__ void cache_add(GradCache& gc, const GradSink& sk, const AdjStage& st, const Lin& L,
+                                          double au, double av) {
+    if (gc.l != st.l || gc.cell != L.cell) {
+        cache_flush(gc, sk);
+        gc.l = st.l;
+        gc.cell = L.cell;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) gc.m[k] = 0.0f;
+    }
+    const float w = (float)st.w;
+    const float g[2] = {(float)au * L.kc, (float)(av * LM_INV_M_PER_DEG)};
+    const float b[4] = {1.0f, L.xi, L.eta, L.xi * L.eta};
+#pragma unroll
+    for (int comp = 0; comp < 2; ++comp) {
+        const float gw = g[comp] * w;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            gc.m[comp * 8 + k] = fmaf(g[comp], b[k], gc.m[comp * 8 + k]);
+            gc.m[comp * 8 + 4 + k] = fmaf(gw, b[k], gc.m[comp * 8 + 4 + k]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kAdjBlock)
+adj_forward_kernel(int64_t n, double* __restrict__ lon, double* __restrict__ lat, AdjField F, int nsteps, double hh,
+                   double h, double h6, int ckpt, double2* __restrict__ ck, uint32_t* __restrict__ oob) {
+    const int64_t i = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
+    if (i >= n) return;
+    double x = lon[i], y = lat[i];
+    uint32_t out = 0;
+    for (int s = 0; s < nsteps; ++s) {
+        if (s % ckpt == 0) ck[(int64_t)(s / ckpt) * n + i] = make_double2(x, y);
+        if (!adj_rk4(F, s, hh, h, h6, &x, &y, nullptr)) ++out;
+    }
+    lon[i] = x;
+    lat[i] = y;
+    if (out) atomicAdd(oob, out);
+}
+
+// Steps s0 .. s0 + m - 1 from the checkpoint ck: the start position of step
+// s0 + k into pos[k * n + i]; the block's cell box into box[blockIdx.x].
+__global__ void __launch_bounds__(kAdjBlock)
+adj_recompute_kernel(int64_t n, const double2* __restrict__ ck, AdjField F, int s0, int m, double hh, double h,
+                     double h6, double2* __restrict__ pos, int4* __restrict__ box) {
+    __shared__ int sb[4];
+    if (threadIdx.x == 0) {
+        sb[0] = sb[2] = 0x7fffffff;
+        sb[1] = sb[3] = -1;
+    }
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
+    CellBox cb;
+    cb.reset();
+    if (i < n) {
+        const double2 p = ck[i];
+        double x = p.x, y = p.y;
+        for (int k = 0; k < m; ++k) {
+            pos[(int64_t)k * n + i] = make_double2(x, y);
+            adj_rk4(F, s0 + k, hh, h, h6, &x, &y, &cb);
+        }
+    }
+    if (cb.jmax >= 0) {
+        atomicMin(&sb[0], cb.jmin);
+        atomicMax(&sb[1], cb.jmax);
+        atomicMin(&sb[2], cb.imin);
+        atomicMax(&sb[3], cb.imax);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) box[blockIdx.x] = make_int4(sb[0], sb[1], sb[2], sb[3]);
+}
+
+// Steps s0 + m - 1 down to s0, from the recomputed start positions; lambda
+// (glon, glat) goes from the end of the segment to its start.  Level pairs
+// [lo, lo + npair) are the ones the segment's stages read; lds_bytes is the
+// dynamic LDS the launch carries.
+__global__ void __launch_bounds__(kAdjBlock, 3)
+adj_backward_kernel(int64_t n, AdjField F, int nlev, int s0, int m, double hh, double h, double h6,
+                    const double2* __restrict__ pos, const int4* __restrict__ box, double* __restrict__ glon,
+                    double* __restrict__ glat, float* __restrict__ gfield, int lo, int npair, int lds_bytes) {
+    extern __shared__ float win[];
+    const int4 b = box[blockIdx.x];
+    if (b.y < 0) return;  // no lane of the block completed a step inside the grid
+    GradSink sk;
+    sk.global = gfield;
+    sk.nlev = nlev;
+    sk.nx = F.g.nx;
+    sk.level_len = F.level_len;
+    sk.lo = lo;
+    sk.npair = npair;
+    sk.j0 = b.x;
+    sk.i0 = b.z;
+    sk.ch = b.y - b.x + 1;
+    sk.cw = b.w - b.z + 1;
+    const int64_t wcount = (int64_t)npair * sk.ch * sk.cw * 16;
+    const bool use_lds = wcount * 4 <= (int64_t)lds_bytes;
+    sk.lds = use_lds ? win : nullptr;
+    if (use_lds) {
+        for (int t = threadIdx.x; t < (int)wcount; t += kAdjBlock) win[t] = 0.0f;
+        __syncthreads();
+    }
+    GradCache gc;
+    gc.l = -1;
+    const int64_t i = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
+    if (i < n) {
+        double lx = glon[i], ly = glat[i];
+        for (int k = m - 1; k >= 0; --k) {
+            const int s = s0 + k;
+            const double2 p = pos[(int64_t)k * n + i];
+            const double x0 = p.x, y0 = p.y;
+            const AdjStage sa = adj_stage(F, 2 * s), sb = adj_stage(F, 2 * s + 1), sc = adj_stage(F, 2 * s + 2);
+            Lin L1, L2, L3, L4;
+            double ud, vd;
+            if (!adj_linearise(F, sa, x0, y0, &ud, &vd, &L1)) continue;
+            if (!adj_linearise(F, sb, fma(ud, hh, x0), fma(vd, hh, y0), &ud, &vd, &L2)) continue;
+            if (!adj_linearise(F, sb, fma(ud, hh, x0), fma(vd, hh, y0), &ud, &vd, &L3)) continue;
+            if (!adj_linearise(F, sc, fma(ud, h, x0), fma(vd, h, y0), &ud, &vd, &L4)) continue;
+            const double a4x = h6 * lx, a4y = h6 * ly;
+            double a3x = 2.0 * a4x, a3y = 2.0 * a4y;
+            double a2x = a3x, a2y = a3y;
+            double a1x = a4x, a1y = a4y;
+            double gx, gy;
+            // stage 4
+            gx = fma(a4x, L4.dux, a4y * L4.dvx);
+            gy = fma(a4x, L4.duy, a4y * L4.dvy);
+            cache_add(gc, sk, sc, L4, a4x, a4y);
+            lx += gx;
+            ly += gy;
+            a3x = fma(h, gx, a3x);
+            a3y = fma(h, gy, a3y);
+            // stage 3
+            gx = fma(a3x, L3.dux, a3y * L3.dvx);
+            gy = fma(a3x, L3.duy, a3y * L3.dvy);
+            cache_add(gc, sk, sb, L3, a3x, a3y);
+            lx += gx;
+            ly += gy;
+            a2x = fma(hh, gx, a2x);
+            a2y = fma(hh, gy, a2y);
+            // stage 2
+            gx = fma(a2x, L2.dux, a2y * L2.dvx);
+            gy = fma(a2x, L2.duy, a2y * L2.dvy);
+            cache_add(gc, sk, sb, L2, a2x, a2y);
+            lx += gx;
+            ly += gy;
+            a1x = fma(hh, gx, a1x);
+            a1y = fma(hh, gy, a1y);
+            // stage 1
+            gx = fma(a1x, L1.dux, a1y * L1.dvx);
+            gy = fma(a1x, L1.duy, a1y * L1.dvy);
+            cache_add(gc, sk, sa, L1, a1x, a1y);
+            lx += gx;
+            ly += gy;
+        }
+        glon[i] = lx;
+        glat[i] = ly;
+    }
+    cache_flush(gc, sk);
+    if (!use_lds) return;
+    __syncthreads();
+    // node (lv, nj, ni) of the window gathers corner (dj, di) of the cells (nj - dj, ni - di): the lower
+    // part of pair lv and the upper part of pair lv - 1 (both parts of pair 0 for a steady field)
+    const int nh = sk.ch + 1, nw = sk.cw + 1, nlw = nlev > 1 ? npair + 1 : 1;
+    const int nodes = nlw * nh * nw * 2;
+    for (int t = threadIdx.x; t < nodes; t += kAdjBlock) {
+        const int comp = t & 1;
+        int r = t >> 1;
+        const int ni = r % nw;
+        r /= nw;
+        const int nj = r % nh;
+        const int lv = r / nh;
+        float acc = 0.0f;
+#pragma unroll
+        for (int part = 0; part < 2; ++part) {
+            const int pr = nlev > 1 ? lv - part : 0;
+            if (pr < 0 || pr >= npair) continue;
+#pragma unroll
+            for (int dj = 0; dj < 2; ++dj) {
+                const int cj = nj - dj;
+                if (cj < 0 || cj >= sk.ch) continue;
+#pragma unroll
+                for (int di = 0; di < 2; ++di) {
+                    const int ci = ni - di;
+                    if (ci < 0 || ci >= sk.cw) continue;
+                    const float* S = win + ((pr * sk.ch + cj) * sk.cw + ci) * 16 + comp * 8;
+                    const float* W = S + 4;
+                    float mm[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        mm[q] = nlev == 1 ? (part == 0 ? S[q] : 0.0f) : (part == 0 ? S[q] - W[q] : W[q]);
+                    acc += corner_of(mm, dj, di);
+                }
+            }
+        }
+        if (acc != 0.0f)
+            atomicAdd(gfield + ((int64_t)(lo + lv) * F.level_len + (int64_t)(sk.j0 + nj) * F.g.nx + (sk.i0 + ni)) * 2 +
+                          comp,
+                      acc);
+    }
+}
+
+}  // namespace lm
+
+namespace {
+
+// Dynamic LDS each backward block may use for its gradient window: 40 KB
+// lets four blocks share a CU's 160 KB; a block whose box needs more adds to
+// global memory directly.
+constexpr int kAdjWindowBytes = 40960;
+
+int adj_check(int64_t n, int32_t nlev, const lm_grid* grid, const int32_t* lev0_host, int32_t nsteps,
+              int32_t ckpt) {
+    LM_REQUIRE(n >= 0 && nsteps >= 0, LM_ERR_ARG, "negative size");
+    LM_REQUIRE(n <= 0x7fffffffLL * lm::kAdjBlock, LM_ERR_ARG, "too many particles");
+    LM_REQUIRE(grid != nullptr, LM_ERR_ARG, "grid is NULL");
+    LM_REQUIRE(grid->nx >= 2 && grid->ny >= 2, LM_ERR_ARG, "grid needs at least 2x2 nodes (got %dx%d)", grid->nx,
+               grid->ny);
+    LM_REQUIRE(grid->nx <= 65536 && grid->ny <= 65536, LM_ERR_ARG, "at most 65536 nodes along each axis");
+    LM_REQUIRE(nlev >= 1, LM_ERR_ARG, "at least one time level");
+    LM_REQUIRE(ckpt >= 1, LM_ERR_ARG, "checkpoint interval must be >= 1");
+    if (n == 0 || nsteps == 0) return LM_OK;
+    LM_REQUIRE(lev0_host != nullptr, LM_ERR_ARG, "lev0_host is NULL");
+    for (int s = 0; s <= 2 * nsteps; ++s) {
+        const int l = lev0_host[s];
+        LM_REQUIRE(l >= 0 && (nlev == 1 ? l == 0 : l + 1 < nlev), LM_ERR_ARG,
+                   "stage time %d: level %d does not fit %d level(s)", s, l, nlev);
+    }
+    return LM_OK;
+}
+
+lm::AdjField adj_field(const float* field, int32_t nlev, const lm_grid* grid, const int32_t* lev0, const double* w) {
+    lm::AdjField F;
+    F.f = reinterpret_cast<const float2*>(field);
+    F.lev0 = lev0;
+    F.w = w;
+    F.level_len = (int64_t)grid->nx * grid->ny;
+    F.lstride = nlev > 1 ? F.level_len : 0;
+    F.g = *grid;
+    return F;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lm_adjoint_forward(int64_t n, double* lon, double* lat, const float* field, int32_t nlev, const lm_grid* grid,
+                       const int32_t* lev0, const double* w, const int32_t* lev0_host, int32_t nsteps, double dt,
+                       int32_t ckpt, double* checkpoints, uint32_t* oob_count, void* stream) {
+    int rc = adj_check(n, nlev, grid, lev0_host, nsteps, ckpt);
+    if (rc) return rc;
+    if (n == 0 || nsteps == 0) return LM_OK;
+    LM_REQUIRE(lon && lat && field && lev0 && w && checkpoints && oob_count, LM_ERR_ARG, "NULL pointer");
+    LM_REQUIRE(((uintptr_t)field & 7) == 0 && ((uintptr_t)checkpoints & 15) == 0, LM_ERR_ARG,
+               "field must be 8-byte and checkpoints 16-byte aligned");
+    const lm::AdjField F = adj_field(field, nlev, grid, lev0, w);
+    LM_LAUNCH(lm::K_ADJ_FORWARD, lm::adj_forward_kernel, dim3(lm::grid_for(n, lm::kAdjBlock)), dim3(lm::kAdjBlock),
+              0, (hipStream_t)stream, n, lon, lat, F, (int)nsteps, 0.5 * dt, dt, dt / 6.0, (int)ckpt,
+              reinterpret_cast<double2*>(checkpoints), oob_count);
+    LM_LAUNCH_CHECK();
+    return LM_OK;
+}
+
+int64_t lm_adjoint_scratch_bytes(int64_t n, int32_t ckpt) {
+    if (n < 0 || ckpt < 1) return -1;
+    const int64_t blocks = (n + lm::kAdjBlock - 1) / lm::kAdjBlock;
+    return (int64_t)ckpt * n * 16 + blocks * 16;
+}
+
+int lm_adjoint_backward(int64_t n, const double* checkpoints, const float* field, int32_t nlev, const lm_grid* grid,
+                        const int32_t* lev0, const double* w, const int32_t* lev0_host, int32_t nsteps, double dt,
+                        int32_t ckpt, void* scratch, int64_t scratch_bytes, double* glon, double* glat, float* gfield,
+                        void* stream) {
+    int rc = adj_check(n, nlev, grid, lev0_host, nsteps, ckpt);
+    if (rc) return rc;
+    if (n == 0 || nsteps == 0) return LM_OK;
+    LM_REQUIRE(checkpoints && field && lev0 && w && scratch && glon && glat && gfield, LM_ERR_ARG, "NULL pointer");
+    LM_REQUIRE(((uintptr_t)field & 7) == 0 && ((uintptr_t)checkpoints & 15) == 0 && ((uintptr_t)scratch & 15) == 0,
+               LM_ERR_ARG, "field must be 8-byte, checkpoints and scratch 16-byte aligned");
+    LM_REQUIRE(scratch_bytes >= lm_adjoint_scratch_bytes(n, ckpt), LM_ERR_ARG,
+               "scratch holds %lld bytes, %lld needed", (long long)scratch_bytes,
+               (long long)lm_adjoint_scratch_bytes(n, ckpt));
+    hipStream_t s = (hipStream_t)stream;
+    const lm::AdjField F = adj_field(field, nlev, grid, lev0, w);
+    const unsigned nb = lm::grid_for(n, lm::kAdjBlock);
+    double2* pos = reinterpret_cast<double2*>(scratch);
+    int4* box = reinterpret_cast<int4*>(pos + (int64_t)ckpt * n);
+    const double2* ck = reinterpret_cast<const double2*>(checkpoints);
+    const double hh = 0.5 * dt, h6 = dt / 6.0;
+    const int nseg = (nsteps + ckpt - 1) / ckpt;
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        const int s0 = seg * ckpt, m = nsteps - s0 < ckpt ? nsteps - s0 : ckpt;
+        int lo = 0x7fffffff, hi = -1;  // level pairs the segment's stage times read
+        for (int q = 2 * s0; q <= 2 * (s0 + m); ++q) {
+            const int l = lev0_host[q];
+            lo = l < lo ? l : lo;
+            hi = l > hi ? l : hi;
+        }
+        LM_LAUNCH(lm::K_ADJ_RECOMPUTE, lm::adj_recompute_kernel, dim3(nb), dim3(lm::kAdjBlock), 0, s, n,
+                  ck + (int64_t)seg * n, F, s0, m, hh, dt, h6, pos, box);
+        LM_LAUNCH_CHECK();
+        LM_LAUNCH(lm::K_ADJ_BACKWARD, lm::adj_backward_kernel, dim3(nb), dim3(lm::kAdjBlock),
+                  (unsigned)kAdjWindowBytes, s, n, F, (int)nlev, s0, m, hh, dt, h6, pos, box, glon, glat, gfield, lo,
+                  hi - lo + 1, kAdjWindowBytes);
+        LM_LAUNCH_CHECK();
+    }
+    return LM_OK;
+}
+
+}  // extern "C"

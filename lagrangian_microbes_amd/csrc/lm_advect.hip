@@ -20,9 +20,6 @@
 
 namespace lm {
 
-// 1 / (1852 * 60): degrees per metre of arc.
-#define LM_INV_M_PER_DEG 0x1.2df73a213fa8bp-17
-
 __device__ __forceinline__ bool sample_uv(const double* __restrict__ f, const lm_grid& g, double x,
                                           double y, double* ud, double* vd) {
     const double fx = (x - g.x0) * g.inv_dx;

@@ -76,6 +76,9 @@ enum KernelId {
     K_COPY,
     K_SORT_HIST,
     K_SORT_SCATTER,
+    K_ADJ_FORWARD,
+    K_ADJ_RECOMPUTE,
+    K_ADJ_BACKWARD,
     K_NUM_KERNELS
 };
 bool timing_on();
@@ -170,6 +173,8 @@ constexpr uint64_t kStreamWalk = 1;
 // with explicit fma.  Written out so CPU and GPU evaluate the same operations.
 // ----------------------------------------------------------------------------
 #define LM_DEG2RAD 0.017453292519943295
+// 1 / (1852 * 60): degrees per metre of arc.
+#define LM_INV_M_PER_DEG 0x1.2df73a213fa8bp-17
 // XCD-aware block order.  MI355X has 8 XCDs, each with its own L2, and hands
 // workgroups to them round-robin by linear workgroup id.  This maps the
 // dispatch order so that XCD x runs the x-th contiguous eighth of the

@@ -26,6 +26,11 @@ Contents and what pins each function (the checks live in tests/):
 - ``pairs_bruteforce`` (C) and :mod:`oracle.pairs` (numpy): query_pairs
   semantics (interaction_simulator.py:93,98).  Pinned against
   scipy.spatial.cKDTree.query_pairs, the library the reference calls.
+- :mod:`oracle.adjoint_ref` (torch float64 on the CPU): the RK4 advection
+  written with differentiable torch operations, whose autograd gradients the
+  hand-written adjoint is checked against.  Pinned by its positions against
+  ``advect_rk4_f64``, by central finite differences of its own forward, and by
+  the closed forms of a uniform zonal flow.
 """
 import ctypes
 import os

@@ -43,7 +43,7 @@ def test_binding_covers_header(liblm):
 
 
 def test_abi_version_and_argument_errors(liblm):
-    assert liblm.lm_abi_version() == 3
+    assert liblm.lm_abi_version() == 4
     # argument validation runs on the host before any launch: no GPU is touched
     rc = liblm.lm_advect_rk4_f64(-1, None, None, None, 0, None, 0, 1.0, None, None, 0, 0.0, 0, 0, 0, None, None)
     assert rc != 0 and b"negative" in liblm.lm_last_error()
@@ -51,3 +51,25 @@ def test_abi_version_and_argument_errors(liblm):
     assert rc != 0 and b"word" in liblm.lm_last_error()
     rc = liblm.lm_query_pairs(None, 0, None, None, 0.1, 2.0, None, None, 0, None, None)
     assert rc != 0 and b"workspace" in liblm.lm_last_error()
+
+
+def test_adjoint_argument_errors(liblm):
+    from lagrangian_microbes_amd._lib import LmGrid
+    import numpy as np
+    g = LmGrid(5, 5, 0.0, 0.0, 4.0, 4.0)
+    lev = np.array([0, 0, 1, 1, 2], dtype=np.int32)  # 2 steps; level 2 + 1 does not exist with 3 levels
+    rc = liblm.lm_adjoint_forward(10, None, None, None, 3, ctypes.byref(g), None, None,
+                                  lev.ctypes.data_as(ctypes.c_void_p), 2, 3600.0, 16, None, None, None)
+    assert rc != 0 and b"level 2" in liblm.lm_last_error()
+    rc = liblm.lm_adjoint_backward(10, None, None, 3, ctypes.byref(g), None, None,
+                                   lev.ctypes.data_as(ctypes.c_void_p), 2, 3600.0, 0, None, 0, None, None, None,
+                                   None)
+    assert rc != 0 and b"checkpoint" in liblm.lm_last_error()
+    ok = np.array([0, 0, 1, 1, 1], dtype=np.int32)
+    rc = liblm.lm_adjoint_backward(10, None, None, 3, ctypes.byref(g), None, None,
+                                   ok.ctypes.data_as(ctypes.c_void_p), 2, 3600.0, 16, None, 0, None, None, None,
+                                   None)
+    assert rc != 0 and b"NULL" in liblm.lm_last_error()
+    # scratch: 16 start positions of 16 bytes per particle, plus one 16-byte box per 256-particle block
+    assert liblm.lm_adjoint_scratch_bytes(1000, 16) == 16 * 1000 * 16 + 4 * 16
+    assert liblm.lm_adjoint_scratch_bytes(-1, 16) == -1
