@@ -8,6 +8,13 @@ sorted pairs, with positions and species recorded on the device.  The default
 workload is 490,000 microbes on a 700x700 lattice in a seeded 30-day
 synthetic field (720 hourly steps); the timed region runs K steps from the
 initial state with inputs already resident in HBM.  Prints one JSON line.
+
+The line's ``adjoint`` object times the differentiable advection of 1,000,000
+particles over 240 hourly steps through 20 float32 field levels
+(lagrangian_microbes_amd.adjoint): the forward with checkpoints every 16
+steps, and loss.backward() of the mean squared great-circle distance to
+targets from a second seeded field (recompute plus reverse sweep);
+``adjoint_forward_ratio`` is backward time over forward time.
 """
 import argparse
 import json
@@ -37,6 +44,8 @@ def parse():
     ap.add_argument("--workload", default="rps_490k_30day")
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint forward/backward timing")
+    ap.add_argument("--adjoint-reps", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=48)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel on BYTES bytes 10 times (byte-counter calibration)")
@@ -172,6 +181,54 @@ def cpu_baseline(w, runner, steps):
     }
 
 
+def adjoint_leg(reps):
+    """The adjoint workload (1M particles, 240 steps, 20 float32 levels): times the
+    differentiable forward and loss.backward() with events on the stream they run on,
+    and the liblm kernels of each pass with their own dispatch events."""
+    import torch
+    from lagrangian_microbes_amd import adjoint, configs, device as dev
+
+    w = configs.adjoint_1m_10day()
+    f, tf = w.velocity_field(), w.target_velocity_field()
+    lon, lat = w.particles()
+    lon_d, lat_d = torch.tensor(lon, device="cuda"), torch.tensor(lat, device="cuda")
+    plan = adjoint.AdjointPlan(f, 0.0, w.dt, w.steps, ckpt=16)
+    tplan = adjoint.AdjointPlan(tf, 0.0, w.dt, w.steps, ckpt=16)
+    tu = torch.tensor(tf.u, dtype=torch.float32, device="cuda")
+    tv = torch.tensor(tf.v, dtype=torch.float32, device="cuda")
+    with torch.no_grad():
+        TX, TY = adjoint.advect(lon_d, lat_d, tu, tv, tplan)
+    u = torch.tensor(f.u, dtype=torch.float32, device="cuda", requires_grad=True)
+    v = torch.tensor(f.v, dtype=torch.float32, device="cuda", requires_grad=True)
+    x0 = lon_d.clone().requires_grad_(True)
+    y0 = lat_d.clone().requires_grad_(True)
+
+    def one(timed):
+        u.grad = v.grad = x0.grad = y0.grad = None
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record()
+        X, Y = adjoint.advect(x0, y0, u, v, plan)
+        e[1].record()
+        L = adjoint.trajectory_loss(X, Y, TX, TY)
+        L.backward()
+        e[2].record()
+        torch.cuda.synchronize()
+        return e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2]), float(L.detach())
+
+    one(False)
+    runs = [one(True) for _ in range(reps)]
+    fwd = min(r[0] for r in runs)
+    bwd = min(r[1] for r in runs)
+    with dev.KernelTimer() as timer:
+        one(True)
+    kt = {k: {"launches": c, "total_ms": round(ms, 3)} for k, (c, ms) in timer.summary().items()}
+    log(f"[bench] adjoint: forward {fwd:.2f} ms, backward {bwd:.2f} ms, ratio {bwd / fwd:.2f}")
+    return {"workload": w.name, "particles": w.n, "steps": w.steps, "checkpoint_every": plan.ckpt,
+            "levels": f.nt, "grid": [f.ny, f.nx], "loss_km2": runs[-1][2], "forward_ms": round(fwd, 3),
+            "backward_ms": round(bwd, 3), "ratio": round(bwd / fwd, 3),
+            "particle_steps_per_s_forward": w.n * w.steps / (fwd * 1e-3), "kernels": kt}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -262,6 +319,7 @@ def main():
                  "achieved_gbs": round(path_bytes / (elapsed / args.steps) / 1e9, 1)},
         "adjoint_forward_ratio": None,
     }
+
     if not args.no_cpu:
         runner.reset()
         run_steps(runner, w.steps // 2, w.steps)
@@ -270,6 +328,11 @@ def main():
         result["cpu_baseline"] = cpu_baseline(w, runner, args.cpu_steps)
     else:
         result["cpu_baseline"] = None
+    if not args.no_adjoint:
+        del runner
+        torch.cuda.empty_cache()
+        result["adjoint"] = adjoint_leg(args.adjoint_reps)
+        result["adjoint_forward_ratio"] = result["adjoint"]["ratio"]
     print(json.dumps(result), flush=True)
 
 

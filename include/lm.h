@@ -211,26 +211,32 @@ int lm_philox_uniforms(int64_t count, uint64_t c0, uint64_t c1, uint64_t c2, uin
  * lm_field_blend_f64.  lev0 and w are device arrays of 2*nsteps + 1 entries;
  * lev0_host is a host copy of lev0, checked against nlev before any launch.
  * A step with a stage outside the grid keeps its position and adds one to
- * *oob_count. */
+ * *oob_count.  Both calls blend the stage times they need into float64
+ * stage fields (16 bytes per node) in the caller's scratch. */
 
 /* Advances lon/lat (in place) by nsteps RK4 steps of dt seconds and stores
  * the positions at the start of steps 0, ckpt, 2*ckpt, ... as interleaved
  * (lon, lat) doubles: checkpoints[(s/ckpt)*n + i][2], ceil(nsteps/ckpt)*n*2
- * doubles, 16-byte aligned. */
+ * doubles, 16-byte aligned.  scratch (16-byte aligned) holds the stage
+ * fields of as many steps as fit: at least 3 fields (1 when nlev == 1);
+ * 2*nsteps + 1 fields run the whole forward in one launch. */
 int lm_adjoint_forward(int64_t n, double* lon, double* lat, const float* field, int32_t nlev, const lm_grid* grid,
                        const int32_t* lev0, const double* w, const int32_t* lev0_host, int32_t nsteps, double dt,
-                       int32_t ckpt, double* checkpoints, uint32_t* oob_count, void* stream);
+                       int32_t ckpt, double* checkpoints, void* scratch, int64_t scratch_bytes, uint32_t* oob_count,
+                       void* stream);
 
-/* Bytes of device scratch lm_adjoint_backward needs for n particles and
- * checkpoint interval ckpt (-1 on invalid arguments). */
-int64_t lm_adjoint_scratch_bytes(int64_t n, int32_t ckpt);
+/* Bytes of device scratch lm_adjoint_backward needs for n particles,
+ * checkpoint interval ckpt and the field's grid and level count (-1 on
+ * invalid arguments). */
+int64_t lm_adjoint_scratch_bytes(int64_t n, int32_t ckpt, const lm_grid* grid, int32_t nlev);
 
 /* Reverse pass over the run that lm_adjoint_forward made with the same
  * arguments.  On entry glon/glat hold dL/d(final lon, lat); on return
  * dL/d(initial lon, lat).  dL/d(field) is ADDED into gfield (float32, the
  * field's layout; the caller zeroes it).  Each segment of ckpt steps is
  * recomputed from its checkpoint into scratch (16-byte aligned, at least
- * lm_adjoint_scratch_bytes(n, ckpt) bytes), then swept in reverse. */
+ * lm_adjoint_scratch_bytes(n, ckpt, grid, nlev) bytes), then swept in
+ * reverse. */
 int lm_adjoint_backward(int64_t n, const double* checkpoints, const float* field, int32_t nlev, const lm_grid* grid,
                         const int32_t* lev0, const double* w, const int32_t* lev0_host, int32_t nsteps, double dt,
                         int32_t ckpt, void* scratch, int64_t scratch_bytes, double* glon, double* glat, float* gfield,

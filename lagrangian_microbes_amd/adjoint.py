@@ -68,6 +68,19 @@ class AdjointPlan:
         return ctypes.c_void_p(self.lev0_host.ctypes.data)
 
 
+# The forward blends the float64 stage fields of as many steps at once as
+# this budget holds (all 481 stage times of 240 steps on an 81x81 grid take
+# 50 MB).
+FORWARD_STAGE_BUDGET = 256 << 20
+
+
+def forward_scratch_bytes(plan):
+    field_bytes = plan.nx * plan.ny * 16
+    if plan.nlev == 1:
+        return field_bytes
+    return max(3, min(2 * plan.nsteps + 1, FORWARD_STAGE_BUDGET // field_bytes)) * field_bytes
+
+
 def forward_positions(lon, lat, field, plan, checkpoints):
     """lm_adjoint_forward on lon/lat (float64, in place); field from
     ``plan.interleave``; checkpoints float64 [nseg, n, 2]."""
@@ -76,9 +89,11 @@ def forward_positions(lon, lat, field, plan, checkpoints):
     require(lat, torch.float64, numel=n, name="lat")
     require(field, torch.float32, numel=plan.nlev * plan.ny * plan.nx * 2, name="field")
     require(checkpoints, torch.float64, min_numel=plan.nseg * n * 2, name="checkpoints")
+    scratch = torch.empty(forward_scratch_bytes(plan), dtype=torch.uint8, device=lon.device)
     check(_lib.load().lm_adjoint_forward(n, ptr(lon), ptr(lat), ptr(field), plan.nlev, ctypes.byref(plan.grid),
                                          ptr(plan.lev0), ptr(plan.w), plan.lev0_host_ptr(), plan.nsteps, plan.dt,
-                                         plan.ckpt, ptr(checkpoints), ptr(plan.oob), stream_ptr()))
+                                         plan.ckpt, ptr(checkpoints), ptr(scratch), scratch.numel(), ptr(plan.oob),
+                                         stream_ptr()))
 
 
 def backward_positions(field, plan, checkpoints, glon, glat, gfield):
@@ -91,7 +106,7 @@ def backward_positions(field, plan, checkpoints, glon, glat, gfield):
     require(gfield, torch.float32, numel=field.numel(), name="gfield")
     require(checkpoints, torch.float64, min_numel=plan.nseg * n * 2, name="checkpoints")
     L = _lib.load()
-    nbytes = L.lm_adjoint_scratch_bytes(n, plan.ckpt)
+    nbytes = L.lm_adjoint_scratch_bytes(n, plan.ckpt, ctypes.byref(plan.grid), plan.nlev)
     if nbytes < 0:
         raise ValueError("invalid particle count or checkpoint interval")
     scratch = torch.empty(max(int(nbytes), 16), dtype=torch.uint8, device=glon.device)

@@ -2,23 +2,27 @@
 // gradients of a loss on the final positions with respect to the initial
 // positions and to every u, v value of every time level.
 //
-// Field: float32 levels [nlev][ny][nx] of interleaved (u, v); a stage at
-// schedule entry s reads level lev0[s] and lev0[s] + 1 (the same level for a
-// steady field, nlev == 1) and blends each corner node as
-// fma(w[s], (double)f1 - (double)f0, (double)f0): the FP64 path's operations
-// on the field widened to double, so forward positions are bit-identical to
-// the FP64 advection of the widened field.
+// Field: float32 levels [nlev][ny][nx] of interleaved (u, v).  Stage time s
+// (t0 + s dt/2, s = 0 .. 2 nsteps) reads level l = lev0[s] and l + 1 (the one
+// level of a steady field, nlev == 1) blended node by node as
+// fma(w[s], (double)f1 - (double)f0, (double)f0): the FP64 path's blend of
+// the field widened to double.  adj_blend_kernel writes these float64 stage
+// fields for a run of stage times into a buffer, so every sample reads four
+// ready double2 corners and forward positions are bit-identical to the FP64
+// advection of the widened field.
 //
-// Forward (lm_adjoint_forward): RK4 steps as the FP64 path, keeping the
-// positions at the start of every segment of `ckpt` steps.
+// Forward (lm_adjoint_forward): RK4 steps as the FP64 path, in chunks of
+// steps whose stage fields fit the caller's buffer, keeping the positions at
+// the start of every segment of `ckpt` steps.
 //
 // Backward (lm_adjoint_backward), segment by segment from the last:
-//   recompute  from the segment's checkpoint, the start position of every step
-//              into scratch, and per block the bounding box of the cells its
-//              stage samples touch
-//   backward   steps in reverse.  Each step first re-samples its four stages
-//              from the stored start position (same operations as the forward,
-//              so the same trajectory) keeping each stage's velocity Jacobian
+//   blend      the segment's 2 m + 1 stage fields (for the recompute)
+//   recompute  from the segment's checkpoint, the start position of every
+//              step into scratch, and per block the bounding box of the cells
+//              its stage samples touch
+//   backward   steps in reverse.  Each step re-samples its four stages from the
+//              stored start position (corners blended from the float32
+//              levels, which stay in L2) keeping each stage's velocity Jacobian
 //              and interpolation weights in registers, then sweeps back with
 //              lambda = dL/dx at the step's end:
 //                a4 = h/6 l, a3 = a2 = h/3 l, a1 = h/6 l, dL/dx0 = l
@@ -30,14 +34,17 @@
 //              weights times 1 - w, w for the two levels).  A step with a stage
 //              outside the grid left the position unchanged: identity.
 // Velocity in degrees per second: ud = u K / cos(lat), vd = v K, K = 1/111120,
-// so dud/dlat = K (du/dlat / c + u sin(lat) (pi/180) / c^2).
+// so dud/dlat = K (du/dlat / c + u sin(lat) (pi/180) / c^2).  Recompute and
+// backward take 1/cos(lat) from a Newton-refined reciprocal instead of the
+// forward's IEEE division: their trajectory only positions the gradient and
+// stays within rounding of the forward's.
 //
 // Field-gradient accumulation: per lane in registers while successive stage
-// samples stay in one cell and level (a particle crosses a 0.25 degree cell
-// in several steps), then per block in an LDS window (the segment's levels
-// over the block's bounding box), then one global atomic per touched window
-// entry.  A block whose window does not fit the LDS it was launched with adds
-// straight to global memory.
+// samples stay in one cell and level pair (a particle crosses a 0.25 degree
+// cell in several steps), then per block in an LDS window (the segment's
+// level pairs over the block's bounding box), then one global atomic per
+// touched node.  A block whose window does not fit the LDS it was launched
+// with adds its moments to a global slab, turned into node values at the end.
 #include "lm_common.h"
 
 namespace lm {
@@ -62,24 +69,30 @@ __device__ __forceinline__ double sin_deg(double deg) {
     return p * x;
 }
 
+// 1 / c by two Newton steps from the hardware reciprocal estimate.
+__device__ __forceinline__ double recip(double c) {
+    double r = __builtin_amdgcn_rcp(c);
+    r = fma(r, fma(-c, r, 1.0), r);
+    return fma(r, fma(-c, r, 1.0), r);
+}
+
+// Stage fields of stage times [k0, k0 + count) in a buffer, plus the
+// schedule the field gradient needs.
 struct AdjField {
-    const float2* f;
+    const double2* st;   // st[(s - k0) * sstride + node]
+    int64_t sstride;     // level_len, or 0 for a steady field (one stage field)
+    int k0;
     const int32_t* lev0;
     const double* w;
-    int64_t level_len;  // nodes per level
-    int64_t lstride;    // level_len, or 0 for a steady field
+    int64_t level_len;
     lm_grid g;
 };
 
 // Schedule entry s: first level and weight.
 struct AdjStage {
-    int l;
+    int s, l;
     double w;
 };
-
-__device__ __forceinline__ AdjStage adj_stage(const AdjField& F, int s) {
-    return AdjStage{F.lev0[s], F.w[s]};
-}
 
 // Cell and fractional position as the FP64 path computes them; false outside.
 __device__ __forceinline__ bool adj_locate(const lm_grid& g, double x, double y, int* i, int* j, double* xi,
@@ -97,25 +110,24 @@ __device__ __forceinline__ bool adj_locate(const lm_grid& g, double x, double y,
     return true;
 }
 
-// Corners (j,i), (j,i+1), (j+1,i), (j+1,i+1), blended in time.
-__device__ __forceinline__ void adj_corners(const AdjField& F, const AdjStage& st, int i, int j, double2 q[4]) {
-    const int64_t o = (int64_t)st.l * F.level_len + (int64_t)j * F.g.nx + i;
-    const int64_t off[4] = {o, o + 1, o + F.g.nx, o + F.g.nx + 1};
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const float2 a = F.f[off[c]], b = F.f[off[c] + F.lstride];
-        q[c] = make_double2(fma(st.w, (double)b.x - (double)a.x, (double)a.x),
-                            fma(st.w, (double)b.y - (double)a.y, (double)a.y));
-    }
+// Corners (j,i), (j,i+1), (j+1,i), (j+1,i+1) of stage time s.
+__device__ __forceinline__ void adj_corners(const AdjField& F, int s, int i, int j, double2 q[4]) {
+    const double2* r0 = F.st + (int64_t)(s - F.k0) * F.sstride + (int64_t)j * F.g.nx + i;
+    const double2* r1 = r0 + F.g.nx;
+    q[0] = r0[0];
+    q[1] = r0[1];
+    q[2] = r1[0];
+    q[3] = r1[1];
 }
 
-__device__ __forceinline__ bool adj_sample(const AdjField& F, const AdjStage& st, double x, double y, double* ud,
-                                           double* vd, int* ci, int* cj) {
+// The FP64 path's sample, bit for bit; also returns the cell.
+__device__ __forceinline__ bool adj_sample(const AdjField& F, int s, double x, double y, double* ud, double* vd,
+                                           int* ci, int* cj) {
     int i, j;
     double xi, eta;
     if (!adj_locate(F.g, x, y, &i, &j, &xi, &eta)) return false;
     double2 q[4];
-    adj_corners(F, st, i, j, q);
+    adj_corners(F, s, i, j, q);
     const double ub = fma(xi, q[1].x - q[0].x, q[0].x);
     const double ut = fma(xi, q[3].x - q[2].x, q[2].x);
     const double u = fma(eta, ut - ub, ub);
@@ -123,6 +135,27 @@ __device__ __forceinline__ bool adj_sample(const AdjField& F, const AdjStage& st
     const double vt = fma(xi, q[3].y - q[2].y, q[2].y);
     const double v = fma(eta, vt - vb, vb);
     *ud = (u * LM_INV_M_PER_DEG) / cos_deg(y);
+    *vd = v * LM_INV_M_PER_DEG;
+    *ci = i;
+    *cj = j;
+    return true;
+}
+
+// The same with 1/cos(lat) from recip(): within rounding of adj_sample.
+__device__ __forceinline__ bool adj_sample_fast(const AdjField& F, int s, double x, double y, double* ud,
+                                                double* vd, int* ci, int* cj) {
+    int i, j;
+    double xi, eta;
+    if (!adj_locate(F.g, x, y, &i, &j, &xi, &eta)) return false;
+    double2 q[4];
+    adj_corners(F, s, i, j, q);
+    const double ub = fma(xi, q[1].x - q[0].x, q[0].x);
+    const double ut = fma(xi, q[3].x - q[2].x, q[2].x);
+    const double u = fma(eta, ut - ub, ub);
+    const double vb = fma(xi, q[1].y - q[0].y, q[0].y);
+    const double vt = fma(xi, q[3].y - q[2].y, q[2].y);
+    const double v = fma(eta, vt - vb, vb);
+    *ud = u * (LM_INV_M_PER_DEG * recip(cos_deg(y)));
     *vd = v * LM_INV_M_PER_DEG;
     *ci = i;
     *cj = j;
@@ -144,19 +177,22 @@ struct CellBox {
     }
 };
 
-// One RK4 step in the FP64 path's operation order; false (position kept) if
-// a stage leaves the grid.  With box != NULL the stage cells of a completed
-// step are added to it.
+// One RK4 step in the FP64 path's operation order (FAST: with
+// adj_sample_fast); false (position kept) if a stage leaves the grid.  With
+// box != NULL the stage cells of a completed step are added to it.
+template <bool FAST>
 __device__ __forceinline__ bool adj_rk4(const AdjField& F, int s, double hh, double h, double h6, double* x,
                                        double* y, CellBox* box) {
-    const AdjStage a = adj_stage(F, 2 * s), b = adj_stage(F, 2 * s + 1), c = adj_stage(F, 2 * s + 2);
     double u1, v1, u2, v2, u3, v3, u4, v4;
     int i1, j1, i2, j2, i3, j3, i4, j4;
     const double x0 = *x, y0 = *y;
-    if (!adj_sample(F, a, x0, y0, &u1, &v1, &i1, &j1)) return false;
-    if (!adj_sample(F, b, fma(u1, hh, x0), fma(v1, hh, y0), &u2, &v2, &i2, &j2)) return false;
-    if (!adj_sample(F, b, fma(u2, hh, x0), fma(v2, hh, y0), &u3, &v3, &i3, &j3)) return false;
-    if (!adj_sample(F, c, fma(u3, h, x0), fma(v3, h, y0), &u4, &v4, &i4, &j4)) return false;
+    auto sample = [&](int st, double px, double py, double* ud, double* vd, int* ci, int* cj) {
+        return FAST ? adj_sample_fast(F, st, px, py, ud, vd, ci, cj) : adj_sample(F, st, px, py, ud, vd, ci, cj);
+    };
+    if (!sample(2 * s, x0, y0, &u1, &v1, &i1, &j1)) return false;
+    if (!sample(2 * s + 1, fma(u1, hh, x0), fma(v1, hh, y0), &u2, &v2, &i2, &j2)) return false;
+    if (!sample(2 * s + 1, fma(u2, hh, x0), fma(v2, hh, y0), &u3, &v3, &i3, &j3)) return false;
+    if (!sample(2 * s + 2, fma(u3, h, x0), fma(v3, h, y0), &u4, &v4, &i4, &j4)) return false;
     const double su = fma(2.0, u2 + u3, u1 + u4);
     const double sv = fma(2.0, v2 + v3, v1 + v4);
     *x = fma(su, h6, x0);
@@ -179,38 +215,53 @@ struct Lin {
     int cell;  // (j << 16) | i
 };
 
-// 1 / c by two Newton steps from the hardware reciprocal estimate (the
-// quotient the trajectory uses stays an IEEE division).
-__device__ __forceinline__ double recip(double c) {
-    double r = __builtin_amdgcn_rcp(c);
-    r = fma(r, fma(-c, r, 1.0), r);
-    return fma(r, fma(-c, r, 1.0), r);
+// The float32 levels as the backward reads them.  Blending corners from the
+// levels (1 MB for 20 levels of 81 x 81 nodes) instead of reading float64
+// stage fields (3.5 MB for a 16-step segment) keeps the reads in L2.
+struct LevelView {
+    const float2* glev;  // [nlev][ny][nx]
+    const int32_t* lev0;
+    const double* w;
+    int64_t level_len;
+    int64_t gstride;  // level_len, or 0 for a steady field
+    lm_grid g;
+};
+
+// Corners of cell (j, i) at stage st, blended as adj_blend_kernel does.
+__device__ __forceinline__ void level_corners(const LevelView& V, const AdjStage& st, int i, int j, double2 q[4]) {
+    const float2* r0 = V.glev + (int64_t)st.l * V.level_len + (int64_t)j * V.g.nx + i;
+    const int64_t ls = V.gstride;
+    const int pitch = V.g.nx;
+    const float2 a[4] = {r0[0], r0[1], r0[pitch], r0[pitch + 1]};
+    const float2 b[4] = {r0[ls], r0[ls + 1], r0[ls + pitch], r0[ls + pitch + 1]};
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        q[c] = make_double2(fma(st.w, (double)b[c].x - (double)a[c].x, (double)a[c].x),
+                            fma(st.w, (double)b[c].y - (double)a[c].y, (double)a[c].y));
 }
 
-// Samples stage st at (x, y) like adj_sample (so ud, vd are bit-identical)
-// and fills L; false outside the grid.
-__device__ __forceinline__ bool adj_linearise(const AdjField& F, const AdjStage& st, double x, double y, double* ud,
+// adj_sample_fast at stage st plus the linearisation; false outside.
+__device__ __forceinline__ bool adj_linearise(const LevelView& V, const AdjStage& st, double x, double y, double* ud,
                                               double* vd, Lin* L) {
     int i, j;
     double xi, eta;
-    if (!adj_locate(F.g, x, y, &i, &j, &xi, &eta)) return false;
+    if (!adj_locate(V.g, x, y, &i, &j, &xi, &eta)) return false;
     double2 q[4];
-    adj_corners(F, st, i, j, q);
+    level_corners(V, st, i, j, q);
     const double ub = fma(xi, q[1].x - q[0].x, q[0].x);
     const double ut = fma(xi, q[3].x - q[2].x, q[2].x);
     const double u = fma(eta, ut - ub, ub);
     const double vb = fma(xi, q[1].y - q[0].y, q[0].y);
     const double vt = fma(xi, q[3].y - q[2].y, q[2].y);
     const double v = fma(eta, vt - vb, vb);
-    const double c = cos_deg(y);
-    *ud = (u * LM_INV_M_PER_DEG) / c;
-    *vd = v * LM_INV_M_PER_DEG;
-    const double rc = recip(c);
+    const double rc = recip(cos_deg(y));
     const double kc = LM_INV_M_PER_DEG * rc;
-    const double du_dx = fma(eta, (q[3].x - q[2].x) - (q[1].x - q[0].x), q[1].x - q[0].x) * F.g.inv_dx;
-    const double dv_dx = fma(eta, (q[3].y - q[2].y) - (q[1].y - q[0].y), q[1].y - q[0].y) * F.g.inv_dx;
-    const double du_dy = (ut - ub) * F.g.inv_dy;
-    const double dv_dy = (vt - vb) * F.g.inv_dy;
+    *ud = u * kc;
+    *vd = v * LM_INV_M_PER_DEG;
+    const double du_dx = fma(eta, (q[3].x - q[2].x) - (q[1].x - q[0].x), q[1].x - q[0].x) * V.g.inv_dx;
+    const double dv_dx = fma(eta, (q[3].y - q[2].y) - (q[1].y - q[0].y), q[1].y - q[0].y) * V.g.inv_dx;
+    const double du_dy = (ut - ub) * V.g.inv_dy;
+    const double dv_dy = (vt - vb) * V.g.inv_dy;
     // d(1/cos(lat))/dlat = (pi/180) sin(lat) / cos(lat)^2
     L->dux = kc * du_dx;
     L->duy = kc * fma(u * LM_DEG2RAD * rc, sin_deg(y), du_dy);
@@ -232,14 +283,16 @@ __device__ __forceinline__ bool adj_linearise(const AdjField& F, const AdjStage&
 // m0 - m1 - m2 + m3, (0,1) is m1 - m3, (1,0) is m2 - m3, (1,1) is m3, with
 // m = S - W for level l and m = W for level l + 1.
 //
-// LDS window: one slot of 16 floats, [comp][S, W][4 moments], per level
-// pair p in [lo, lo + npair) and cell of the block's box (ch x cw cells from
-// (j0, i0)); the write-out turns slots into node sums.
+// Slots: 16 floats, [comp][S, W][4 moments], per level pair and cell.  A
+// block's LDS window holds the slots of level pairs [lo, lo + npair) over the
+// cells of its box (ch x cw cells from (j0, i0)); a flush outside the window,
+// or from a block whose window does not fit, goes to the global slab of all
+// (nlev - 1 or 1) pairs x (ny - 1) x (nx - 1) cells.  Window and slab become
+// node gradients at the end: the block's write-out and adj_slab_kernel.
 struct GradSink {
-    float* lds;     // NULL: add to global memory
-    float* global;  // [nlev][ny][nx][2]
-    int nlev, nx;
-    int64_t level_len;
+    float* lds;   // NULL: the block's window did not fit
+    float* slab;  // [pairs][ny - 1][nx - 1][16]
+    int cx, cy;   // cells along x, y
     int lo, npair, j0, i0, ch, cw;
 };
 
@@ -260,33 +313,60 @@ __device__ __forceinline__ void cache_flush(GradCache& gc, const GradSink& sk) {
     if (gc.l < 0) return;
     const int cj = gc.cell >> 16, ci = gc.cell & 0xffff;
     const int wl = gc.l - sk.lo, wj = cj - sk.j0, wi = ci - sk.i0;
-    // the recompute's box covers every sample; the test keeps LDS writes in the window regardless
+    // the recompute's box covers every sample, so the slab takes only blocks whose window did not fit
     if (sk.lds && (unsigned)wl < (unsigned)sk.npair && (unsigned)wj < (unsigned)sk.ch && (unsigned)wi < (unsigned)sk.cw) {
         float* slot = sk.lds + ((wl * sk.ch + wj) * sk.cw + wi) * 16;
 #pragma unroll
         for (int k = 0; k < 16; ++k) atomicAdd(slot + k, gc.m[k]);
     } else {
+        float* slot = sk.slab + (((int64_t)gc.l * sk.cy + cj) * sk.cx + ci) * 16;
 #pragma unroll
-        for (int comp = 0; comp < 2; ++comp) {
-            const float* S = gc.m + comp * 8;
-            const float* W = S + 4;
-            const float lower[4] = {S[0] - W[0], S[1] - W[1], S[2] - W[2], S[3] - W[3]};
-#pragma unroll
-            for (int lev = 0; lev < 2; ++lev) {
-                const int lv = min(gc.l + lev, sk.nlev - 1);  // a steady field's one level takes both parts
-                const float* m = lev == 0 ? lower : W;
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const float v = corner_of(m, c >> 1, c & 1);
-                    if (v == 0.0f) continue;
-                    const int64_t at =
-                        ((int64_t)lv * sk.level_len + (int64_t)(cj + (c >> 1)) * sk.nx + (ci + (c & 1))) * 2 + comp;
-                    atomicAdd(sk.global + at, v);
-                }
-            }
-        }
+        for (int k = 0; k < 16; ++k) atomicAdd(slot + k, gc.m[k]);
     }
     gc.l = -1;
+}
+
+// Node value of one (u or v) component from the slots around node (nj, ni)
+// of level lv: corner (dj, di) of cell (nj - dj, ni - di), the lower part of
+// pair lv and the upper part of pair lv - 1 (all of pair 0 when steady).
+// slot(p, cj, ci) returns the slot's 8 floats of this component or NULL.
+template <typename SlotFn>
+__device__ __forceinline__ float node_from_slots(int lv, int nj, int ni, bool steady, SlotFn slot) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+        if (steady && part == 1) continue;
+        const int pr = steady ? 0 : lv - part;
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int di = 0; di < 2; ++di) {
+                const float* S = slot(pr, nj - dj, ni - di);
+                if (!S) continue;
+                const float* W = S + 4;
+                float mm[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) mm[q] = steady ? S[q] : (part == 0 ? S[q] - W[q] : W[q]);
+                acc += corner_of(mm, dj, di);
+            }
+    }
+    return acc;
+}
+
+// Adds one stage sample to the cache's moments: velocity adjoints (gu, gv)
+// already scaled to m/s, time weight w, fractional position (xi, eta).
+__device__ __forceinline__ void cache_accumulate(GradCache& gc, float w, float xi, float eta, float gu, float gv) {
+    const float b[4] = {1.0f, xi, eta, xi * eta};
+    const float g[2] = {gu, gv};
+#pragma unroll
+    for (int comp = 0; comp < 2; ++comp) {
+        const float gw = g[comp] * w;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            gc.m[comp * 8 + k] = fmaf(g[comp], b[k], gc.m[comp * 8 + k]);
+            gc.m[comp * 8 + 4 + k] = fmaf(gw, b[k], gc.m[comp * 8 + 4 + k]);
+        }
+    }
 }
 
 // Field gradient of one stage sample with adjoint (au, av) of its velocity.
@@ -299,30 +379,38 @@ __device__ __forceinline__ void cache_add(GradCache& gc, const GradSink& sk, con
 #pragma unroll
         for (int k = 0; k < 16; ++k) gc.m[k] = 0.0f;
     }
-    const float w = (float)st.w;
-    const float g[2] = {(float)au * L.kc, (float)(av * LM_INV_M_PER_DEG)};
-    const float b[4] = {1.0f, L.xi, L.eta, L.xi * L.eta};
-#pragma unroll
-    for (int comp = 0; comp < 2; ++comp) {
-        const float gw = g[comp] * w;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            gc.m[comp * 8 + k] = fmaf(g[comp], b[k], gc.m[comp * 8 + k]);
-            gc.m[comp * 8 + 4 + k] = fmaf(gw, b[k], gc.m[comp * 8 + 4 + k]);
-        }
-    }
+    cache_accumulate(gc, (float)st.w, L.xi, L.eta, (float)au * L.kc, (float)(av * LM_INV_M_PER_DEG));
 }
 
+// out[k][node] = blend of stage time k0 + k (k < count, one block row each).
 __global__ void __launch_bounds__(kAdjBlock)
-adj_forward_kernel(int64_t n, double* __restrict__ lon, double* __restrict__ lat, AdjField F, int nsteps, double hh,
-                   double h, double h6, int ckpt, double2* __restrict__ ck, uint32_t* __restrict__ oob) {
-    const int64_t i = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
+adj_blend_kernel(const float2* __restrict__ lev, int64_t level_len, int64_t lstride, const int32_t* __restrict__ lev0,
+                 const double* __restrict__ w, int k0, double2* __restrict__ out) {
+    const int k = blockIdx.y;
+    const int64_t node = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
+    if (node >= level_len) return;
+    const int s = k0 + k;
+    const double wt = w[s];
+    const float2* a = lev + (int64_t)lev0[s] * level_len + node;
+    const float2 f0 = a[0], f1 = a[lstride];
+    out[(int64_t)k * level_len + node] = make_double2(fma(wt, (double)f1.x - (double)f0.x, (double)f0.x),
+                                                      fma(wt, (double)f1.y - (double)f0.y, (double)f0.y));
+}
+
+// Steps [s_begin, s_end) of the run; the positions at the start of each
+// step s with s % ckpt == 0 go to checkpoint s / ckpt.
+__global__ void __launch_bounds__(kAdjBlock)
+adj_forward_kernel(int64_t n, double* __restrict__ lon, double* __restrict__ lat, AdjField F, int s_begin, int s_end,
+                   double hh, double h, double h6, int ckpt, double2* __restrict__ ck, uint32_t* __restrict__ oob) {
+    uint32_t bx, by;
+    xcd_block(&bx, &by);  // an XCD's blocks cover neighbouring particles: their stage-field cells share its L2
+    const int64_t i = (int64_t)bx * kAdjBlock + threadIdx.x;
     if (i >= n) return;
     double x = lon[i], y = lat[i];
     uint32_t out = 0;
-    for (int s = 0; s < nsteps; ++s) {
+    for (int s = s_begin; s < s_end; ++s) {
         if (s % ckpt == 0) ck[(int64_t)(s / ckpt) * n + i] = make_double2(x, y);
-        if (!adj_rk4(F, s, hh, h, h6, &x, &y, nullptr)) ++out;
+        if (!adj_rk4<false>(F, s, hh, h, h6, &x, &y, nullptr)) ++out;
     }
     lon[i] = x;
     lat[i] = y;
@@ -330,7 +418,7 @@ adj_forward_kernel(int64_t n, double* __restrict__ lon, double* __restrict__ lat
 }
 
 // Steps s0 .. s0 + m - 1 from the checkpoint ck: the start position of step
-// s0 + k into pos[k * n + i]; the block's cell box into box[blockIdx.x].
+// s0 + k into pos[k * n + i]; the block's cell box into box[block].
 __global__ void __launch_bounds__(kAdjBlock)
 adj_recompute_kernel(int64_t n, const double2* __restrict__ ck, AdjField F, int s0, int m, double hh, double h,
                      double h6, double2* __restrict__ pos, int4* __restrict__ box) {
@@ -340,7 +428,9 @@ adj_recompute_kernel(int64_t n, const double2* __restrict__ ck, AdjField F, int 
         sb[1] = sb[3] = -1;
     }
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
+    uint32_t bx, by;
+    xcd_block(&bx, &by);  // the backward maps blocks the same way
+    const int64_t i = (int64_t)bx * kAdjBlock + threadIdx.x;
     CellBox cb;
     cb.reset();
     if (i < n) {
@@ -348,7 +438,7 @@ adj_recompute_kernel(int64_t n, const double2* __restrict__ ck, AdjField F, int 
         double x = p.x, y = p.y;
         for (int k = 0; k < m; ++k) {
             pos[(int64_t)k * n + i] = make_double2(x, y);
-            adj_rk4(F, s0 + k, hh, h, h6, &x, &y, &cb);
+            adj_rk4<true>(F, s0 + k, hh, h, h6, &x, &y, &cb);
         }
     }
     if (cb.jmax >= 0) {
@@ -358,54 +448,57 @@ adj_recompute_kernel(int64_t n, const double2* __restrict__ ck, AdjField F, int 
         atomicMax(&sb[3], cb.imax);
     }
     __syncthreads();
-    if (threadIdx.x == 0) box[blockIdx.x] = make_int4(sb[0], sb[1], sb[2], sb[3]);
+    if (threadIdx.x == 0) box[bx] = make_int4(sb[0], sb[1], sb[2], sb[3]);
 }
 
 // Steps s0 + m - 1 down to s0, from the recomputed start positions; lambda
 // (glon, glat) goes from the end of the segment to its start.  Level pairs
 // [lo, lo + npair) are the ones the segment's stages read; lds_bytes is the
-// dynamic LDS the launch carries.
-__global__ void __launch_bounds__(kAdjBlock, 3)
-adj_backward_kernel(int64_t n, AdjField F, int nlev, int s0, int m, double hh, double h, double h6,
+// dynamic LDS the launch carries for the block's gradient window.
+__global__ void __launch_bounds__(kAdjBlock)
+adj_backward_kernel(int64_t n, LevelView V, int nlev, int s0, int m, double hh, double h, double h6,
                     const double2* __restrict__ pos, const int4* __restrict__ box, double* __restrict__ glon,
-                    double* __restrict__ glat, float* __restrict__ gfield, int lo, int npair, int lds_bytes) {
+                    double* __restrict__ glat, float* __restrict__ gfield, float* __restrict__ slab, int lo, int npair,
+                    int lds_bytes) {
     extern __shared__ float win[];
-    const int4 b = box[blockIdx.x];
+    uint32_t bx, by;
+    xcd_block(&bx, &by);  // as the recompute, whose box of logical block bx this block reads
+    const int4 b = box[bx];
     if (b.y < 0) return;  // no lane of the block completed a step inside the grid
     GradSink sk;
-    sk.global = gfield;
-    sk.nlev = nlev;
-    sk.nx = F.g.nx;
-    sk.level_len = F.level_len;
+    sk.slab = slab;
+    sk.cx = V.g.nx - 1;
+    sk.cy = V.g.ny - 1;
     sk.lo = lo;
     sk.npair = npair;
     sk.j0 = b.x;
     sk.i0 = b.z;
     sk.ch = b.y - b.x + 1;
     sk.cw = b.w - b.z + 1;
-    const int64_t wcount = (int64_t)npair * sk.ch * sk.cw * 16;
-    const bool use_lds = wcount * 4 <= (int64_t)lds_bytes;
-    sk.lds = use_lds ? win : nullptr;
-    if (use_lds) {
-        for (int t = threadIdx.x; t < (int)wcount; t += kAdjBlock) win[t] = 0.0f;
+    const int64_t wfloats = (int64_t)npair * sk.ch * sk.cw * 16;
+    const bool win_on = wfloats * 4 <= (int64_t)lds_bytes;
+    sk.lds = win_on ? win : nullptr;
+    if (win_on) {
+        for (int t = threadIdx.x; t < (int)wfloats; t += kAdjBlock) win[t] = 0.0f;
         __syncthreads();
     }
     GradCache gc;
     gc.l = -1;
-    const int64_t i = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
+    const int64_t i = (int64_t)bx * kAdjBlock + threadIdx.x;
     if (i < n) {
         double lx = glon[i], ly = glat[i];
         for (int k = m - 1; k >= 0; --k) {
             const int s = s0 + k;
             const double2 p = pos[(int64_t)k * n + i];
             const double x0 = p.x, y0 = p.y;
-            const AdjStage sa = adj_stage(F, 2 * s), sb = adj_stage(F, 2 * s + 1), sc = adj_stage(F, 2 * s + 2);
+            const AdjStage sa{2 * s, V.lev0[2 * s], V.w[2 * s]}, sb{2 * s + 1, V.lev0[2 * s + 1], V.w[2 * s + 1]},
+                sc{2 * s + 2, V.lev0[2 * s + 2], V.w[2 * s + 2]};
             Lin L1, L2, L3, L4;
             double ud, vd;
-            if (!adj_linearise(F, sa, x0, y0, &ud, &vd, &L1)) continue;
-            if (!adj_linearise(F, sb, fma(ud, hh, x0), fma(vd, hh, y0), &ud, &vd, &L2)) continue;
-            if (!adj_linearise(F, sb, fma(ud, hh, x0), fma(vd, hh, y0), &ud, &vd, &L3)) continue;
-            if (!adj_linearise(F, sc, fma(ud, h, x0), fma(vd, h, y0), &ud, &vd, &L4)) continue;
+            if (!adj_linearise(V, sa, x0, y0, &ud, &vd, &L1)) continue;
+            if (!adj_linearise(V, sb, fma(ud, hh, x0), fma(vd, hh, y0), &ud, &vd, &L2)) continue;
+            if (!adj_linearise(V, sb, fma(ud, hh, x0), fma(vd, hh, y0), &ud, &vd, &L3)) continue;
+            if (!adj_linearise(V, sc, fma(ud, h, x0), fma(vd, h, y0), &ud, &vd, &L4)) continue;
             const double a4x = h6 * lx, a4y = h6 * ly;
             double a3x = 2.0 * a4x, a3y = 2.0 * a4y;
             double a2x = a3x, a2y = a3y;
@@ -446,10 +539,8 @@ adj_backward_kernel(int64_t n, AdjField F, int nlev, int s0, int m, double hh, d
         glat[i] = ly;
     }
     cache_flush(gc, sk);
-    if (!use_lds) return;
+    if (!win_on) return;
     __syncthreads();
-    // node (lv, nj, ni) of the window gathers corner (dj, di) of the cells (nj - dj, ni - di): the lower
-    // part of pair lv and the upper part of pair lv - 1 (both parts of pair 0 for a steady field)
     const int nh = sk.ch + 1, nw = sk.cw + 1, nlw = nlev > 1 ? npair + 1 : 1;
     const int nodes = nlw * nh * nw * 2;
     for (int t = threadIdx.x; t < nodes; t += kAdjBlock) {
@@ -459,34 +550,41 @@ adj_backward_kernel(int64_t n, AdjField F, int nlev, int s0, int m, double hh, d
         r /= nw;
         const int nj = r % nh;
         const int lv = r / nh;
-        float acc = 0.0f;
-#pragma unroll
-        for (int part = 0; part < 2; ++part) {
-            const int pr = nlev > 1 ? lv - part : 0;
-            if (pr < 0 || pr >= npair) continue;
-#pragma unroll
-            for (int dj = 0; dj < 2; ++dj) {
-                const int cj = nj - dj;
-                if (cj < 0 || cj >= sk.ch) continue;
-#pragma unroll
-                for (int di = 0; di < 2; ++di) {
-                    const int ci = ni - di;
-                    if (ci < 0 || ci >= sk.cw) continue;
-                    const float* S = win + ((pr * sk.ch + cj) * sk.cw + ci) * 16 + comp * 8;
-                    const float* W = S + 4;
-                    float mm[4];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        mm[q] = nlev == 1 ? (part == 0 ? S[q] : 0.0f) : (part == 0 ? S[q] - W[q] : W[q]);
-                    acc += corner_of(mm, dj, di);
-                }
-            }
-        }
+        const float acc = node_from_slots(lv, nj, ni, nlev == 1, [&](int p, int cj, int ci) -> const float* {
+            if (p < 0 || p >= npair || cj < 0 || cj >= sk.ch || ci < 0 || ci >= sk.cw) return nullptr;
+            return sk.lds + ((p * sk.ch + cj) * sk.cw + ci) * 16 + comp * 8;
+        });
         if (acc != 0.0f)
-            atomicAdd(gfield + ((int64_t)(lo + lv) * F.level_len + (int64_t)(sk.j0 + nj) * F.g.nx + (sk.i0 + ni)) * 2 +
+            atomicAdd(gfield + ((int64_t)(lo + lv) * V.level_len + (int64_t)(sk.j0 + nj) * V.g.nx + (sk.i0 + ni)) * 2 +
                           comp,
                       acc);
     }
+}
+
+// Clears the global slab (a kernel rather than a memset node, so a captured
+// backward replays correctly).
+__global__ void __launch_bounds__(kAdjBlock) adj_zero_kernel(float4* __restrict__ p, int64_t n4) {
+    const int64_t t = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
+    if (t < n4) p[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Adds the global slab's node values into gfield: one thread per (level,
+// node, component).
+__global__ void __launch_bounds__(kAdjBlock)
+adj_slab_kernel(const float* __restrict__ slab, int nlev, lm_grid g, float* __restrict__ gfield) {
+    const int64_t L = (int64_t)g.nx * g.ny;
+    const int64_t t = (int64_t)blockIdx.x * kAdjBlock + threadIdx.x;
+    if (t >= (int64_t)nlev * L * 2) return;
+    const int comp = (int)(t & 1);
+    const int64_t node = (t >> 1) % L;
+    const int lv = (int)((t >> 1) / L);
+    const int ni = (int)(node % g.nx), nj = (int)(node / g.nx);
+    const int cx = g.nx - 1, cy = g.ny - 1, npair = nlev > 1 ? nlev - 1 : 1;
+    const float acc = node_from_slots(lv, nj, ni, nlev == 1, [&](int p, int cj, int ci) -> const float* {
+        if (p < 0 || p >= npair || cj < 0 || cj >= cy || ci < 0 || ci >= cx) return nullptr;
+        return slab + (((int64_t)p * cy + cj) * cx + ci) * 16 + comp * 8;
+    });
+    if (acc != 0.0f) gfield[t] += acc;
 }
 
 }  // namespace lm
@@ -518,42 +616,78 @@ int adj_check(int64_t n, int32_t nlev, const lm_grid* grid, const int32_t* lev0_
     return LM_OK;
 }
 
-lm::AdjField adj_field(const float* field, int32_t nlev, const lm_grid* grid, const int32_t* lev0, const double* w) {
-    lm::AdjField F;
-    F.f = reinterpret_cast<const float2*>(field);
-    F.lev0 = lev0;
-    F.w = w;
-    F.level_len = (int64_t)grid->nx * grid->ny;
-    F.lstride = nlev > 1 ? F.level_len : 0;
-    F.g = *grid;
-    return F;
+int64_t level_len_of(const lm_grid* g) { return (int64_t)g->nx * g->ny; }
+
+// Global moment slab: 16 floats per level pair and cell.
+int64_t slab_bytes(const lm_grid* g, int32_t nlev) {
+    return (int64_t)(nlev > 1 ? nlev - 1 : 1) * (g->nx - 1) * (g->ny - 1) * 64;
+}
+
+// Stage fields for stage times [k0, k0 + count) into buf (one field when
+// steady); returns the view the kernels read.
+int adj_blend(const float* field, int32_t nlev, const lm_grid* grid, const int32_t* lev0, const double* w, int k0,
+              int count, double2* buf, hipStream_t s, lm::AdjField* F) {
+    const int64_t L = level_len_of(grid);
+    const bool steady = nlev == 1;
+    const int nblend = steady ? 1 : count;
+    for (int a = 0; a < nblend; a += 65535) {
+        const int c = nblend - a < 65535 ? nblend - a : 65535;
+        LM_LAUNCH(lm::K_ADJ_BLEND, lm::adj_blend_kernel, dim3(lm::grid_for(L, lm::kAdjBlock), (unsigned)c),
+                  dim3(lm::kAdjBlock), 0, s, reinterpret_cast<const float2*>(field), L, steady ? (int64_t)0 : L, lev0,
+                  w, k0 + a, buf + (int64_t)a * L);
+        LM_LAUNCH_CHECK();
+    }
+    F->st = buf;
+    F->sstride = steady ? 0 : L;
+    F->k0 = k0;
+    F->lev0 = lev0;
+    F->w = w;
+    F->level_len = L;
+    F->g = *grid;
+    return LM_OK;
 }
 
 }  // namespace
 
 extern "C" {
 
+int64_t lm_adjoint_scratch_bytes(int64_t n, int32_t ckpt, const lm_grid* grid, int32_t nlev) {
+    if (n < 0 || ckpt < 1 || grid == nullptr || grid->nx < 2 || grid->ny < 2 || nlev < 1) return -1;
+    const int64_t blocks = (n + lm::kAdjBlock - 1) / lm::kAdjBlock;
+    const int64_t stages = nlev == 1 ? 1 : 2 * (int64_t)ckpt + 1;
+    return (int64_t)ckpt * n * 16 + blocks * 16 + stages * level_len_of(grid) * 16 + slab_bytes(grid, nlev);
+}
+
 int lm_adjoint_forward(int64_t n, double* lon, double* lat, const float* field, int32_t nlev, const lm_grid* grid,
                        const int32_t* lev0, const double* w, const int32_t* lev0_host, int32_t nsteps, double dt,
-                       int32_t ckpt, double* checkpoints, uint32_t* oob_count, void* stream) {
+                       int32_t ckpt, double* checkpoints, void* scratch, int64_t scratch_bytes, uint32_t* oob_count,
+                       void* stream) {
     int rc = adj_check(n, nlev, grid, lev0_host, nsteps, ckpt);
     if (rc) return rc;
     if (n == 0 || nsteps == 0) return LM_OK;
-    LM_REQUIRE(lon && lat && field && lev0 && w && checkpoints && oob_count, LM_ERR_ARG, "NULL pointer");
-    LM_REQUIRE(((uintptr_t)field & 7) == 0 && ((uintptr_t)checkpoints & 15) == 0, LM_ERR_ARG,
-               "field must be 8-byte and checkpoints 16-byte aligned");
-    const lm::AdjField F = adj_field(field, nlev, grid, lev0, w);
-    LM_LAUNCH(lm::K_ADJ_FORWARD, lm::adj_forward_kernel, dim3(lm::grid_for(n, lm::kAdjBlock)), dim3(lm::kAdjBlock),
-              0, (hipStream_t)stream, n, lon, lat, F, (int)nsteps, 0.5 * dt, dt, dt / 6.0, (int)ckpt,
-              reinterpret_cast<double2*>(checkpoints), oob_count);
-    LM_LAUNCH_CHECK();
+    LM_REQUIRE(lon && lat && field && lev0 && w && checkpoints && scratch && oob_count, LM_ERR_ARG, "NULL pointer");
+    LM_REQUIRE(((uintptr_t)field & 7) == 0 && ((uintptr_t)checkpoints & 15) == 0 && ((uintptr_t)scratch & 15) == 0,
+               LM_ERR_ARG, "field must be 8-byte, checkpoints and scratch 16-byte aligned");
+    const int64_t fbytes = level_len_of(grid) * 16;
+    // steps per chunk: as many as the scratch holds the stage fields of
+    int64_t chunk = nlev == 1 ? nsteps : (scratch_bytes / fbytes - 1) / 2;
+    LM_REQUIRE(chunk >= 1 && (nlev > 1 || scratch_bytes >= fbytes), LM_ERR_ARG,
+               "scratch holds %lld bytes; one step needs %lld", (long long)scratch_bytes,
+               (long long)(nlev == 1 ? fbytes : 3 * fbytes));
+    chunk = chunk > nsteps ? nsteps : chunk;
+    hipStream_t s = (hipStream_t)stream;
+    double2* buf = reinterpret_cast<double2*>(scratch);
+    for (int a = 0; a < nsteps; a += (int)chunk) {
+        const int b = nsteps - a < chunk ? nsteps : a + (int)chunk;
+        lm::AdjField F;
+        rc = adj_blend(field, nlev, grid, lev0, w, 2 * a, 2 * (b - a) + 1, buf, s, &F);
+        if (rc) return rc;
+        LM_LAUNCH(lm::K_ADJ_FORWARD, lm::adj_forward_kernel, dim3(lm::grid_for(n, lm::kAdjBlock)),
+                  dim3(lm::kAdjBlock), 0, s, n, lon, lat, F, a, b, 0.5 * dt, dt, dt / 6.0, (int)ckpt,
+                  reinterpret_cast<double2*>(checkpoints), oob_count);
+        LM_LAUNCH_CHECK();
+    }
     return LM_OK;
-}
-
-int64_t lm_adjoint_scratch_bytes(int64_t n, int32_t ckpt) {
-    if (n < 0 || ckpt < 1) return -1;
-    const int64_t blocks = (n + lm::kAdjBlock - 1) / lm::kAdjBlock;
-    return (int64_t)ckpt * n * 16 + blocks * 16;
 }
 
 int lm_adjoint_backward(int64_t n, const double* checkpoints, const float* field, int32_t nlev, const lm_grid* grid,
@@ -566,16 +700,28 @@ int lm_adjoint_backward(int64_t n, const double* checkpoints, const float* field
     LM_REQUIRE(checkpoints && field && lev0 && w && scratch && glon && glat && gfield, LM_ERR_ARG, "NULL pointer");
     LM_REQUIRE(((uintptr_t)field & 7) == 0 && ((uintptr_t)checkpoints & 15) == 0 && ((uintptr_t)scratch & 15) == 0,
                LM_ERR_ARG, "field must be 8-byte, checkpoints and scratch 16-byte aligned");
-    LM_REQUIRE(scratch_bytes >= lm_adjoint_scratch_bytes(n, ckpt), LM_ERR_ARG,
-               "scratch holds %lld bytes, %lld needed", (long long)scratch_bytes,
-               (long long)lm_adjoint_scratch_bytes(n, ckpt));
+    const int64_t need = lm_adjoint_scratch_bytes(n, ckpt, grid, nlev);
+    LM_REQUIRE(scratch_bytes >= need, LM_ERR_ARG, "scratch holds %lld bytes, %lld needed", (long long)scratch_bytes,
+               (long long)need);
     hipStream_t s = (hipStream_t)stream;
-    const lm::AdjField F = adj_field(field, nlev, grid, lev0, w);
     const unsigned nb = lm::grid_for(n, lm::kAdjBlock);
     double2* pos = reinterpret_cast<double2*>(scratch);
     int4* box = reinterpret_cast<int4*>(pos + (int64_t)ckpt * n);
+    double2* stage = reinterpret_cast<double2*>(box + nb);
+    float* slab = reinterpret_cast<float*>(stage + (nlev == 1 ? 1 : 2 * (int64_t)ckpt + 1) * level_len_of(grid));
     const double2* ck = reinterpret_cast<const double2*>(checkpoints);
+    const int64_t slab4 = slab_bytes(grid, nlev) / 16;
+    LM_LAUNCH(lm::K_ADJ_SLAB, lm::adj_zero_kernel, dim3(lm::grid_for(slab4, lm::kAdjBlock)), dim3(lm::kAdjBlock), 0, s,
+              reinterpret_cast<float4*>(slab), slab4);
+    LM_LAUNCH_CHECK();
     const double hh = 0.5 * dt, h6 = dt / 6.0;
+    lm::LevelView V{};
+    V.glev = reinterpret_cast<const float2*>(field);
+    V.lev0 = lev0;
+    V.w = w;
+    V.level_len = level_len_of(grid);
+    V.gstride = nlev > 1 ? V.level_len : 0;
+    V.g = *grid;
     const int nseg = (nsteps + ckpt - 1) / ckpt;
     for (int seg = nseg - 1; seg >= 0; --seg) {
         const int s0 = seg * ckpt, m = nsteps - s0 < ckpt ? nsteps - s0 : ckpt;
@@ -585,14 +731,20 @@ int lm_adjoint_backward(int64_t n, const double* checkpoints, const float* field
             lo = l < lo ? l : lo;
             hi = l > hi ? l : hi;
         }
+        lm::AdjField F;
+        rc = adj_blend(field, nlev, grid, lev0, w, 2 * s0, 2 * m + 1, stage, s, &F);
+        if (rc) return rc;
         LM_LAUNCH(lm::K_ADJ_RECOMPUTE, lm::adj_recompute_kernel, dim3(nb), dim3(lm::kAdjBlock), 0, s, n,
                   ck + (int64_t)seg * n, F, s0, m, hh, dt, h6, pos, box);
         LM_LAUNCH_CHECK();
         LM_LAUNCH(lm::K_ADJ_BACKWARD, lm::adj_backward_kernel, dim3(nb), dim3(lm::kAdjBlock),
-                  (unsigned)kAdjWindowBytes, s, n, F, (int)nlev, s0, m, hh, dt, h6, pos, box, glon, glat, gfield, lo,
-                  hi - lo + 1, kAdjWindowBytes);
+                  (unsigned)kAdjWindowBytes, s, n, V, (int)nlev, s0, m, hh, dt, h6, pos, box, glon, glat, gfield, slab,
+                  lo, hi - lo + 1, kAdjWindowBytes);
         LM_LAUNCH_CHECK();
     }
+    LM_LAUNCH(lm::K_ADJ_SLAB, lm::adj_slab_kernel, dim3(lm::grid_for((int64_t)nlev * level_len_of(grid) * 2, lm::kAdjBlock)),
+              dim3(lm::kAdjBlock), 0, s, slab, (int)nlev, *grid, gfield);
+    LM_LAUNCH_CHECK();
     return LM_OK;
 }
 

@@ -79,6 +79,8 @@ enum KernelId {
     K_ADJ_FORWARD,
     K_ADJ_RECOMPUTE,
     K_ADJ_BACKWARD,
+    K_ADJ_BLEND,
+    K_ADJ_SLAB,
     K_NUM_KERNELS
 };
 bool timing_on();

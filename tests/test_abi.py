@@ -59,7 +59,7 @@ def test_adjoint_argument_errors(liblm):
     g = LmGrid(5, 5, 0.0, 0.0, 4.0, 4.0)
     lev = np.array([0, 0, 1, 1, 2], dtype=np.int32)  # 2 steps; level 2 + 1 does not exist with 3 levels
     rc = liblm.lm_adjoint_forward(10, None, None, None, 3, ctypes.byref(g), None, None,
-                                  lev.ctypes.data_as(ctypes.c_void_p), 2, 3600.0, 16, None, None, None)
+                                  lev.ctypes.data_as(ctypes.c_void_p), 2, 3600.0, 16, None, None, 0, None, None)
     assert rc != 0 and b"level 2" in liblm.lm_last_error()
     rc = liblm.lm_adjoint_backward(10, None, None, 3, ctypes.byref(g), None, None,
                                    lev.ctypes.data_as(ctypes.c_void_p), 2, 3600.0, 0, None, 0, None, None, None,
@@ -70,6 +70,10 @@ def test_adjoint_argument_errors(liblm):
                                    ok.ctypes.data_as(ctypes.c_void_p), 2, 3600.0, 16, None, 0, None, None, None,
                                    None)
     assert rc != 0 and b"NULL" in liblm.lm_last_error()
-    # scratch: 16 start positions of 16 bytes per particle, plus one 16-byte box per 256-particle block
-    assert liblm.lm_adjoint_scratch_bytes(1000, 16) == 16 * 1000 * 16 + 4 * 16
-    assert liblm.lm_adjoint_scratch_bytes(-1, 16) == -1
+    # scratch: 16 start positions of 16 bytes per particle, one 16-byte box per 256-particle block,
+    # 33 float64 stage fields of 5x5 (u, v) nodes (one for a steady field) and a slab of 16 floats per
+    # level pair (2 of 3 levels; 1 when steady) and 4x4 cell
+    base = 16 * 1000 * 16 + 4 * 16
+    assert liblm.lm_adjoint_scratch_bytes(1000, 16, ctypes.byref(g), 3) == base + 33 * 25 * 16 + 2 * 16 * 64
+    assert liblm.lm_adjoint_scratch_bytes(1000, 16, ctypes.byref(g), 1) == base + 25 * 16 + 1 * 16 * 64
+    assert liblm.lm_adjoint_scratch_bytes(-1, 16, ctypes.byref(g), 3) == -1

@@ -208,8 +208,9 @@ def test_checkpoint_interval_does_not_change_gradients(gpu, case10k):
     base = gpu_grads(f, lon, lat, tlon, tlat, w.dt, w.steps, ckpt=16)
     for ckpt in (1, 7, 24, 64):
         g = gpu_grads(f, lon, lat, tlon, tlat, w.dt, w.steps, ckpt=ckpt)
-        # the position adjoint is the same arithmetic per particle whatever the segmentation
-        assert np.array_equal(g["gx"], base["gx"]) and np.array_equal(g["gy"], base["gy"]), ckpt
+        # the recompute restarts from the forward's exact positions at each checkpoint and runs within
+        # rounding of them in between, so segmentation moves the position adjoint by rounding only
+        assert normwise(g["gx"], base["gx"]) <= 1e-12 and normwise(g["gy"], base["gy"]) <= 1e-12, ckpt
         # field gradients differ only in float32 summation order
         assert normwise(g["gu"], base["gu"]) <= 1e-5 and normwise(g["gv"], base["gv"]) <= 1e-5, ckpt
 
