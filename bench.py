@@ -14,7 +14,10 @@ particles over 240 hourly steps through 20 float32 field levels
 (lagrangian_microbes_amd.adjoint): the forward with checkpoints every 16
 steps, and loss.backward() of the mean squared great-circle distance to
 targets from a second seeded field (recompute plus reverse sweep);
-``adjoint_forward_ratio`` is backward time over forward time.
+``adjoint_forward_ratio`` is backward time over forward time.  The
+``fp32_path`` object runs the FP32 path's workload (4M particles, 2,880
+half-hour steps, 1/12-degree field) in FP32 and FP64 from identical inputs
+and reports both rates and the distance between their positions.
 """
 import argparse
 import json
@@ -46,6 +49,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint forward/backward timing")
     ap.add_argument("--adjoint-reps", type=int, default=3)
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-path run at 4M particles")
+    ap.add_argument("--fp32-steps", type=int, default=2880)
     ap.add_argument("--cpu-steps", type=int, default=48)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel on BYTES bytes 10 times (byte-counter calibration)")
@@ -179,6 +184,49 @@ def cpu_baseline(w, runner, steps):
                    f"per-pair Python RPS loop {t_rps:.2f}s, single thread, {platform.processor() or 'cpu'}"),
         "parity_with_gpu": parity,
     }
+
+
+def fp32_leg(steps):
+    """The FP32 path on its workload (4M particles, 1/12-degree 21-level field,
+    half-hour steps): the fused run in FP32 and in FP64 from identical inputs,
+    timed, with the distance between their positions after 1 day, 10 days and
+    the whole run."""
+    import torch
+    from lagrangian_microbes_amd import configs
+    from lagrangian_microbes_amd.simulator import FusedRunner
+
+    w = configs.rps_4m_60day()
+    f = w.velocity_field()
+    lon, lat = w.particles()
+    marks = [m for m in (48, 480) if m < steps] + [steps]
+    out, snaps = {}, {}
+    for prec in ("f32", "f64"):
+        r = FusedRunner(lon, lat, w.species(), f, w.radius, w.params, w.key, w.dt, chunk=48, precision=prec)
+        r.run(48)
+        torch.cuda.synchronize()
+        r.reset()
+        elapsed, done, snaps[prec] = 0.0, 0, []
+        for m in marks:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r.run(m - done)
+            torch.cuda.synchronize()
+            elapsed += time.perf_counter() - t0
+            done = m
+            snaps[prec].append((r.lon.cpu().numpy(), r.lat.cpu().numpy()))
+        r.check_domain()
+        out[prec] = {"ms_per_step": round(1e3 * elapsed / steps, 4),
+                     "particle_steps_per_s": w.n * steps / elapsed}
+        log(f"[bench] {w.name} {prec}: {out[prec]['ms_per_step']} ms/step")
+        del r
+        torch.cuda.empty_cache()
+    err = {}
+    for m, a, b in zip(marks, snaps["f32"], snaps["f64"]):
+        d = np.hypot(a[0] - b[0], a[1] - b[1])
+        err[f"after_{m}_steps"] = {"median_deg": float(np.median(d)), "p99_deg": float(np.percentile(d, 99)),
+                                   "max_deg": float(d.max())}
+    return {"workload": w.name, "particles": w.n, "steps": steps, "dt_s": w.dt, "field": w.describe()["field"],
+            "f32": out["f32"], "f64": out["f64"], "f32_vs_f64_position_error": err}
 
 
 def adjoint_leg(reps):
@@ -328,9 +376,11 @@ def main():
         result["cpu_baseline"] = cpu_baseline(w, runner, args.cpu_steps)
     else:
         result["cpu_baseline"] = None
+    del runner
+    torch.cuda.empty_cache()
+    if not args.no_fp32:
+        result["fp32_path"] = fp32_leg(args.fp32_steps)
     if not args.no_adjoint:
-        del runner
-        torch.cuda.empty_cache()
         result["adjoint"] = adjoint_leg(args.adjoint_reps)
         result["adjoint_forward_ratio"] = result["adjoint"]["ratio"]
     print(json.dumps(result), flush=True)

@@ -85,6 +85,24 @@ int lm_advect_rk4_f64(int64_t n, double* lon, double* lat,
                       double walk_amp, uint64_t key0, uint64_t key1, uint64_t step0,
                       uint32_t* oob_count, void* stream);
 
+/* FP32 path: the same blend on float32 levels (fmaf((float)w, f1 - f0, f0)),
+ * for lm_advect_rk4_f32.  Same call site as lm_field_blend_f64. */
+int lm_field_blend_f32(const float* levels, int64_t level_len, const int32_t* lev0,
+                       const double* w, int32_t nstage, float* out, void* stream);
+
+/* FP32 path of lm_advect_rk4_f64: same arguments and semantics with float32
+ * stage fields.  Interpolation, the m/s -> deg/s conversion (cos(lat) as a
+ * float32 polynomial) and the RK4 stage velocities are float32; positions,
+ * stage positions, the cell lookup and the accumulation
+ * x += h/6 (k1 + 2 k2 + 2 k3 + k4) are float64.  Replaces the same call as
+ * lm_advect_rk4_f64 (particle_advecter.py:222-242) for large runs. */
+int lm_advect_rk4_f32(int64_t n, double* lon, double* lat,
+                      const float* stage_fields, int64_t stage_stride, const lm_grid* grid,
+                      int32_t nsteps, double dt,
+                      float* rec_lon, float* rec_lat, int64_t rec_stride,
+                      double walk_amp, uint64_t key0, uint64_t key1, uint64_t step0,
+                      uint32_t* oob_count, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Fixed-radius pair search                                                 */
 /* ------------------------------------------------------------------------ */

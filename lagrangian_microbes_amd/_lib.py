@@ -30,6 +30,9 @@ _SIGNATURES = {
     "lm_field_blend_f64": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_i32, c_vp, c_vp]),
     "lm_advect_rk4_f64": (ctypes.c_int, [c_i64, c_vp, c_vp, c_vp, c_i64, ctypes.POINTER(LmGrid), c_i32, c_dbl,
                                          c_vp, c_vp, c_i64, c_dbl, c_u64, c_u64, c_u64, c_vp, c_vp]),
+    "lm_field_blend_f32": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_i32, c_vp, c_vp]),
+    "lm_advect_rk4_f32": (ctypes.c_int, [c_i64, c_vp, c_vp, c_vp, c_i64, ctypes.POINTER(LmGrid), c_i32, c_dbl,
+                                         c_vp, c_vp, c_i64, c_dbl, c_u64, c_u64, c_u64, c_vp, c_vp]),
     "lm_pairs_ws_create": (ctypes.c_int, [c_i64, c_i32, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl,
                                           ctypes.POINTER(c_vp)]),
     "lm_pairs_ws_destroy": (None, [c_vp]),

@@ -81,6 +81,8 @@ enum KernelId {
     K_ADJ_BACKWARD,
     K_ADJ_BLEND,
     K_ADJ_SLAB,
+    K_ADVECT_F32,
+    K_BLEND_F32,
     K_NUM_KERNELS
 };
 bool timing_on();

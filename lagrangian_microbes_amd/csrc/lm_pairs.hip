@@ -460,6 +460,10 @@ int lm_pairs_ws_create(int64_t n_max, int32_t steps_max, double lon_min, double 
     uint32_t T = next_pow2_u32(want);
     if ((uint64_t)T >= cells) T = next_pow2_u32(cells);
     if (T < 1024) T = 1024;
+    // the binning splits a bucket key into 12 bits counted in LDS and one radix pass of at most
+    // kMaxDigitBits: at most 2^22 buckets (about one per particle at 4M particles)
+    const uint32_t t_max = 1u << (12 + lm::kMaxDigitBits);
+    if (T > t_max) T = t_max;
     g.T = T;
     g.mask = T - 1;
     w->tab_stride = lm::scan_row_stride((int64_t)T);

@@ -228,16 +228,18 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st) {
 // stable -- a run keeps index order, which the pair order relies on: 64
 // particles at a time in order, a lane's rank among equal roots from one
 // ballot per root bit, a running count per root in LDS.  Loads go kGroupRounds
-// rounds at a time.
+// rounds at a time.  FB fine bits per block: 9 up to 2^19 particles (one
+// radix pass of at most 10 bits covers the rest), 12 up to 2^22.
 constexpr uint32_t kGroupFineBits = 9;
-constexpr uint32_t kGroupFine = 1u << kGroupFineBits;
+constexpr uint32_t kGroupFineBitsWide = 12;
 constexpr uint32_t kGroupRounds = 8;
 
+template <uint32_t FB>
 __global__ void __launch_bounds__(64)
 rps_group_kernel(int64_t n, RpsArrays A, RpsStrides st, const uint32_t* __restrict__ kin,
                  const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                  const uint32_t* __restrict__ dstart, int64_t drow, int64_t dstride, uint32_t fine) {
-    __shared__ uint32_t cnt[kGroupFine];
+    __shared__ uint32_t cnt[1u << FB];
     const uint32_t d = blockIdx.x, nsuper = gridDim.x, lane = threadIdx.x;
     const int64_t s = blockIdx.y, ro = s * st.row;
     const uint32_t nn = (uint32_t)n;
@@ -310,7 +312,7 @@ rps_group_kernel(int64_t n, RpsArrays A, RpsStrides st, const uint32_t* __restri
             const uint32_t f = k[r] & fmask;
             unsigned long long m = __ballot(valid);
 #pragma unroll
-            for (uint32_t bit = 0; bit < kGroupFineBits; ++bit) {
+            for (uint32_t bit = 0; bit < FB; ++bit) {
                 const bool on = (f >> bit) & 1u;
                 const unsigned long long bm = __ballot(valid && on);
                 m &= on ? bm : ~bm;
@@ -989,8 +991,12 @@ int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair
         // grouping by root in two levels: a radix pass on the root's high
         // bits, then a block per super-bucket counts its roots in LDS
         const int kb = key_bits_for(n);
-        const int fine_bits = kb < (int)lm::kGroupFineBits ? kb : (int)lm::kGroupFineBits;
+        const int fb = kb - (int)lm::kGroupFineBits > lm::kMaxDigitBits ? (int)lm::kGroupFineBitsWide
+                                                                          : (int)lm::kGroupFineBits;
+        const int fine_bits = kb < fb ? kb : fb;
         const int super_bits = kb - fine_bits;
+        LM_REQUIRE(super_bits <= lm::kMaxDigitBits, LM_ERR_ARG, "at most 2^%d particles",
+                   (int)lm::kGroupFineBitsWide + lm::kMaxDigitBits);
         uint32_t* pk[2] = {A.pkey, A.ptk};
         uint32_t* pv[2] = {A.pval, A.ptv};
         const uint32_t* dstart = nullptr;
@@ -1002,9 +1008,15 @@ int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair
                 return rc;
             in = 1;
         }
-        LM_LAUNCH(lm::K_RPS_GROUP, lm::rps_group_kernel, dim3(1u << super_bits, S), dim3(64), 0, s, n, A, sp,
-                  (const uint32_t*)pk[in], (const uint32_t*)pv[in], pk[1 - in], pv[1 - in], dstart, ws->sort.row,
-                  dstride, 1u << fine_bits);
+        if (fb == (int)lm::kGroupFineBits) {
+            LM_LAUNCH(lm::K_RPS_GROUP, lm::rps_group_kernel<lm::kGroupFineBits>, dim3(1u << super_bits, S), dim3(64),
+                      0, s, n, A, sp, (const uint32_t*)pk[in], (const uint32_t*)pv[in], pk[1 - in], pv[1 - in],
+                      dstart, ws->sort.row, dstride, 1u << fine_bits);
+        } else {
+            LM_LAUNCH(lm::K_RPS_GROUP, lm::rps_group_kernel<lm::kGroupFineBitsWide>, dim3(1u << super_bits, S),
+                      dim3(64), 0, s, n, A, sp, (const uint32_t*)pk[in], (const uint32_t*)pv[in], pk[1 - in],
+                      pv[1 - in], dstart, ws->sort.row, dstride, 1u << fine_bits);
+        }
         LM_LAUNCH_CHECK();
         A.spk = pk[1 - in];
         A.ids = pv[1 - in];

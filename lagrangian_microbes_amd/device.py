@@ -22,19 +22,25 @@ def lib():
 # velocity field on the device
 # ---------------------------------------------------------------------------
 class DeviceField:
-    """A VelocityField's levels on the GPU as interleaved (u, v) float64.
+    """A VelocityField's levels on the GPU as interleaved (u, v), float64 or
+    (precision="f32") float32.
 
     ``stage_fields(t0_rel, dt, nsteps)`` returns the 2*nsteps+1 node-wise
     time-blended fields a run of RK4 steps samples (t0, t0+dt/2, ..., t0+n dt),
-    or the single level of a steady field, with the stride to pass to
-    ``advect_rk4_f64``.
+    or the single level of a steady field, with the stride to pass to the
+    advection; ``advect(...)`` blends and advects with the kernel of the
+    field's precision.
     """
 
-    def __init__(self, field, device=None):
+    def __init__(self, field, device=None, precision="f64"):
+        if precision not in ("f64", "f32"):
+            raise ValueError(f"precision must be 'f64' or 'f32' (got {precision!r})")
         self.field = field
+        self.precision = precision
+        self.dtype = torch.float64 if precision == "f64" else torch.float32
         self.device = torch.device(device or "cuda")
         inter = field.interleaved()  # [nt, ny, nx, 2] float64
-        self.levels = torch.from_numpy(np.ascontiguousarray(inter)).to(self.device)
+        self.levels = torch.from_numpy(np.ascontiguousarray(inter)).to(self.device, self.dtype)
         self.level_len = int(np.prod(inter.shape[1:]))
         self.grid = LmGrid(field.nx, field.ny, field.x0, field.y0, 1.0 / field.dx, 1.0 / field.dy)
         self._stage_buf = None
@@ -50,22 +56,41 @@ class DeviceField:
         ns = lev0.size
         need = ns * self.level_len
         if self._stage_buf is None or self._stage_buf.numel() < need:
-            self._stage_buf = torch.empty(need, dtype=torch.float64, device=self.device)
+            self._stage_buf = torch.empty(need, dtype=self.dtype, device=self.device)
         out = self._stage_buf[:need]
         lev0_d = torch.from_numpy(lev0.astype(np.int32)).to(self.device)
         w_d = torch.from_numpy(w.astype(np.float64)).to(self.device)
-        field_blend_f64(self.levels, self.level_len, lev0_d, w_d, out, lev0_host=lev0)
+        blend = field_blend_f64 if self.precision == "f64" else field_blend_f32
+        blend(self.levels, self.level_len, lev0_d, w_d, out, lev0_host=lev0)
         return out, self.level_len
+
+    def advect(self, lon, lat, t0_rel, dt, nsteps, rec_lon=None, rec_lat=None, rec_stride=None, walk_amp=0.0,
+               key=(0, 0), step0=0, oob=None):
+        """nsteps RK4 steps from t0_rel (seconds on the field's time axis)."""
+        stage, stride = self.stage_fields(t0_rel, dt, nsteps)
+        run = advect_rk4_f64 if self.precision == "f64" else advect_rk4_f32
+        return run(lon, lat, stage, stride, self.grid, nsteps, dt, rec_lon, rec_lat, rec_stride, walk_amp, key,
+                   step0, oob)
 
 
 def field_blend_f64(levels, level_len, lev0, w, out, lev0_host=None):
     """Node-wise time blend (lm_field_blend_f64); lev0_host, if given, is the
     host copy of lev0 used for the range check (saves a device read-back)."""
-    require(levels, torch.float64, name="levels")
+    _field_blend(levels, level_len, lev0, w, out, lev0_host, torch.float64, lib().lm_field_blend_f64)
+
+
+def field_blend_f32(levels, level_len, lev0, w, out, lev0_host=None):
+    """The float32 blend of the FP32 path (lm_field_blend_f32)."""
+    _field_blend(levels, level_len, lev0, w, out, lev0_host, torch.float32, lib().lm_field_blend_f32)
+
+
+def _field_blend(levels, level_len, lev0, w, out, lev0_host, dtype, entry):
+    require(levels, dtype, name="levels")
     require(lev0, torch.int32, name="lev0")
     require(w, torch.float64, numel=lev0.numel(), name="w")
     nstage = lev0.numel()
-    require(out, torch.float64, min_numel=nstage * level_len, name="out")
+    require(out, dtype, min_numel=nstage * level_len, name="out")
+    esize = out.element_size()
     nt = levels.numel() // level_len
     l0 = np.asarray(lev0_host) if lev0_host is not None else lev0.cpu().numpy()
     if l0.size != nstage:
@@ -74,18 +99,30 @@ def field_blend_f64(levels, level_len, lev0, w, out, lev0_host=None):
         raise ValueError("lev0 out of range of the field's time levels")
     for a in range(0, nstage, 65535):
         b = min(nstage, a + 65535)
-        check(lib().lm_field_blend_f64(ptr(levels), level_len, c_vp(lev0.data_ptr() + 4 * a),
-                                       c_vp(w.data_ptr() + 8 * a), b - a,
-                                       c_vp(out.data_ptr() + 8 * a * level_len), stream_ptr()))
+        check(entry(ptr(levels), level_len, c_vp(lev0.data_ptr() + 4 * a), c_vp(w.data_ptr() + 8 * a), b - a,
+                    c_vp(out.data_ptr() + esize * a * level_len), stream_ptr()))
 
 
 def advect_rk4_f64(lon, lat, stage_fields, stage_stride, grid, nsteps, dt, rec_lon=None, rec_lat=None,
                    rec_stride=None, walk_amp=0.0, key=(0, 0), step0=0, oob=None):
     """RK4 steps on device lon/lat (float64, in place); see lm_advect_rk4_f64."""
+    return _advect(lon, lat, stage_fields, stage_stride, grid, nsteps, dt, rec_lon, rec_lat, rec_stride, walk_amp,
+                   key, step0, oob, torch.float64, lib().lm_advect_rk4_f64)
+
+
+def advect_rk4_f32(lon, lat, stage_fields, stage_stride, grid, nsteps, dt, rec_lon=None, rec_lat=None,
+                   rec_stride=None, walk_amp=0.0, key=(0, 0), step0=0, oob=None):
+    """The FP32 path (lm_advect_rk4_f32): float32 stage fields, float64 positions."""
+    return _advect(lon, lat, stage_fields, stage_stride, grid, nsteps, dt, rec_lon, rec_lat, rec_stride, walk_amp,
+                   key, step0, oob, torch.float32, lib().lm_advect_rk4_f32)
+
+
+def _advect(lon, lat, stage_fields, stage_stride, grid, nsteps, dt, rec_lon, rec_lat, rec_stride, walk_amp, key,
+            step0, oob, dtype, entry):
     n = lon.numel()
     require(lon, torch.float64, name="lon")
     require(lat, torch.float64, numel=n, name="lat")
-    require(stage_fields, torch.float64, name="stage_fields")
+    require(stage_fields, dtype, name="stage_fields")
     flen = 2 * grid.nx * grid.ny
     if stage_stride == 0:
         if stage_fields.numel() < flen:
@@ -101,10 +138,9 @@ def advect_rk4_f64(lon, lat, stage_fields, stage_stride, grid, nsteps, dt, rec_l
         require(rec_lat, torch.float32, name="rec_lat", min_numel=(nsteps - 1) * rec_stride + n)
     else:
         rec_stride = 0
-    check(lib().lm_advect_rk4_f64(n, ptr(lon), ptr(lat), ptr(stage_fields), int(stage_stride),
-                                  ctypes.byref(grid), int(nsteps), float(dt), ptr(rec_lon), ptr(rec_lat),
-                                  int(rec_stride), float(walk_amp), int(key[0]), int(key[1]), int(step0),
-                                  ptr(oob), stream_ptr()))
+    check(entry(n, ptr(lon), ptr(lat), ptr(stage_fields), int(stage_stride), ctypes.byref(grid), int(nsteps),
+                float(dt), ptr(rec_lon), ptr(rec_lat), int(rec_stride), float(walk_amp), int(key[0]), int(key[1]),
+                int(step0), ptr(oob), stream_ptr()))
     return oob
 
 

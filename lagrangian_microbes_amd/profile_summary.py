@@ -33,6 +33,10 @@ TIMER_NAMES = {
     "rps_comps_kernel": "rps_comps", "rps_fill_kernel": "rps_fill", "rps_deps_kernel": "rps_deps",
     "rps_levels_kernel": "rps_levels", "rps_play_kernel": "rps_play", "copy16_kernel": "copy16",
     "philox_kernel": "philox", "seg_copy_kernel": "copy16",
+    "advect_rk4_f32_kernel": "advect_rk4_f32", "field_blend_f32_kernel": "field_blend_f32",
+    "adj_forward_kernel": "adj_forward", "adj_recompute_kernel": "adj_recompute",
+    "adj_backward_kernel": "adj_backward", "adj_blend_kernel": "adj_blend", "adj_slab_kernel": "adj_slab",
+    "adj_zero_kernel": "adj_slab",
 }
 
 

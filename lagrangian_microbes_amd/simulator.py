@@ -33,11 +33,13 @@ class FusedRunner:
 
     lon0/lat0: float64 initial positions; species0: int8; field: VelocityField;
     params: {"pRS", "pPR", "pSP"}; key: Philox key of the pair uniforms;
-    walk_key / Kh (deg^2/s): the random walk (off when Kh == 0).
+    walk_key / Kh (deg^2/s): the random walk (off when Kh == 0);
+    precision: "f64" (the FP64 path) or "f32" (float32 field, interpolation
+    and velocities; float64 positions and pair test).
     """
 
     def __init__(self, lon0, lat0, species0, field, radius, params, key, dt_s, norm=2.0, chunk=100, Kh=0.0,
-                 walk_key=(0, 0x6B68), device=None, bbox=None):
+                 walk_key=(0, 0x6B68), device=None, bbox=None, precision="f64"):
         self.device = torch.device(device or "cuda")
         self.n = int(np.asarray(lon0).size)
         self.lon0 = np.asarray(lon0, dtype=np.float64)
@@ -49,7 +51,7 @@ class FusedRunner:
         self.chunk = int(chunk)
         self.walk_amp = walk_amplitude(Kh, self.dt_s)
         self.walk_key = tuple(walk_key)
-        self.dfield = dev.DeviceField(field, self.device)
+        self.dfield = dev.DeviceField(field, self.device, precision)
         self.engine = InteractionEngine(self.n, bbox or field.bounds, radius, norm, self.device,
                                         steps_max=self.chunk, nsets=2)
         self.rec_x = torch.empty((self.chunk, self.n), dtype=torch.float32, device=self.device)
@@ -78,9 +80,8 @@ class FusedRunner:
         k = min(int(nsteps), self.chunk)
         if k <= 0:
             return 0
-        stage, stride = self.dfield.stage_fields(self.clock_steps * self.dt_s, self.dt_s, k)
-        dev.advect_rk4_f64(self.lon, self.lat, stage, stride, self.dfield.grid, k, self.dt_s, self.rec_x,
-                           self.rec_y, self.n, self.walk_amp, self.walk_key, self.step, self.oob)
+        self.dfield.advect(self.lon, self.lat, self.clock_steps * self.dt_s, self.dt_s, k, self.rec_x, self.rec_y,
+                           self.n, self.walk_amp, self.walk_key, self.step, self.oob)
         self.engine.run_chunk(self.rec_x[:k], self.rec_y[:k], self.species, self.params, self.key, self.step,
                               rec_species=self.rec_s[:k], npairs=self.npairs[:k])
         self.step += k
@@ -128,10 +129,8 @@ class FusedRunner:
             with torch.cuda.stream(side):
                 if played[b] is not None:
                     side.wait_event(played[b])
-                stage, stride = self.dfield.stage_fields(clock0[c] * self.dt_s, self.dt_s, k)
-                dev.advect_rk4_f64(self.lon, self.lat, stage, stride, self.dfield.grid, k, self.dt_s,
-                                   self._rec_sets[b][0], self._rec_sets[b][1], self.n, self.walk_amp,
-                                   self.walk_key, step0[c], self.oob)
+                self.dfield.advect(self.lon, self.lat, clock0[c] * self.dt_s, self.dt_s, k, self._rec_sets[b][0],
+                                   self._rec_sets[b][1], self.n, self.walk_amp, self.walk_key, step0[c], self.oob)
             group(c)
 
         prepare(0)

@@ -26,6 +26,12 @@ Contents and what pins each function (the checks live in tests/):
 - ``pairs_bruteforce`` (C) and :mod:`oracle.pairs` (numpy): query_pairs
   semantics (interaction_simulator.py:93,98).  Pinned against
   scipy.spatial.cKDTree.query_pairs, the library the reference calls.
+- ``advect_rk4_f32`` / ``field_blend_f32`` / ``cos_deg_f32`` (C): the FP32
+  path's operations (float32 field, interpolation and velocities; float64
+  positions), the GPU kernel's restatement.  Pinned by the cosine polynomial
+  against numpy.cos in float32 rounding, closed-form flows, and its distance
+  from ``advect_rk4_f64`` on the same inputs (float32 interpolation error
+  only).
 - :mod:`oracle.adjoint_ref` (torch float64 on the CPU): the RK4 advection
   written with differentiable torch operations, whose autograd gradients the
   hand-written adjoint is checked against.  Pinned by its positions against
@@ -62,6 +68,13 @@ def lib():
         L.oracle_advect_rk4_f64.argtypes = [i64, vp, vp, vp, i64, i32, i32, dbl, dbl, dbl, dbl, i32, dbl,
                                             vp, vp, i64, dbl, u64, u64, u64]
         L.oracle_advect_rk4_f64.restype = i64
+        L.oracle_cos_deg_f32.argtypes = [ctypes.c_float]
+        L.oracle_cos_deg_f32.restype = ctypes.c_float
+        L.oracle_field_blend_f32.argtypes = [vp, i64, vp, vp, i32, vp]
+        L.oracle_field_blend_f32.restype = None
+        L.oracle_advect_rk4_f32.argtypes = [i64, vp, vp, vp, i64, i32, i32, dbl, dbl, dbl, dbl, i32, dbl,
+                                            vp, vp, i64, dbl, u64, u64, u64]
+        L.oracle_advect_rk4_f32.restype = i64
         L.oracle_rps_sequential.argtypes = [i64, vp, vp, vp, i64, dbl, dbl, dbl, u64, u64, u64]
         L.oracle_rps_sequential.restype = None
         L.oracle_pairs_bruteforce.argtypes = [i64, vp, vp, dbl, dbl, vp, vp, i64]
@@ -122,6 +135,41 @@ def advect_rk4_f64(lon, lat, stage_fields, grid, nsteps, dt, steady=False, recor
     rl = np.zeros((nsteps, n), dtype=np.float32) if record else None
     rt = np.zeros((nsteps, n), dtype=np.float32) if record else None
     oob = lib().oracle_advect_rk4_f64(
+        n, _ptr(lon), _ptr(lat), _ptr(f), stride, int(nx), int(ny), float(x0), float(y0), float(idx),
+        float(idy), int(nsteps), float(dt), _ptr(rl) if record else None, _ptr(rt) if record else None,
+        n, float(walk_amp), int(key[0]), int(key[1]), int(step0))
+    return lon, lat, rl, rt, int(oob)
+
+
+def cos_deg_f32(lat):
+    return lib().oracle_cos_deg_f32(float(lat))
+
+
+def field_blend_f32(levels, lev0, w):
+    """levels: float32 [nt, ny, nx, 2]; returns float32 [nstage, ny, nx, 2]."""
+    levels = np.ascontiguousarray(levels, dtype=np.float32)
+    lev0 = np.ascontiguousarray(lev0, dtype=np.int32)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    ln = int(np.prod(levels.shape[1:]))
+    out = np.empty((len(lev0),) + levels.shape[1:], dtype=np.float32)
+    lib().oracle_field_blend_f32(_ptr(levels), ln, _ptr(lev0), _ptr(w), len(lev0), _ptr(out))
+    return out
+
+
+def advect_rk4_f32(lon, lat, stage_fields, grid, nsteps, dt, steady=False, record=True, walk_amp=0.0, key=(0, 0),
+                   step0=0):
+    """The FP32 path's RK4 on copies of lon/lat (float64); stage_fields float32
+    [2*nsteps+1, ny, nx, 2] (or [1, ...] with steady=True).  Returns as
+    advect_rk4_f64."""
+    lon = np.array(lon, dtype=np.float64, copy=True)
+    lat = np.array(lat, dtype=np.float64, copy=True)
+    n = lon.size
+    f = np.ascontiguousarray(stage_fields, dtype=np.float32)
+    stride = 0 if steady else int(np.prod(f.shape[1:]))
+    nx, ny, x0, y0, idx, idy = grid
+    rl = np.zeros((nsteps, n), dtype=np.float32) if record else None
+    rt = np.zeros((nsteps, n), dtype=np.float32) if record else None
+    oob = lib().oracle_advect_rk4_f32(
         n, _ptr(lon), _ptr(lat), _ptr(f), stride, int(nx), int(ny), float(x0), float(y0), float(idx),
         float(idy), int(nsteps), float(dt), _ptr(rl) if record else None, _ptr(rt) if record else None,
         n, float(walk_amp), int(key[0]), int(key[1]), int(step0))

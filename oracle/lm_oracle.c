@@ -183,6 +183,106 @@ int64_t oracle_advect_rk4_f64(int64_t n, double* lon, double* lat, const double*
 }
 
 /* ------------------------------------------------------------------------ */
+/* FP32 path (the float32-field variant of the same method): float32 blend,  */
+/* interpolation and velocities; float64 positions, stage positions, cell     */
+/* lookup and accumulation.  Same operations in the same order as the GPU.   */
+/* ------------------------------------------------------------------------ */
+void oracle_field_blend_f32(const float* levels, int64_t len, const int32_t* lev0, const double* w,
+                            int32_t nstage, float* out) {
+    for (int32_t s = 0; s < nstage; ++s) {
+        const float* a = levels + (int64_t)lev0[s] * len;
+        const float ws = (float)w[s];
+        for (int64_t k = 0; k < len; ++k) {
+            const float f0 = a[k], f1 = a[len + k];
+            out[(int64_t)s * len + k] = fmaf(ws, f1 - f0, f0);
+        }
+    }
+}
+
+#define DEG2RAD_F 0x1.1df46ap-6f          /* pi / 180 rounded to float */
+#define INV_M_PER_DEG_F 0x1.2df73ap-17f   /* 1 / (1852 * 60) rounded to float */
+
+/* cos(lat * pi/180) in float: Taylor series through x^12, Horner in x^2 with
+ * fmaf; coefficients (-1)^k / (2k)! rounded to float. */
+float oracle_cos_deg_f32(float lat_deg) {
+    const float x = lat_deg * DEG2RAD_F;
+    const float z = x * x;
+    float p = 0x1.1eed8ep-29f;
+    p = fmaf(p, z, -0x1.27e4fcp-22f);
+    p = fmaf(p, z, 0x1.a01a02p-16f);
+    p = fmaf(p, z, -0x1.6c16c2p-10f);
+    p = fmaf(p, z, 0x1.555556p-5f);
+    p = fmaf(p, z, -0.5f);
+    return fmaf(p, z, 1.0f);
+}
+
+static int sample_uv_f32(const float* f, const grid_t* g, double x, double y, float* ud, float* vd) {
+    const double fx = (x - g->x0) * g->inv_dx;
+    const double fy = (y - g->y0) * g->inv_dy;
+    if (!(fx >= 0.0 && fx <= (double)(g->nx - 1) && fy >= 0.0 && fy <= (double)(g->ny - 1))) return 0;
+    int i = (int)fx, j = (int)fy;
+    if (i > g->nx - 2) i = g->nx - 2;
+    if (j > g->ny - 2) j = g->ny - 2;
+    const float xi = (float)(fx - (double)i), eta = (float)(fy - (double)j);
+    const float* r0 = f + 2 * ((int64_t)j * g->nx + i);
+    const float* r1 = r0 + 2 * (int64_t)g->nx;
+    const float ub = fmaf(xi, r0[2] - r0[0], r0[0]);
+    const float ut = fmaf(xi, r1[2] - r1[0], r1[0]);
+    const float u = fmaf(eta, ut - ub, ub);
+    const float vb = fmaf(xi, r0[3] - r0[1], r0[1]);
+    const float vt = fmaf(xi, r1[3] - r1[1], r1[1]);
+    const float v = fmaf(eta, vt - vb, vb);
+    *ud = (u * INV_M_PER_DEG_F) / oracle_cos_deg_f32((float)y);
+    *vd = v * INV_M_PER_DEG_F;
+    return 1;
+}
+
+static int rk4_step_f32(const float* f0, const float* fh, const float* f1, const grid_t* g, double hh, double h,
+                        double h6, double* x, double* y) {
+    float u1, v1, u2, v2, u3, v3, u4, v4;
+    const double x0 = *x, y0 = *y;
+    if (!sample_uv_f32(f0, g, x0, y0, &u1, &v1)) return 0;
+    if (!sample_uv_f32(fh, g, fma((double)u1, hh, x0), fma((double)v1, hh, y0), &u2, &v2)) return 0;
+    if (!sample_uv_f32(fh, g, fma((double)u2, hh, x0), fma((double)v2, hh, y0), &u3, &v3)) return 0;
+    if (!sample_uv_f32(f1, g, fma((double)u3, h, x0), fma((double)v3, h, y0), &u4, &v4)) return 0;
+    const float su = fmaf(2.0f, u2 + u3, u1 + u4);
+    const float sv = fmaf(2.0f, v2 + v3, v1 + v4);
+    *x = fma((double)su, h6, x0);
+    *y = fma((double)sv, h6, y0);
+    return 1;
+}
+
+int64_t oracle_advect_rk4_f32(int64_t n, double* lon, double* lat, const float* stage_fields,
+                              int64_t stage_stride, int32_t nx, int32_t ny, double x0, double y0,
+                              double inv_dx, double inv_dy, int32_t nsteps, double dt, float* rec_lon,
+                              float* rec_lat, int64_t rec_stride, double walk_amp, uint64_t k0,
+                              uint64_t k1, uint64_t step0) {
+    grid_t g = {nx, ny, x0, y0, inv_dx, inv_dy};
+    const double hh = 0.5 * dt, h6 = dt / 6.0;
+    int64_t oob = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        double x = lon[i], y = lat[i];
+        for (int32_t s = 0; s < nsteps; ++s) {
+            const float* f0 = stage_fields + (int64_t)(2 * s) * stage_stride;
+            if (!rk4_step_f32(f0, f0 + stage_stride, f0 + 2 * stage_stride, &g, hh, dt, h6, &x, &y)) ++oob;
+            if (rec_lon) {
+                rec_lon[(int64_t)s * rec_stride + i] = (float)x;
+                rec_lat[(int64_t)s * rec_stride + i] = (float)y;
+            }
+            if (walk_amp > 0.0) {
+                uint64_t ctr[4] = {(uint64_t)i, step0 + (uint64_t)s, 1, 0}, key[2] = {k0, k1}, o[4];
+                oracle_philox4x64_10(ctr, key, o);
+                y = fma(fma(2.0, u01(o[0]), -1.0), walk_amp, y);
+                x = fma(fma(2.0, u01(o[1]), -1.0), walk_amp, x);
+            }
+        }
+        lon[i] = x;
+        lat[i] = y;
+    }
+    return oob;
+}
+
+/* ------------------------------------------------------------------------ */
 /* rock-paper-scissors, sequential over the sorted pair list                 */
 /* ------------------------------------------------------------------------ */
 enum { ROCK = 1, PAPER = 2, SCISSORS = 3 };

@@ -110,3 +110,43 @@ def test_against_independent_restatement(name, steps, stride):
     assert np.max(np.abs(lo - x)) < 1e-12 and np.max(np.abs(la - y)) < 1e-12
     assert np.max(np.abs(lo - lon)) > 0.05  # the field really moves particles
     assert np.array_equal(rl, rec[:, 0].astype(np.float32)) or np.max(np.abs(rl - rec[:, 0])) < 1e-4
+
+
+def test_cos_polynomial_f32_against_numpy():
+    lat = np.linspace(-90, 90, 20001)
+    ours = np.array([oracle.cos_deg_f32(v) for v in lat], dtype=np.float64)
+    # float32 rounding of the argument and of the result: a few units of 2^-24
+    assert np.max(np.abs(ours - np.cos(np.deg2rad(lat)))) < 4e-7
+
+
+def test_fp32_restatement_uniform_flows():
+    # constant velocities: every stage sees the same float32 velocity, so the position moves by
+    # 24 equal float32 increments accumulated in float64
+    f = const_field(0.0, 0.3)
+    lon = np.array([205.0]); lat = np.array([30.0])
+    lev0, w = f.stage_schedule(0.0, 3600.0, 24)
+    sf = f.interleaved()[:1].astype(np.float32)
+    lo, la, _, _, oob = oracle.advect_rk4_f32(lon, lat, sf, grid_of(f), 24, 3600.0, steady=True)
+    assert oob == 0 and lo[0] == 205.0
+    vdeg = np.float32(np.float32(0.3) * np.float32(float.fromhex("0x1.2df73ap-17")))
+    assert abs(la[0] - (30.0 + 24 * 3600.0 * float(vdeg))) < 1e-10
+
+
+def test_fp32_restatement_close_to_fp64():
+    # the FP32 path differs from the FP64 one by float32 interpolation and velocity rounding only:
+    # over one day of the 1/12-degree 21-level field the positions agree to ~1e-8 degrees
+    w = configs.rps_4m_60day()
+    f = w.velocity_field()
+    lon, lat = w.particles()
+    sel = np.arange(0, w.n, 40009)[:100]
+    lon, lat = lon[sel], lat[sel]
+    steps = 48
+    lev0, wt = f.stage_schedule(0.0, w.dt, steps)
+    a = oracle.advect_rk4_f64(lon, lat, oracle.field_blend_f64(f.interleaved(), lev0, wt), grid_of(f), steps,
+                              w.dt, record=False)
+    b = oracle.advect_rk4_f32(lon, lat, oracle.field_blend_f32(f.interleaved().astype(np.float32), lev0, wt),
+                              grid_of(f), steps, w.dt, record=False)
+    d = np.hypot(a[0] - b[0], a[1] - b[1])
+    assert a[4] == b[4] == 0
+    assert d.max() < 5e-8, d.max()
+    assert np.median(np.hypot(a[0] - lon, a[1] - lat)) > 0.05  # the particles do move
