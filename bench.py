@@ -214,9 +214,9 @@ def fp32_leg(steps):
             elapsed += time.perf_counter() - t0
             done = m
             snaps[prec].append((r.lon.cpu().numpy(), r.lat.cpu().numpy()))
-        r.check_domain()
         out[prec] = {"ms_per_step": round(1e3 * elapsed / steps, 4),
-                     "particle_steps_per_s": w.n * steps / elapsed}
+                     "particle_steps_per_s": w.n * steps / elapsed,
+                     "particle_steps_outside_field": int(r.oob.item())}
         log(f"[bench] {w.name} {prec}: {out[prec]['ms_per_step']} ms/step")
         del r
         torch.cuda.empty_cache()

@@ -10,7 +10,8 @@ Choices the workload descriptions leave open are fixed here:
 - the 10-degree patches are the reference's canonical 25-35 N, 205-215 E
   (rock_paper_scissors_example.py:22, sandbox/rock_paper_scissors_advect.py);
 - fields extend a margin beyond the patch (2 degrees for the 1-degree patch,
-  5 degrees otherwise) so that no particle reaches the field edge;
+  10 degrees for the 60-day run, 5 degrees otherwise) so that no particle
+  reaches the field edge;
 - the rock-paper-scissors key is (species seed, run index).
 """
 from dataclasses import dataclass, field as dc_field
@@ -99,9 +100,11 @@ def rps_490k_30day():
 
 
 def rps_4m_60day():
-    """4,000,000 microbes on 2000x2000 over 20x20 degrees, 1/12 degree field, 21 levels 3 days apart."""
+    """4,000,000 microbes on 2000x2000 over 20x20 degrees, 1/12 degree field, 21 levels 3 days apart.
+    The field extends 10 degrees beyond the patch: over 60 days the eddies carry some particles more
+    than 5 degrees."""
     return Workload(name="rps_4m_60day", n=4_000_000, lat_range=(20.0, 40.0), lon_range=(200.0, 220.0),
-                    field_spacing=1.0 / 12.0, field_margin=5.0, field_levels=21, field_level_interval=3 * DAY,
+                    field_spacing=1.0 / 12.0, field_margin=10.0, field_levels=21, field_level_interval=3 * DAY,
                     field_modes=32, field_rms=0.3, field_seed=2, dt=0.5 * HOUR, steps=2880, p=0.6,
                     species_seed=2, precision="f32")
 
