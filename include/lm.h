@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define LM_ABI_VERSION 4
+#define LM_ABI_VERSION 5
 
 #define LM_OK 0
 #define LM_ERR_ARG (-1)      /* invalid argument                             */
@@ -119,6 +119,13 @@ int lm_pairs_ws_create(int64_t n_max, int32_t steps_max, double lon_min, double 
 void lm_pairs_ws_destroy(lm_pairs_ws* ws);
 /* Device bytes held by the workspace. */
 int64_t lm_pairs_ws_bytes(const lm_pairs_ws* ws);
+/* The same with cells of side cell_scale * max_radius (1 <= cell_scale <= 16;
+ * 1 is lm_pairs_ws_create): larger cells hold more candidates each but need
+ * fewer table entries, a trade the ensemble's autotuner measures.  No
+ * reference counterpart (cKDTree has no cell size). */
+int lm_pairs_ws_create_ex(int64_t n_max, int32_t steps_max, double lon_min, double lon_max,
+                          double lat_min, double lat_max, double max_radius, double cell_scale,
+                          lm_pairs_ws** out);
 /* Number of cell buckets T of the workspace's cell list (a power of two). */
 int64_t lm_pairs_ws_buckets(const lm_pairs_ws* ws);
 
@@ -146,6 +153,17 @@ int lm_query_pairs_batch(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float
                          const float* lat, int64_t pos_stride, double radius, double p,
                          int32_t* pair_i, int32_t* pair_j, int64_t pair_stride, int64_t* npairs,
                          void* stream);
+
+/* Ensemble variant: particle k belongs to run run[k] (device uint16[n]); runs
+ * are independent, so only pairs of one run are returned.  Each run's pairs
+ * are exactly those lm_query_pairs_batch finds on that run alone (indices
+ * shifted by the run's first index); the list stays sorted by (i, j).
+ * Replaces the query of interaction_simulator.py:88-98 for many runs at
+ * once. */
+int lm_query_pairs_runs(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float* lon,
+                        const float* lat, int64_t pos_stride, const uint16_t* run, double radius,
+                        double p, int32_t* pair_i, int32_t* pair_j, int64_t pair_stride,
+                        int64_t* npairs, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Rock-paper-scissors update                                               */
@@ -191,6 +209,27 @@ int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair
                    const int32_t* pair_j, int64_t pair_stride, const int64_t* npairs,
                    double pRS, double pPR, double pSP, uint64_t key0, uint64_t key1,
                    uint64_t step0, void* stream);
+/* Per-run parameters of an ensemble (a device array, one entry per run): the
+ * Philox key and the forward probabilities. */
+typedef struct lm_run_params {
+    uint64_t key0, key1;
+    double pRS, pPR, pSP;
+} lm_run_params;
+
+/* Ensemble variant of lm_rps_prepare: run r holds particles
+ * [run_start[r], run_start[r+1]) (device int64[nruns+1]; run[k] (device
+ * uint16[n]) is the same map per particle) and the pair lists hold pairs of
+ * one run each (lm_query_pairs_runs).  A pair of run r draws word 0 of
+ * Philox4x64-10(counter = (its rank among run r's pairs of the step,
+ * step0 + s, 0, 0), key = params[r].key) and uses params[r]'s p: every run's
+ * species evolve exactly as lm_rps_prepare/lm_rps_play give on that run
+ * alone.  No reference counterpart: the reference runs ensembles as separate
+ * processes (SLURM scripts). */
+int lm_rps_prepare_runs(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair_i,
+                        const int32_t* pair_j, int64_t pair_stride, const int64_t* npairs,
+                        const uint16_t* run, int32_t nruns, const int64_t* run_start,
+                        const lm_run_params* params, uint64_t step0, void* stream);
+
 int lm_rps_play(lm_rps_ws* ws, int32_t step, int64_t n, const int8_t* species_in,
                 int8_t* species_out, void* stream);
 

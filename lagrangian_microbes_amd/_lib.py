@@ -35,12 +35,16 @@ _SIGNATURES = {
                                          c_vp, c_vp, c_i64, c_dbl, c_u64, c_u64, c_u64, c_vp, c_vp]),
     "lm_pairs_ws_create": (ctypes.c_int, [c_i64, c_i32, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl,
                                           ctypes.POINTER(c_vp)]),
+    "lm_pairs_ws_create_ex": (ctypes.c_int, [c_i64, c_i32, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl,
+                                             ctypes.POINTER(c_vp)]),
     "lm_pairs_ws_destroy": (None, [c_vp]),
     "lm_pairs_ws_bytes": (c_i64, [c_vp]),
     "lm_pairs_ws_buckets": (c_i64, [c_vp]),
     "lm_query_pairs": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_dbl, c_dbl, c_vp, c_vp, c_i64, c_vp, c_vp]),
     "lm_query_pairs_batch": (ctypes.c_int, [c_vp, c_i32, c_i64, c_vp, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_vp,
                                             c_i64, c_vp, c_vp]),
+    "lm_query_pairs_runs": (ctypes.c_int, [c_vp, c_i32, c_i64, c_vp, c_vp, c_i64, c_vp, c_dbl, c_dbl, c_vp, c_vp,
+                                           c_i64, c_vp, c_vp]),
     "lm_rps_ws_create": (ctypes.c_int, [c_i64, c_i32, ctypes.POINTER(c_vp)]),
     "lm_rps_ws_destroy": (None, [c_vp]),
     "lm_rps_ws_bytes": (c_i64, [c_vp]),
@@ -48,6 +52,8 @@ _SIGNATURES = {
                                      c_u64, c_u64, c_u64, c_vp]),
     "lm_rps_prepare": (ctypes.c_int, [c_vp, c_i32, c_i64, c_vp, c_vp, c_i64, c_vp, c_dbl, c_dbl, c_dbl,
                                       c_u64, c_u64, c_u64, c_vp]),
+    "lm_rps_prepare_runs": (ctypes.c_int, [c_vp, c_i32, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_i32, c_vp, c_vp,
+                                           c_u64, c_vp]),
     "lm_rps_play": (ctypes.c_int, [c_vp, c_i32, c_i64, c_vp, c_vp, c_vp]),
     "lm_rps_component_stats": (ctypes.c_int, [c_vp, c_i32, c_vp, c_vp]),
     "lm_philox_uniforms": (ctypes.c_int, [c_i64, c_u64, c_u64, c_u64, c_u64, c_u64, c_u64, c_i32, c_vp, c_vp]),
@@ -81,7 +87,7 @@ def load(path=LIB_PATH):
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        if L.lm_abi_version() != 4:
+        if L.lm_abi_version() != 5:
             raise RuntimeError("liblm.so ABI version mismatch; rebuild it")
         _lib = L
     return _lib

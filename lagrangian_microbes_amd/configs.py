@@ -14,6 +14,7 @@ Choices the workload descriptions leave open are fixed here:
   reaches the field edge;
 - the rock-paper-scissors key is (species seed, run index).
 """
+import math
 from dataclasses import dataclass, field as dc_field
 
 import numpy as np
@@ -128,3 +129,75 @@ def adjoint_10k_24h():
 
 WORKLOADS = {w().name: w for w in (rps_10k_steady, rps_490k_30day, rps_4m_60day, adjoint_1m_10day,
                                    adjoint_10k_24h)}
+
+
+@dataclass
+class EnsembleWorkload:
+    """Independent runs sharing one velocity field: each run is a Workload
+    (its lattice, species seed, p and Philox key) and all of them read
+    ``field_of``'s field."""
+    name: str
+    runs: list
+    field_of: Workload
+    steps: int
+
+    @property
+    def n(self):
+        return sum(r.n for r in self.runs)
+
+    def velocity_field(self):
+        return self.field_of.velocity_field()
+
+    def particles(self):
+        parts = [r.particles() for r in self.runs]
+        return np.concatenate([p[0] for p in parts]), np.concatenate([p[1] for p in parts])
+
+    def species(self):
+        return np.concatenate([r.species() for r in self.runs])
+
+    def describe(self):
+        sizes = [r.n for r in self.runs]
+        return {"workload": self.name, "runs": len(self.runs), "particles": self.n, "min_run": min(sizes),
+                "max_run": max(sizes), "steps": self.steps, "p_range": [self.runs[0].p, self.runs[-1].p],
+                "field": self.field_of.describe()["field"]}
+
+
+ENSEMBLE_CENTRES = ((210.0, 30.0), (207.5, 27.5), (212.5, 27.5), (207.5, 32.5), (212.5, 32.5))
+
+
+def ensemble_runs(sizes, steps, name, spacing=10.0 / 699.0):
+    """Runs on square lattices of side m = sqrt(n) with the given spacing
+    (default the 490k run's, 10/699 degrees), centred in turn on
+    ENSEMBLE_CENTRES; p from 0.50 to 0.70 evenly over the runs; species seed
+    100 + r, Philox key (100 + r, r); the 490k run's 30-day field, hourly
+    steps, 1 km, FP32."""
+    base = rps_490k_30day()
+    runs = []
+    R = len(sizes)
+    for r, n in enumerate(sizes):
+        m = int(round(math.sqrt(n)))
+        half = 0.5 * (m - 1) * spacing
+        lon_c, lat_c = ENSEMBLE_CENTRES[r % len(ENSEMBLE_CENTRES)]
+        p = 0.50 + 0.20 * r / (R - 1) if R > 1 else 0.50
+        runs.append(Workload(name=f"{name}_run{r}", n=m * m, lat_range=(lat_c - half, lat_c + half),
+                             lon_range=(lon_c - half, lon_c + half), field_spacing=base.field_spacing,
+                             field_margin=base.field_margin, field_levels=base.field_levels,
+                             field_level_interval=base.field_level_interval, field_modes=base.field_modes,
+                             field_rms=base.field_rms, field_seed=base.field_seed, dt=HOUR, steps=steps, p=p,
+                             species_seed=100 + r, run=r, precision="f32"))
+    return EnsembleWorkload(name=name, runs=runs, field_of=base, steps=steps)
+
+
+def ensemble_64runs():
+    """64 runs, particle counts log-uniform in [50k, 500k] (seed 4) rounded to
+    square lattices, 240 hourly steps."""
+    rng = np.random.default_rng(4)
+    sizes = np.exp(rng.uniform(math.log(50_000), math.log(500_000), 64))
+    return ensemble_runs(sizes, 240, "ensemble_64runs_10day")
+
+
+def ensemble_small():
+    """Five small runs of uneven size for the per-run parity tests, on lattices
+    finer than the radius (0.007 degrees) so that they interact from the
+    first step."""
+    return ensemble_runs([400, 1225, 2500, 1681, 784], 24, "ensemble_5runs_1day", spacing=0.007)

@@ -20,6 +20,11 @@
 // A batch of S steps (S position snapshots) runs every phase once with
 // blockIdx.y = step, each step with its own tables and pair region, so a
 // launch carries S times the parallelism of one step.  No global atomics.
+// Ensembles: with a run id per particle (independent runs sharing the index
+// range), the bucket is (run * cells + cell) mod T and the record carries the
+// run, so a stencil visits the cells of its own run and candidates of other
+// runs whose cells alias into the same buckets are dropped: pairs never cross
+// runs.
 // Distances use the float32 positions widened to double, with the same
 // operations cKDTree applies: dx*dx + dy*dy <= r*r for p = 2.  Since a
 // difference of two nearby floats is exact in double and its square is exact,
@@ -37,8 +42,9 @@ constexpr int kNbr = 8;  // neighbours kept per particle before the selection pa
 struct CellGrid {
     double x0, y0, inv_h;
     int32_t nx, ny;
-    uint32_t mask;  // T - 1
+    uint32_t mask;   // T - 1
     uint32_t T;
+    uint32_t cells;  // nx * ny (mod 2^32): the key distance between runs
 };
 
 enum MetricKind { kL2 = 0, kL1 = 1, kLinf = 2, kLp = 3 };
@@ -71,8 +77,8 @@ __device__ __forceinline__ void cell_of(const CellGrid& g, float xf, float yf, i
     *cy = iy;
 }
 
-__device__ __forceinline__ uint32_t bucket_of(const CellGrid& g, int cx, int cy) {
-    return ((uint32_t)cy * (uint32_t)g.nx + (uint32_t)cx) & g.mask;
+__device__ __forceinline__ uint32_t bucket_of(const CellGrid& g, int cx, int cy, uint32_t run) {
+    return (run * g.cells + (uint32_t)cy * (uint32_t)g.nx + (uint32_t)cx) & g.mask;
 }
 
 __device__ __forceinline__ bool within(const Metric& m, float xa, float ya, float xb, float yb) {
@@ -91,15 +97,15 @@ __device__ __forceinline__ bool within(const Metric& m, float xa, float ya, floa
 // index past the records.
 template <typename F>
 __device__ __forceinline__ void for_stencil(const CellGrid& g, const uint32_t* __restrict__ start,
-                                            uint32_t nrec, int cx, int cy, F&& visit) {
+                                            uint32_t nrec, int cx, int cy, uint32_t run, F&& visit) {
     const int c0 = cx > 0 ? cx - 1 : 0;
     const int c1 = cx < g.nx - 1 ? cx + 1 : g.nx - 1;
 #pragma unroll
     for (int dyc = -1; dyc <= 1; ++dyc) {
         const int row = cy + dyc;
         if (row < 0 || row >= g.ny) continue;
-        const uint32_t b0 = bucket_of(g, c0, row);
-        const uint32_t b1 = bucket_of(g, c1, row);
+        const uint32_t b0 = bucket_of(g, c0, row, run);
+        const uint32_t b1 = bucket_of(g, c1, row, run);
         if (b0 <= b1) {
             const uint32_t e = min(start[b1 + 1], nrec);
             for (uint32_t s = start[b0]; s < e; ++s) visit(s);
@@ -113,14 +119,15 @@ __device__ __forceinline__ void for_stencil(const CellGrid& g, const uint32_t* _
 }
 
 __global__ void __launch_bounds__(kBlock)
-bin_keys_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y, CellGrid g, Strides st,
-                uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+bin_keys_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y,
+                const uint16_t* __restrict__ run, CellGrid g, Strides st, uint32_t* __restrict__ keys,
+                uint32_t* __restrict__ vals) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const int64_t s = blockIdx.y;
     int cx, cy;
     cell_of(g, x[s * st.pos + i], y[s * st.pos + i], &cx, &cy);
-    keys[s * st.rec + i] = bucket_of(g, cx, cy);
+    keys[s * st.rec + i] = bucket_of(g, cx, cy, run ? (uint32_t)run[i] : 0u);
     vals[s * st.rec + i] = (uint32_t)i;
 }
 
@@ -129,15 +136,18 @@ bin_keys_kernel(int64_t n, const float* __restrict__ x, const float* __restrict_
 // counting its `fine` buckets in LDS.  It writes their starts (a contiguous
 // slice of the table; the last super-bucket also the end, start[T] = n) and
 // places the records {x, y, index} of its particles in bucket order (the
-// order inside a bucket is free: the pair list is sorted later).
+// order inside a bucket is free: the pair list is sorted later).  FINE_MAX
+// buckets per block: 4096 up to 2^22 buckets, 8192 up to 2^23.
 constexpr uint32_t kFineMax = 4096;
+constexpr uint32_t kFineMaxWide = 8192;
 
+template <uint32_t FINE_MAX>
 __global__ void __launch_bounds__(kBlock)
-bin_table_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y, CellGrid g, Strides st,
-                 const uint32_t* __restrict__ keys_all, const uint32_t* __restrict__ vals_all,
-                 const uint32_t* __restrict__ dstart, int64_t drow, int64_t dstride, uint32_t fine,
-                 uint32_t* __restrict__ start_all, float4* __restrict__ recs_all) {
-    __shared__ uint32_t cnt[kFineMax];
+bin_table_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y,
+                 const uint16_t* __restrict__ run, CellGrid g, Strides st, const uint32_t* __restrict__ keys_all,
+                 const uint32_t* __restrict__ vals_all, const uint32_t* __restrict__ dstart, int64_t drow,
+                 int64_t dstride, uint32_t fine, uint32_t* __restrict__ start_all, float4* __restrict__ recs_all) {
+    __shared__ uint32_t cnt[FINE_MAX];
     __shared__ uint32_t scan_lds[kBlock / 64];
     const uint32_t d = blockIdx.x, nsuper = gridDim.x, t = threadIdx.x;
     const int64_t s = blockIdx.y;
@@ -162,11 +172,11 @@ bin_table_kernel(int64_t n, const float* __restrict__ x, const float* __restrict
     uint32_t sum = 0;
     for (uint32_t f = f0; f < f1; ++f) sum += cnt[f];
     uint32_t all;
-    uint32_t run = block_exclusive_scan(sum, scan_lds, &all);
+    uint32_t acc = block_exclusive_scan(sum, scan_lds, &all);
     for (uint32_t f = f0; f < f1; ++f) {
         const uint32_t c = cnt[f];
-        cnt[f] = run;
-        run += c;
+        cnt[f] = acc;
+        acc += c;
     }
     __syncthreads();
     const uint64_t base = (uint64_t)d * fine;
@@ -178,7 +188,8 @@ bin_table_kernel(int64_t n, const float* __restrict__ x, const float* __restrict
         const uint32_t i = vals[e];
         const uint32_t pos = a + atomicAdd(&cnt[keys[e] & fmask], 1u);
         if (i < nn && pos < b)
-            recs_all[s * st.rec + pos] = make_float4(x[s * st.pos + i], y[s * st.pos + i], __uint_as_float(i), 0.0f);
+            recs_all[s * st.rec + pos] = make_float4(x[s * st.pos + i], y[s * st.pos + i], __uint_as_float(i),
+                                                     __uint_as_float(run ? (uint32_t)run[i] : 0u));
     }
 }
 
@@ -186,7 +197,7 @@ bin_table_kernel(int64_t n, const float* __restrict__ x, const float* __restrict
 // three cells is one bucket range, or two when it wraps around the table.
 // All bucket starts are loaded at once.  Ranges are clamped to nrec.
 __device__ __forceinline__ void stencil_ranges(const CellGrid& g, const uint32_t* __restrict__ start, uint32_t nrec,
-                                               int cx, int cy, uint32_t (&rb)[6], uint32_t (&re)[6]) {
+                                               int cx, int cy, uint32_t run, uint32_t (&rb)[6], uint32_t (&re)[6]) {
     const int c0 = cx > 0 ? cx - 1 : 0;
     const int c1 = cx < g.nx - 1 ? cx + 1 : g.nx - 1;
     uint32_t lo[6], hi[6];
@@ -195,8 +206,8 @@ __device__ __forceinline__ void stencil_ranges(const CellGrid& g, const uint32_t
     for (int r = 0; r < 3; ++r) {
         const int row = cy + r - 1;
         const bool valid = row >= 0 && row < g.ny;
-        const uint32_t b0 = valid ? bucket_of(g, c0, row) : 0u;
-        const uint32_t b1 = valid ? bucket_of(g, c1, row) : 0u;
+        const uint32_t b0 = valid ? bucket_of(g, c0, row, run) : 0u;
+        const uint32_t b1 = valid ? bucket_of(g, c1, row, run) : 0u;
         const bool wrap = b0 > b1;
         // first segment [b0, wrap ? T : b1 + 1), second [0, b1 + 1) if wrapped
         lo[2 * r] = b0;
@@ -235,10 +246,11 @@ pair_count_kernel(int64_t n, CellGrid g, Metric m, Strides st, const uint32_t* _
     const float4 me = recs[s0];
     const uint32_t i = __float_as_uint(me.z);
     if (i >= (uint64_t)n) return;
+    const uint32_t my_run = __float_as_uint(me.w);
     int cx, cy;
     cell_of(g, me.x, me.y, &cx, &cy);
     uint32_t rb[6], re[6];
-    stencil_ranges(g, start, (uint32_t)n, cx, cy, rb, re);
+    stencil_ranges(g, start, (uint32_t)n, cx, cy, my_run, rb, re);
     // the ranges as one sequence: candidate t lies in range r when
     // pre[r] <= t < pre[r + 1]
     uint32_t pre[7];
@@ -264,8 +276,8 @@ pair_count_kernel(int64_t n, CellGrid g, Metric m, Strides st, const uint32_t* _
             // |dx| <= r and |dy| <= r hold for every p >= 1; the float32
             // differences of nearby positions are exact, and m.box is above
             // r, so the prefilter only drops pairs the exact test drops
-            if (t0 + u < total && j > i && fabsf(o[u].x - me.x) <= m.box && fabsf(o[u].y - me.y) <= m.box &&
-                within(m, me.x, me.y, o[u].x, o[u].y)) {
+            if (t0 + u < total && j > i && __float_as_uint(o[u].w) == my_run && fabsf(o[u].x - me.x) <= m.box &&
+                fabsf(o[u].y - me.y) <= m.box && within(m, me.x, me.y, o[u].x, o[u].y)) {
                 if (cnt < kNbr) row[cnt] = j;
                 ++cnt;
             }
@@ -299,8 +311,8 @@ __device__ __forceinline__ void emit_sorted(const uint32_t* row, uint32_t c, uin
 }
 
 __global__ void __launch_bounds__(kBlock)
-pair_emit_kernel(int64_t n, const float* __restrict__ x_all, const float* __restrict__ y_all, CellGrid g,
-                 Metric m, Strides st, const uint32_t* __restrict__ start_all, const float4* __restrict__ recs_all,
+pair_emit_kernel(int64_t n, const float* __restrict__ x_all, const float* __restrict__ y_all,
+                 const uint16_t* __restrict__ run, CellGrid g, Metric m, Strides st, const uint32_t* __restrict__ start_all, const float4* __restrict__ recs_all,
                  const uint32_t* __restrict__ nbr_cnt_all, const uint32_t* __restrict__ nbr_all,
                  const uint32_t* __restrict__ row_off_all, int32_t* __restrict__ pair_i_all,
                  int32_t* __restrict__ pair_j_all) {
@@ -334,6 +346,7 @@ pair_emit_kernel(int64_t n, const float* __restrict__ x_all, const float* __rest
     const uint32_t* start = start_all + s * st.tab;
     const float4* recs = recs_all + s * st.rec;
     const float xf = x_all[s * st.pos + i], yf = y_all[s * st.pos + i];
+    const uint32_t my_run = run ? (uint32_t)run[i] : 0u;
     int cx, cy;
     cell_of(g, xf, yf, &cx, &cy);
     uint32_t last = i;
@@ -341,10 +354,10 @@ pair_emit_kernel(int64_t n, const float* __restrict__ x_all, const float* __rest
         uint32_t v[kSel];
 #pragma unroll
         for (int q = 0; q < kSel; ++q) v[q] = 0xffffffffu;
-        for_stencil(g, start, (uint32_t)n, cx, cy, [&](uint32_t q0) {
+        for_stencil(g, start, (uint32_t)n, cx, cy, my_run, [&](uint32_t q0) {
             const float4 rec = recs[q0];
             uint32_t j = __float_as_uint(rec.z);
-            if (j > last && j < v[kSel - 1] && within(m, xf, yf, rec.x, rec.y)) {
+            if (j > last && j < v[kSel - 1] && __float_as_uint(rec.w) == my_run && within(m, xf, yf, rec.x, rec.y)) {
 #pragma unroll
                 for (int q = 0; q < kSel; ++q) {  // insert into the sorted register array
                     const uint32_t lo = min(v[q], j), hi = max(v[q], j);
@@ -428,6 +441,11 @@ extern "C" {
 
 int lm_pairs_ws_create(int64_t n_max, int32_t steps_max, double lon_min, double lon_max, double lat_min,
                        double lat_max, double max_radius, lm_pairs_ws** out) {
+    return lm_pairs_ws_create_ex(n_max, steps_max, lon_min, lon_max, lat_min, lat_max, max_radius, 1.0, out);
+}
+
+int lm_pairs_ws_create_ex(int64_t n_max, int32_t steps_max, double lon_min, double lon_max, double lat_min,
+                          double lat_max, double max_radius, double cell_scale, lm_pairs_ws** out) {
     LM_REQUIRE(out != nullptr, LM_ERR_ARG, "out is NULL");
     *out = nullptr;
     LM_REQUIRE(n_max >= 0 && n_max < (int64_t)0x7fffffff, LM_ERR_ARG, "n_max out of range");
@@ -436,6 +454,8 @@ int lm_pairs_ws_create(int64_t n_max, int32_t steps_max, double lon_min, double 
                    std::isfinite(lat_max) && lon_max >= lon_min && lat_max >= lat_min,
                LM_ERR_ARG, "invalid bounding box");
     LM_REQUIRE(std::isfinite(max_radius) && max_radius > 0.0, LM_ERR_ARG, "max_radius must be > 0");
+    LM_REQUIRE(std::isfinite(cell_scale) && cell_scale >= 1.0 && cell_scale <= 16.0, LM_ERR_ARG,
+               "cell_scale must be in [1, 16]");
 
     lm_pairs_ws* w = new lm_pairs_ws();
     w->n_max = n_max;
@@ -443,7 +463,7 @@ int lm_pairs_ws_create(int64_t n_max, int32_t steps_max, double lon_min, double 
     w->max_radius = max_radius;
     // cells strictly larger than the radius so rounding in the cell index can
     // never separate two points within the radius by more than one cell
-    double h = max_radius * (1.0 + 1e-6);
+    double h = max_radius * cell_scale * (1.0 + 1e-6);
     const double ext = (lon_max - lon_min) > (lat_max - lat_min) ? (lon_max - lon_min) : (lat_max - lat_min);
     if (ext / h > 30000.0) h = ext / 30000.0;
     lm::CellGrid& g = w->grid;
@@ -460,12 +480,13 @@ int lm_pairs_ws_create(int64_t n_max, int32_t steps_max, double lon_min, double 
     uint32_t T = next_pow2_u32(want);
     if ((uint64_t)T >= cells) T = next_pow2_u32(cells);
     if (T < 1024) T = 1024;
-    // the binning splits a bucket key into 12 bits counted in LDS and one radix pass of at most
-    // kMaxDigitBits: at most 2^22 buckets (about one per particle at 4M particles)
-    const uint32_t t_max = 1u << (12 + lm::kMaxDigitBits);
+    // the binning splits a bucket key into at most 13 bits counted in LDS and one radix pass of at
+    // most kMaxDigitBits: at most 2^23 buckets
+    const uint32_t t_max = 1u << (13 + lm::kMaxDigitBits);
     if (T > t_max) T = t_max;
     g.T = T;
     g.mask = T - 1;
+    g.cells = (uint32_t)((uint64_t)g.nx * (uint64_t)g.ny);
     w->tab_stride = lm::scan_row_stride((int64_t)T);
     w->row_stride = lm::scan_row_stride(n_max);
 
@@ -500,6 +521,13 @@ int64_t lm_pairs_ws_buckets(const lm_pairs_ws* ws) { return ws ? (int64_t)ws->gr
 int lm_query_pairs_batch(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float* lon, const float* lat,
                          int64_t pos_stride, double radius, double p, int32_t* pair_i, int32_t* pair_j,
                          int64_t pair_stride, int64_t* npairs, void* stream) {
+    return lm_query_pairs_runs(ws, nsteps, n, lon, lat, pos_stride, nullptr, radius, p, pair_i, pair_j, pair_stride,
+                               npairs, stream);
+}
+
+int lm_query_pairs_runs(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float* lon, const float* lat,
+                        int64_t pos_stride, const uint16_t* run, double radius, double p, int32_t* pair_i,
+                        int32_t* pair_j, int64_t pair_stride, int64_t* npairs, void* stream) {
     LM_REQUIRE(ws != nullptr, LM_ERR_ARG, "workspace is NULL");
     LM_REQUIRE(npairs != nullptr, LM_ERR_ARG, "npairs is NULL");
     LM_REQUIRE(nsteps >= 0 && nsteps <= ws->steps_max, LM_ERR_ARG, "nsteps=%d outside 0..%d of the workspace",
@@ -548,12 +576,13 @@ int lm_query_pairs_batch(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float
     st.nbr = ws->n_max * lm::kNbr;
     st.pair = pair_stride;
     const dim3 grid(lm::grid_for(n), (unsigned)nsteps);
-    LM_LAUNCH(lm::K_BIN_KEYS, lm::bin_keys_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, g, st, ws->keys[0],
-              ws->vals[0]);
+    LM_LAUNCH(lm::K_BIN_KEYS, lm::bin_keys_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, run, g, st,
+              ws->keys[0], ws->vals[0]);
     LM_LAUNCH_CHECK();
     // two-level binning: one radix pass on the high bits of the bucket, then
     // a block per super-bucket sorts its fine buckets in LDS
-    const int fine_bits = ws->key_bits < 12 ? ws->key_bits : 12;
+    // 12 fine bits in LDS (13 above 2^22 buckets), the rest in one radix pass
+    const int fine_bits = ws->key_bits <= 12 ? ws->key_bits : ws->key_bits <= 22 ? 12 : ws->key_bits - lm::kMaxDigitBits;
     const int super_bits = ws->key_bits - fine_bits;
     const uint32_t* dstart = nullptr;
     int64_t dstride = 0;
@@ -565,9 +594,15 @@ int lm_query_pairs_batch(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float
         if (rc) return rc;
         res = 1;
     }
-    LM_LAUNCH(lm::K_BIN_TABLE, lm::bin_table_kernel, dim3(1u << super_bits, (unsigned)nsteps), dim3(lm::kBlock), 0,
-              s, n, lon, lat, g, st, (const uint32_t*)ws->keys[res], (const uint32_t*)ws->vals[res], dstart,
-              ws->sort.row, dstride, 1u << fine_bits, ws->start, ws->recs);
+    if (fine_bits <= 12) {
+        LM_LAUNCH(lm::K_BIN_TABLE, lm::bin_table_kernel<lm::kFineMax>, dim3(1u << super_bits, (unsigned)nsteps),
+                  dim3(lm::kBlock), 0, s, n, lon, lat, run, g, st, (const uint32_t*)ws->keys[res],
+                  (const uint32_t*)ws->vals[res], dstart, ws->sort.row, dstride, 1u << fine_bits, ws->start, ws->recs);
+    } else {
+        LM_LAUNCH(lm::K_BIN_TABLE, lm::bin_table_kernel<lm::kFineMaxWide>, dim3(1u << super_bits, (unsigned)nsteps),
+                  dim3(lm::kBlock), 0, s, n, lon, lat, run, g, st, (const uint32_t*)ws->keys[res],
+                  (const uint32_t*)ws->vals[res], dstart, ws->sort.row, dstride, 1u << fine_bits, ws->start, ws->recs);
+    }
     LM_LAUNCH_CHECK();
     LM_LAUNCH(lm::K_PAIR_COUNT, lm::pair_count_kernel, grid, dim3(lm::kBlock), 0, s, n, g, m, st,
               (const uint32_t*)ws->start, (const float4*)ws->recs, ws->nbr_cnt, ws->nbr);
@@ -575,7 +610,7 @@ int lm_query_pairs_batch(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float
     rc = lm::exclusive_scan_u32_batched(ws->nbr_cnt, n, st.row, ws->row_off, st.row, nsteps, ws->scan_tmp, npairs,
                                         s);
     if (rc) return rc;
-    LM_LAUNCH(lm::K_PAIR_EMIT, lm::pair_emit_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, g, m, st,
+    LM_LAUNCH(lm::K_PAIR_EMIT, lm::pair_emit_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, run, g, m, st,
               (const uint32_t*)ws->start, (const float4*)ws->recs, (const uint32_t*)ws->nbr_cnt,
               (const uint32_t*)ws->nbr, (const uint32_t*)ws->row_off, pair_i, pair_j);
     LM_LAUNCH_CHECK();

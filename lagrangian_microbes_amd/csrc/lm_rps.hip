@@ -180,10 +180,22 @@ rps_init_kernel(int64_t n, RpsArrays A, RpsStrides st) {
     if (v < 4) A.lists[4 * s + v] = 0;
 }
 
+// Runs: an ensemble of independent runs sharing the index range (run r holds
+// particles [run_start[r], run_start[r+1]) and never pairs across runs).  A
+// pair of run r draws its uniform as run r would alone: counter (its rank
+// among run r's pairs of the step, step, 0, 0), run r's key, run r's p.
+struct RunsView {
+    const uint16_t* run;       // per particle
+    const lm_run_params* rp;   // per run
+    const uint32_t* off;       // [step][run]: first pair of the run in the step's list
+    int nruns;
+};
+
+template <bool RUNS>
 __global__ void __launch_bounds__(kBlock)
 rps_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __restrict__ pj_all,
                  const int64_t* __restrict__ npairs, RpsArrays A, RpsStrides st, double pRS, double pPR,
-                 double pSP, uint64_t k0, uint64_t k1, uint64_t step0) {
+                 double pSP, uint64_t k0, uint64_t k1, uint64_t step0, RunsView R) {
     const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
     const int64_t P = step_pairs(npairs, s, st.pair);
     if (blockIdx.x == 0 && threadIdx.x == 0) A.plen[s] = P < 0 ? 0 : P;
@@ -194,9 +206,20 @@ rps_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* _
     uint8_t* bits = A.bits + po;
     for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
         const uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
-        const double u =
-            u01_from_u64(philox4x64_10((uint64_t)k, step0 + (uint64_t)s, kStreamPairs, 0, k0, k1).v[0]);
-        bits[k] = (uint8_t)((u < pRS ? 1u : 0u) | (u < pPR ? 2u : 0u) | (u < pSP ? 4u : 0u));
+        uint64_t rank = (uint64_t)k, q0 = k0, q1 = k1;
+        double a_rs = pRS, a_pr = pPR, a_sp = pSP;
+        if (RUNS) {
+            const uint32_t r = a < (uint64_t)n ? (uint32_t)R.run[a] : 0u;
+            const lm_run_params q = R.rp[r];
+            rank = (uint64_t)(k - (int64_t)R.off[s * R.nruns + r]);
+            q0 = q.key0;
+            q1 = q.key1;
+            a_rs = q.pRS;
+            a_pr = q.pPR;
+            a_sp = q.pSP;
+        }
+        const double u = u01_from_u64(philox4x64_10(rank, step0 + (uint64_t)s, kStreamPairs, 0, q0, q1).v[0]);
+        bits[k] = (uint8_t)((u < a_rs ? 1u : 0u) | (u < a_pr ? 2u : 0u) | (u < a_sp ? 4u : 0u));
         if (a < (uint64_t)n) {
             if (k == 0 || (uint32_t)pi[k - 1] != a) A.rowbeg[ro + a] = (uint32_t)k;
             if (k + 1 == P || (uint32_t)pi[k + 1] != a) A.rowend[ro + a] = (uint32_t)(k + 1);
@@ -207,6 +230,28 @@ rps_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* _
             }
         }
     }
+}
+
+// off[s][r] = first index k of step s's sorted list with pair_i[k] >= run_start[r]
+// (binary search; the list is sorted by i and runs are index ranges).
+__global__ void __launch_bounds__(kBlock)
+rps_run_offsets_kernel(int nsteps, int nruns, const int64_t* __restrict__ run_start, const int32_t* __restrict__ pi_all,
+                       const int64_t* __restrict__ npairs, int64_t stride, uint32_t* __restrict__ off) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= (int64_t)nsteps * nruns) return;
+    const int64_t s = t / nruns;
+    const int r = (int)(t - s * nruns);
+    int64_t P = step_pairs(npairs, s, stride);
+    if (P < 0) P = 0;
+    const int32_t* pi = pi_all + s * stride;
+    const int64_t v = run_start[r];
+    int64_t lo = 0, hi = P;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)pi[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    off[t] = (uint32_t)lo;
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -229,9 +274,11 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st) {
 // particles at a time in order, a lane's rank among equal roots from one
 // ballot per root bit, a running count per root in LDS.  Loads go kGroupRounds
 // rounds at a time.  FB fine bits per block: 9 up to 2^19 particles (one
-// radix pass of at most 10 bits covers the rest), 12 up to 2^22.
+// radix pass of at most 10 bits covers the rest), 12 up to 2^22, 14 up to
+// 2^24 (ensembles).
 constexpr uint32_t kGroupFineBits = 9;
 constexpr uint32_t kGroupFineBitsWide = 12;
+constexpr uint32_t kGroupFineBitsHuge = 14;
 constexpr uint32_t kGroupRounds = 8;
 
 template <uint32_t FB>
@@ -831,6 +878,8 @@ struct lm_rps_ws {
     int64_t last_n = 0;       // particles of the last prepare
     int32_t last_steps = 0;   // steps of the last prepare
     int64_t pair_stride_used = 0;  // per-step pair stride of the last prepare
+    uint32_t* run_off = nullptr;   // [steps][runs] first pair of each run (ensembles)
+    int64_t run_off_cap = 0;
 };
 
 template <typename T>
@@ -875,6 +924,7 @@ static void rps_ws_free(lm_rps_ws* w) {
     (void)hipFree(A.plen);
     (void)hipFree(A.depth);
     lm::sort_scratch_free(&w->sort);
+    (void)hipFree(w->run_off);
     rps_free_pairs(w);
     delete w;
 }
@@ -953,9 +1003,10 @@ void lm_rps_ws_destroy(lm_rps_ws* ws) { rps_ws_free(ws); }
 
 int64_t lm_rps_ws_bytes(const lm_rps_ws* ws) { return ws ? ws->bytes : 0; }
 
-int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair_i, const int32_t* pair_j,
-                   int64_t pair_stride, const int64_t* npairs, double pRS, double pPR, double pSP,
-                   uint64_t key0, uint64_t key1, uint64_t step0, void* stream) {
+static int rps_prepare_impl(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair_i, const int32_t* pair_j,
+                            int64_t pair_stride, const int64_t* npairs, double pRS, double pPR, double pSP,
+                            uint64_t key0, uint64_t key1, uint64_t step0, const uint16_t* run, int32_t nruns,
+                            const int64_t* run_start, const lm_run_params* params, void* stream) {
     LM_REQUIRE(ws != nullptr, LM_ERR_ARG, "workspace is NULL");
     LM_REQUIRE(nsteps >= 1 && nsteps <= ws->steps_max, LM_ERR_ARG, "nsteps=%d outside 1..%d of the workspace",
                nsteps, ws->steps_max);
@@ -983,20 +1034,39 @@ int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair
     if (pair_stride > 0) {
         // one thread per pair slot of the capacity (threads past a step's count exit)
         const unsigned npb = lm::grid_for(pair_stride);
-        LM_LAUNCH(lm::K_RPS_UNION, lm::rps_union_kernel, dim3(npb, S), dim3(lm::kBlock), 0, s, n,
-                  pair_i, pair_j, npairs, A, sp, pRS, pPR, pSP, key0, key1, step0);
+        lm::RunsView R{run, params, nullptr, nruns};
+        if (run != nullptr) {
+            const int64_t need = (int64_t)ws->steps_max * nruns;
+            if (ws->run_off_cap < need) {
+                (void)hipFree(ws->run_off);
+                ws->run_off = nullptr;
+                ws->run_off_cap = 0;
+                if ((rc = rws_alloc(&ws->run_off, need, &ws->bytes))) return rc;
+                ws->run_off_cap = need;
+            }
+            LM_LAUNCH(lm::K_RPS_INIT, lm::rps_run_offsets_kernel, dim3(lm::grid_for((int64_t)S * nruns)),
+                      dim3(lm::kBlock), 0, s, (int)S, (int)nruns, run_start, pair_i, npairs, pair_stride, ws->run_off);
+            LM_LAUNCH_CHECK();
+            R.off = ws->run_off;
+            LM_LAUNCH(lm::K_RPS_UNION, lm::rps_union_kernel<true>, dim3(npb, S), dim3(lm::kBlock), 0, s, n, pair_i,
+                      pair_j, npairs, A, sp, pRS, pPR, pSP, key0, key1, step0, R);
+        } else {
+            LM_LAUNCH(lm::K_RPS_UNION, lm::rps_union_kernel<false>, dim3(npb, S), dim3(lm::kBlock), 0, s, n, pair_i,
+                      pair_j, npairs, A, sp, pRS, pPR, pSP, key0, key1, step0, R);
+        }
         LM_LAUNCH_CHECK();
         LM_LAUNCH(lm::K_RPS_KEYS, lm::rps_keys_kernel, dim3(nbp, S), dim3(lm::kBlock), 0, s, n, A, sp);
         LM_LAUNCH_CHECK();
         // grouping by root in two levels: a radix pass on the root's high
         // bits, then a block per super-bucket counts its roots in LDS
         const int kb = key_bits_for(n);
-        const int fb = kb - (int)lm::kGroupFineBits > lm::kMaxDigitBits ? (int)lm::kGroupFineBitsWide
-                                                                          : (int)lm::kGroupFineBits;
+        const int fb = kb - (int)lm::kGroupFineBits <= lm::kMaxDigitBits ? (int)lm::kGroupFineBits
+                       : kb - (int)lm::kGroupFineBitsWide <= lm::kMaxDigitBits ? (int)lm::kGroupFineBitsWide
+                                                                               : (int)lm::kGroupFineBitsHuge;
         const int fine_bits = kb < fb ? kb : fb;
         const int super_bits = kb - fine_bits;
         LM_REQUIRE(super_bits <= lm::kMaxDigitBits, LM_ERR_ARG, "at most 2^%d particles",
-                   (int)lm::kGroupFineBitsWide + lm::kMaxDigitBits);
+                   (int)lm::kGroupFineBitsHuge + lm::kMaxDigitBits);
         uint32_t* pk[2] = {A.pkey, A.ptk};
         uint32_t* pv[2] = {A.pval, A.ptv};
         const uint32_t* dstart = nullptr;
@@ -1012,8 +1082,12 @@ int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair
             LM_LAUNCH(lm::K_RPS_GROUP, lm::rps_group_kernel<lm::kGroupFineBits>, dim3(1u << super_bits, S), dim3(64),
                       0, s, n, A, sp, (const uint32_t*)pk[in], (const uint32_t*)pv[in], pk[1 - in], pv[1 - in],
                       dstart, ws->sort.row, dstride, 1u << fine_bits);
-        } else {
+        } else if (fb == (int)lm::kGroupFineBitsWide) {
             LM_LAUNCH(lm::K_RPS_GROUP, lm::rps_group_kernel<lm::kGroupFineBitsWide>, dim3(1u << super_bits, S),
+                      dim3(64), 0, s, n, A, sp, (const uint32_t*)pk[in], (const uint32_t*)pv[in], pk[1 - in],
+                      pv[1 - in], dstart, ws->sort.row, dstride, 1u << fine_bits);
+        } else {
+            LM_LAUNCH(lm::K_RPS_GROUP, lm::rps_group_kernel<lm::kGroupFineBitsHuge>, dim3(1u << super_bits, S),
                       dim3(64), 0, s, n, A, sp, (const uint32_t*)pk[in], (const uint32_t*)pv[in], pk[1 - in],
                       pv[1 - in], dstart, ws->sort.row, dstride, 1u << fine_bits);
         }
@@ -1040,6 +1114,22 @@ int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair
     ws->last_n = n;
     ws->last_steps = nsteps;
     return LM_OK;
+}
+
+int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair_i, const int32_t* pair_j,
+                   int64_t pair_stride, const int64_t* npairs, double pRS, double pPR, double pSP,
+                   uint64_t key0, uint64_t key1, uint64_t step0, void* stream) {
+    return rps_prepare_impl(ws, nsteps, n, pair_i, pair_j, pair_stride, npairs, pRS, pPR, pSP, key0, key1, step0,
+                            nullptr, 0, nullptr, nullptr, stream);
+}
+
+int lm_rps_prepare_runs(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair_i, const int32_t* pair_j,
+                        int64_t pair_stride, const int64_t* npairs, const uint16_t* run, int32_t nruns,
+                        const int64_t* run_start, const lm_run_params* params, uint64_t step0, void* stream) {
+    LM_REQUIRE(nruns >= 1 && nruns <= 65536, LM_ERR_ARG, "nruns must be in 1..65536");
+    LM_REQUIRE(run && run_start && params, LM_ERR_ARG, "NULL run arrays");
+    return rps_prepare_impl(ws, nsteps, n, pair_i, pair_j, pair_stride, npairs, 0.0, 0.0, 0.0, 0, 0, step0, run,
+                            nruns, run_start, params, stream);
 }
 
 int lm_rps_play(lm_rps_ws* ws, int32_t step, int64_t n, const int8_t* species_in, int8_t* species_out,

@@ -185,14 +185,15 @@ class PairSearch:
     expected; particles outside are still handled exactly.
     """
 
-    def __init__(self, n_max, bbox, max_radius, device=None, steps_max=1):
+    def __init__(self, n_max, bbox, max_radius, device=None, steps_max=1, cell_scale=1.0):
         self.n_max = int(n_max)
         self.steps_max = int(steps_max)
         self.max_radius = float(max_radius)
+        self.cell_scale = float(cell_scale)
         self.bbox = tuple(float(b) for b in bbox)
         h = c_vp()
-        check(lib().lm_pairs_ws_create(self.n_max, self.steps_max, self.bbox[0], self.bbox[1], self.bbox[2],
-                                       self.bbox[3], self.max_radius, ctypes.byref(h)))
+        check(lib().lm_pairs_ws_create_ex(self.n_max, self.steps_max, self.bbox[0], self.bbox[1], self.bbox[2],
+                                          self.bbox[3], self.max_radius, self.cell_scale, ctypes.byref(h)))
         self._ws = h
         self.device = torch.device(device or "cuda")
         self.pair_i = None
@@ -227,9 +228,10 @@ class PairSearch:
         check(lib().lm_query_pairs(self._ws, n, ptr(lon), ptr(lat), float(radius), float(p), ptr(pair_i),
                                    ptr(pair_j), pair_i.numel(), ptr(npairs), stream_ptr()))
 
-    def query_batch(self, lon, lat, radius, p, pair_i, pair_j, npairs):
+    def query_batch(self, lon, lat, radius, p, pair_i, pair_j, npairs, runs=None):
         """Asynchronous, one snapshot per row: lon/lat [S, n] float32, pairs
-        [S, cap] int32 (capacity cap per step), npairs [S] int64."""
+        [S, cap] int32 (capacity cap per step), npairs [S] int64.  With
+        ``runs`` (EnsembleRuns) only pairs within a run are returned."""
         if lon.dim() != 2:
             raise ValueError("lon/lat must be [steps, n]")
         S, n = lon.shape
@@ -242,6 +244,12 @@ class PairSearch:
         require(npairs, torch.int64, min_numel=S, name="npairs")
         if n > self.n_max or S > self.steps_max:
             raise ValueError(f"{S} x {n} exceeds the workspace's {self.steps_max} x {self.n_max}")
+        if runs is not None:
+            runs.check(n)
+            check(lib().lm_query_pairs_runs(self._ws, S, n, ptr(lon), ptr(lat), n, ptr(runs.run), float(radius),
+                                            float(p), ptr(pair_i), ptr(pair_j), pair_i.shape[1], ptr(npairs),
+                                            stream_ptr()))
+            return
         check(lib().lm_query_pairs_batch(self._ws, S, n, ptr(lon), ptr(lat), n, float(radius), float(p),
                                          ptr(pair_i), ptr(pair_j), pair_i.shape[1], ptr(npairs), stream_ptr()))
 
@@ -262,6 +270,48 @@ class PairSearch:
                 return self.pair_i[:P], self.pair_j[:P]
             self.ensure_capacity(P + P // 4 + 1024)
         raise RuntimeError("pair buffer growth did not converge")
+
+
+# ---------------------------------------------------------------------------
+# ensembles
+# ---------------------------------------------------------------------------
+RUN_PARAMS = np.dtype([("key0", "<u8"), ("key1", "<u8"), ("pRS", "<f8"), ("pPR", "<f8"), ("pSP", "<f8")])
+
+
+class EnsembleRuns:
+    """Independent runs sharing one index range: run r holds particles
+    [start[r], start[r+1]) with its own pair probabilities and Philox key.
+
+    sizes: particles per run; params: one {"pRS", "pPR", "pSP"} per run;
+    keys: one (key0, key1) per run.  Device arrays: ``run`` (int16 per
+    particle, read as uint16), ``run_start`` (int64, nruns + 1) and
+    ``params`` (lm_run_params per run).
+    """
+
+    def __init__(self, sizes, params, keys, device=None):
+        sizes = [int(v) for v in sizes]
+        if not sizes or min(sizes) < 0 or len(sizes) > 32767:
+            raise ValueError("1..32767 runs of non-negative size")
+        if len(params) != len(sizes) or len(keys) != len(sizes):
+            raise ValueError("one params dict and one key per run")
+        self.device = torch.device(device or "cuda")
+        self.sizes = sizes
+        self.nruns = len(sizes)
+        self.start = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        self.n = int(self.start[-1])
+        self.run = torch.from_numpy(np.repeat(np.arange(self.nruns, dtype=np.int16), sizes)).to(self.device)
+        self.run_start = torch.from_numpy(self.start).to(self.device)
+        table = np.zeros(self.nruns, dtype=RUN_PARAMS)
+        for r, (p, k) in enumerate(zip(params, keys)):
+            table[r] = (int(k[0]), int(k[1]), float(p["pRS"]), float(p["pPR"]), float(p["pSP"]))
+        self.params = torch.from_numpy(table.view(np.uint8).copy()).to(self.device)
+
+    def check(self, n):
+        if n != self.n:
+            raise ValueError(f"{n} particles, but the runs hold {self.n}")
+
+    def slice(self, r):
+        return slice(int(self.start[r]), int(self.start[r + 1]))
 
 
 # ---------------------------------------------------------------------------
@@ -302,8 +352,9 @@ class RPSUpdater:
                                   pair_i.numel(), float(pRS), float(pPR), float(pSP), int(key[0]),
                                   int(key[1]), int(step), stream_ptr()))
 
-    def prepare(self, n, pair_i, pair_j, npairs, pRS, pPR, pSP, key, step0):
-        """Group the pairs of S steps ([S, cap] buffers, counts npairs[S])."""
+    def prepare(self, n, pair_i, pair_j, npairs, pRS, pPR, pSP, key, step0, runs=None):
+        """Group the pairs of S steps ([S, cap] buffers, counts npairs[S]).
+        With ``runs`` (EnsembleRuns) each run uses its own key and p."""
         if pair_i.dim() != 2:
             raise ValueError("pair buffers must be [steps, capacity]")
         S, cap = pair_i.shape
@@ -312,6 +363,12 @@ class RPSUpdater:
         require(npairs, torch.int64, min_numel=S, name="npairs")
         if n > self.n_max or S > self.steps_max:
             raise ValueError(f"{S} x {n} exceeds the workspace's {self.steps_max} x {self.n_max}")
+        if runs is not None:
+            runs.check(n)
+            check(lib().lm_rps_prepare_runs(self._ws, S, int(n), ptr(pair_i), ptr(pair_j), cap, ptr(npairs),
+                                            ptr(runs.run), runs.nruns, ptr(runs.run_start), ptr(runs.params),
+                                            int(step0), stream_ptr()))
+            return
         check(lib().lm_rps_prepare(self._ws, S, int(n), ptr(pair_i), ptr(pair_j), cap, ptr(npairs), float(pRS),
                                    float(pPR), float(pSP), int(key[0]), int(key[1]), int(step0), stream_ptr()))
 

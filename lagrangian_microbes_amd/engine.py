@@ -31,13 +31,15 @@ class InteractionEngine:
     addressed by the ``*_set`` methods; ``run_chunk`` uses set 0.
     """
 
-    def __init__(self, n, bbox, radius, norm=2.0, device=None, steps_max=1, initial_capacity=None, nsets=1):
+    def __init__(self, n, bbox, radius, norm=2.0, device=None, steps_max=1, initial_capacity=None, nsets=1,
+                 runs=None, cell_scale=1.0):
         self.n = int(n)
         self.radius = float(radius)
         self.norm = float(norm)
         self.steps_max = int(steps_max)
         self.device = torch.device(device or "cuda")
-        self.search = dev.PairSearch(self.n, bbox, self.radius, self.device, self.steps_max)
+        self.runs = runs  # EnsembleRuns: independent runs, each with its own key and p
+        self.search = dev.PairSearch(self.n, bbox, self.radius, self.device, self.steps_max, cell_scale)
         self.nsets = int(nsets)
         self.rps_sets = [dev.RPSUpdater(self.n, self.device, self.steps_max) for _ in range(self.nsets)]
         self.npairs_sets = [torch.zeros(self.steps_max, dtype=torch.int64, device=self.device)
@@ -78,8 +80,9 @@ class InteractionEngine:
             raise ValueError(f"{S} steps exceed the engine's batch of {self.steps_max}")
         pi, pj = (t[:S] for t in self.pair_sets[b])
         cnt = self.npairs_sets[b][:S]
-        self.search.query_batch(rec_x, rec_y, self.radius, self.norm, pi, pj, cnt)
-        self.rps_sets[b].prepare(self.n, pi, pj, cnt, params["pRS"], params["pPR"], params["pSP"], key, step0)
+        self.search.query_batch(rec_x, rec_y, self.radius, self.norm, pi, pj, cnt, self.runs)
+        self.rps_sets[b].prepare(self.n, pi, pj, cnt, params["pRS"], params["pPR"], params["pSP"], key, step0,
+                                 self.runs)
 
     def worst_count(self, b, S):
         """Largest pair count of set b's S steps (synchronises the stream)."""

@@ -35,11 +35,14 @@ class FusedRunner:
     params: {"pRS", "pPR", "pSP"}; key: Philox key of the pair uniforms;
     walk_key / Kh (deg^2/s): the random walk (off when Kh == 0);
     precision: "f64" (the FP64 path) or "f32" (float32 field, interpolation
-    and velocities; float64 positions and pair test).
+    and velocities; float64 positions and pair test); runs: an
+    EnsembleRuns splitting the particles into independent runs (then each
+    run's own p and key apply and params/key are unused); cell_scale: the
+    pair search's cell side over the radius.
     """
 
     def __init__(self, lon0, lat0, species0, field, radius, params, key, dt_s, norm=2.0, chunk=100, Kh=0.0,
-                 walk_key=(0, 0x6B68), device=None, bbox=None, precision="f64"):
+                 walk_key=(0, 0x6B68), device=None, bbox=None, precision="f64", runs=None, cell_scale=1.0):
         self.device = torch.device(device or "cuda")
         self.n = int(np.asarray(lon0).size)
         self.lon0 = np.asarray(lon0, dtype=np.float64)
@@ -53,7 +56,7 @@ class FusedRunner:
         self.walk_key = tuple(walk_key)
         self.dfield = dev.DeviceField(field, self.device, precision)
         self.engine = InteractionEngine(self.n, bbox or field.bounds, radius, norm, self.device,
-                                        steps_max=self.chunk, nsets=2)
+                                        steps_max=self.chunk, nsets=2, runs=runs, cell_scale=cell_scale)
         self.rec_x = torch.empty((self.chunk, self.n), dtype=torch.float32, device=self.device)
         self.rec_y = torch.empty((self.chunk, self.n), dtype=torch.float32, device=self.device)
         self.rec_s = torch.empty((self.chunk, self.n), dtype=torch.int8, device=self.device)

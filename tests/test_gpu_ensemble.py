@@ -1,0 +1,85 @@
+"""Ensembles on the GPU: independent runs of uneven size in one index range
+(lm_query_pairs_runs, lm_rps_prepare_runs, FusedRunner(runs=...)).
+
+Every run of the batched ensemble must evolve exactly as that run alone
+through the plain FP32 path: the same float32 records, the same pair counts
+and the same species at every step (each run keeps its own p and Philox key,
+and the uniforms are ranked within the run).  Pairs never cross runs, also
+when different runs occupy the same cells.
+"""
+import numpy as np
+import pytest
+import torch
+
+from lagrangian_microbes_amd import configs, device as dev
+from lagrangian_microbes_amd.simulator import FusedRunner
+
+pytestmark = pytest.mark.gpu
+
+
+def run_alone(w, field, chunk):
+    lon, lat = w.particles()
+    r = FusedRunner(lon, lat, w.species(), field, w.radius, w.params, w.key, w.dt, chunk=chunk, precision="f32")
+    recs = []
+
+    def keep(k, rx, ry, rs, cnt):
+        recs.append((rx.cpu().numpy().copy(), rs.cpu().numpy().copy(), cnt.cpu().numpy().copy()))
+
+    r.run(w.steps, keep)
+    r.check_domain()
+    rx = np.concatenate([a for a, _, _ in recs])
+    rs = np.concatenate([b for _, b, _ in recs])
+    cnt = np.concatenate([c for _, _, c in recs])
+    return rx, rs, cnt
+
+
+@pytest.mark.parametrize("cell_scale,chunk", [(1.0, 10), (2.0, 24)])
+def test_ensemble_matches_runs_alone(gpu, cell_scale, chunk):
+    e = configs.ensemble_small()
+    field = e.velocity_field()
+    lon, lat = e.particles()
+    runs = dev.EnsembleRuns([w.n for w in e.runs], [w.params for w in e.runs], [w.key for w in e.runs])
+    r = FusedRunner(lon, lat, e.species(), field, e.runs[0].radius, {"pRS": 0, "pPR": 0, "pSP": 0}, (0, 0),
+                    e.runs[0].dt, chunk=chunk, precision="f32", runs=runs, cell_scale=cell_scale)
+    recs = []
+
+    def keep(k, rx, ry, rs, cnt):
+        recs.append((rx.cpu().numpy().copy(), rs.cpu().numpy().copy(), cnt.cpu().numpy().copy()))
+
+    r.run(e.steps, keep)
+    r.check_domain()
+    ex = np.concatenate([a for a, _, _ in recs])
+    es = np.concatenate([b for _, b, _ in recs])
+    ecnt = np.concatenate([c for _, _, c in recs])
+    total = np.zeros(e.steps, dtype=np.int64)
+    for k, w in enumerate(e.runs):
+        rx, rs, cnt = run_alone(w, field, chunk)
+        sl = runs.slice(k)
+        assert np.array_equal(ex[:, sl], rx), k
+        assert np.array_equal(es[:, sl], rs), k
+        total += cnt
+    assert np.array_equal(ecnt, total)
+    assert total.sum() > 1000  # the runs do interact
+
+
+def test_pairs_never_cross_runs(gpu):
+    # two runs on the same lattice: every particle has a twin at the same position in the other run
+    w = configs.rps_10k_steady()
+    lon, lat = w.particles()
+    lon2, lat2 = np.concatenate([lon, lon]), np.concatenate([lat, lat])
+    x = torch.from_numpy(lon2.astype(np.float32)).cuda()[None]
+    y = torch.from_numpy(lat2.astype(np.float32)).cuda()[None]
+    runs = dev.EnsembleRuns([w.n, w.n], [w.params, w.params], [w.key, w.key])
+    s = dev.PairSearch(2 * w.n, w.velocity_field().bounds, 0.02)
+    pi = torch.empty((1, 400000), dtype=torch.int32, device="cuda")
+    pj = torch.empty_like(pi)
+    cnt = torch.zeros(1, dtype=torch.int64, device="cuda")
+    s.query_batch(x, y, 0.02, 2.0, pi, pj, cnt, runs)
+    P = int(cnt.item())
+    a, b = pi[0, :P].cpu().numpy(), pj[0, :P].cpu().numpy()
+    assert P > 0 and np.all((a < w.n) == (b < w.n))
+    # each run finds the pairs it finds alone; the second run's are the first's shifted by n
+    one, _ = dev.PairSearch(w.n, w.velocity_field().bounds, 0.02).query_sync(x[0, :w.n].contiguous(),
+                                                                             y[0, :w.n].contiguous(), 0.02)
+    m = a < w.n
+    assert np.array_equal(a[m], one.cpu().numpy()) and np.array_equal(a[~m], one.cpu().numpy() + w.n)

@@ -131,3 +131,16 @@ def test_factory_species():
 def test_pretty():
     assert utils.pretty_time(2e-9) == "2 ns" and utils.pretty_time(90) == "1.5 mins"
     assert utils.pretty_filesize(2048) == "2.0 KiB"
+
+
+def test_ensemble_runs_layout():
+    from lagrangian_microbes_amd.device import EnsembleRuns, RUN_PARAMS
+    r = EnsembleRuns([3, 0, 2], [{"pRS": 0.5, "pPR": 0.6, "pSP": 0.7}] * 3, [(100, 0), (101, 1), (102, 2)],
+                     device="cpu")
+    assert r.n == 5 and list(r.start) == [0, 3, 3, 5]
+    assert r.run.tolist() == [0, 0, 0, 2, 2]
+    # lm_run_params: two uint64 keys and three doubles, 40 bytes per run
+    assert RUN_PARAMS.itemsize == 40 and r.params.numel() == 3 * 40
+    t = r.params.numpy().view(RUN_PARAMS)
+    assert t["key0"].tolist() == [100, 101, 102] and t["pSP"].tolist() == [0.7] * 3
+    assert r.slice(2) == slice(3, 5)
