@@ -17,7 +17,9 @@ targets from a second seeded field (recompute plus reverse sweep);
 ``adjoint_forward_ratio`` is backward time over forward time.  The
 ``fp32_path`` object runs the FP32 path's workload (4M particles, 2,880
 half-hour steps, 1/12-degree field) in FP32 and FP64 from identical inputs
-and reports both rates and the distance between their positions.
+and reports both rates and the distance between their positions.  The
+``ensemble`` object runs 64 independent runs (14.7M particles) batched with
+the autotuner's launch shape against the same runs one at a time.
 """
 import argparse
 import json
@@ -50,6 +52,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint forward/backward timing")
     ap.add_argument("--adjoint-reps", type=int, default=3)
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-path run at 4M particles")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-run ensemble")
     ap.add_argument("--fp32-steps", type=int, default=2880)
     ap.add_argument("--cpu-steps", type=int, default=48)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -229,6 +232,70 @@ def fp32_leg(steps):
             "f32": out["f32"], "f64": out["f64"], "f32_vs_f64_position_error": err}
 
 
+def ensemble_leg():
+    """The 64-run ensemble (14.7M particles, uneven runs, 240 hourly steps,
+    FP32): the launch shape picked by the autotuner, the batched run timed
+    from the initial state, then every run alone through the plain FP32 path
+    (one FusedRunner per run, setup outside the timing), for the speedup."""
+    import torch
+    from lagrangian_microbes_amd import autotune, configs, device as dev
+    from lagrangian_microbes_amd.simulator import FusedRunner
+
+    e = configs.ensemble_64runs()
+    field = e.velocity_field()
+    lon, lat = e.particles()
+    species = e.species()
+    w0 = e.runs[0]
+
+    def make(cell_scale, chunk, play_blocks):
+        runs = dev.EnsembleRuns([w.n for w in e.runs], [w.params for w in e.runs], [w.key for w in e.runs])
+        return FusedRunner(lon, lat, species, field, w0.radius, {"pRS": 0, "pPR": 0, "pSP": 0}, (0, 0), w0.dt,
+                           chunk=chunk, precision="f32", runs=runs, cell_scale=cell_scale, play_blocks=play_blocks,
+                           graphs=True)
+
+    log(f"[bench] ensemble {e.n} particles in {len(e.runs)} runs: autotuning")
+    best, table = autotune.tune(make, steps=24)
+    log(f"[bench] ensemble autotuner picked {best}")
+    r = make(**best)
+    r.run(best["chunk"])
+    torch.cuda.synchronize()
+    r.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r.run(e.steps)
+    torch.cuda.synchronize()
+    batched = time.perf_counter() - t0
+    oob = int(r.oob.item())
+    del r
+    torch.cuda.empty_cache()
+    per_run = []
+    for w in e.runs:
+        lo, la = w.particles()
+        rr = FusedRunner(lo, la, w.species(), field, w.radius, w.params, w.key, w.dt, precision="f32")
+        rr.run(24)
+        torch.cuda.synchronize()
+        rr.reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rr.run(w.steps)
+        torch.cuda.synchronize()
+        per_run.append(time.perf_counter() - t0)
+        del rr
+    torch.cuda.empty_cache()
+    seq = float(sum(per_run))
+    ms = np.array(per_run) * 1e3 / e.steps
+    log(f"[bench] ensemble: batched {batched:.3f}s, runs alone {seq:.3f}s, speedup {seq / batched:.2f}")
+    return {**e.describe(), "autotuned": {**best, "play_blocks": list(best["play_blocks"])},
+            "tuning_ms_per_step": [[c["cell_scale"], c["chunk"], list(c["play_blocks"]), round(t * 1e3, 4)]
+                                   for c, t in table],
+            "ms_per_step": round(1e3 * batched / e.steps, 4), "particle_steps_per_s": e.n * e.steps / batched,
+            "particle_steps_outside_field": oob, "runs_alone_s": round(seq, 4),
+            "runs_alone_particle_steps_per_s": e.n * e.steps / seq,
+            "runs_alone_ms_per_step": {"min": round(float(ms.min()), 4), "median": round(float(np.median(ms)), 4),
+                                       "max": round(float(ms.max()), 4)},
+            "speedup_vs_runs_alone": round(seq / batched, 3)}
+
+
 def adjoint_leg(reps):
     """The adjoint workload (1M particles, 240 steps, 20 float32 levels): times the
     differentiable forward and loss.backward() with events on the stream they run on,
@@ -380,6 +447,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_fp32:
         result["fp32_path"] = fp32_leg(args.fp32_steps)
+    if not args.no_ensemble:
+        result["ensemble"] = ensemble_leg()
     if not args.no_adjoint:
         result["adjoint"] = adjoint_leg(args.adjoint_reps)
         result["adjoint_forward_ratio"] = result["adjoint"]["ratio"]

@@ -209,6 +209,13 @@ int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair
                    const int32_t* pair_j, int64_t pair_stride, const int64_t* npairs,
                    double pRS, double pPR, double pSP, uint64_t key0, uint64_t key1,
                    uint64_t step0, void* stream);
+/* Launch shape of lm_rps_play: workgroups (64 lanes) that take wide
+ * components (one wavefront each, by level) and that take small components
+ * (one lane each); both loop over their components, so any counts >= 1 give
+ * the same result (defaults 2048 and 2048).  No reference counterpart (set by
+ * the ensemble's autotuner). */
+int lm_rps_ws_set_play_blocks(lm_rps_ws* ws, int32_t wide_blocks, int32_t narrow_blocks);
+
 /* Per-run parameters of an ensemble (a device array, one entry per run): the
  * Philox key and the forward probabilities. */
 typedef struct lm_run_params {

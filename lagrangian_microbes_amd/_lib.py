@@ -55,6 +55,7 @@ _SIGNATURES = {
     "lm_rps_prepare_runs": (ctypes.c_int, [c_vp, c_i32, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_i32, c_vp, c_vp,
                                            c_u64, c_vp]),
     "lm_rps_play": (ctypes.c_int, [c_vp, c_i32, c_i64, c_vp, c_vp, c_vp]),
+    "lm_rps_ws_set_play_blocks": (ctypes.c_int, [c_vp, c_i32, c_i32]),
     "lm_rps_component_stats": (ctypes.c_int, [c_vp, c_i32, c_vp, c_vp]),
     "lm_philox_uniforms": (ctypes.c_int, [c_i64, c_u64, c_u64, c_u64, c_u64, c_u64, c_u64, c_i32, c_vp, c_vp]),
     "lm_adjoint_forward": (ctypes.c_int, [c_i64, c_vp, c_vp, c_vp, c_i32, ctypes.POINTER(LmGrid), c_vp, c_vp, c_vp,

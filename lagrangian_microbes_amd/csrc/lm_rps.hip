@@ -880,6 +880,8 @@ struct lm_rps_ws {
     int64_t pair_stride_used = 0;  // per-step pair stride of the last prepare
     uint32_t* run_off = nullptr;   // [steps][runs] first pair of each run (ensembles)
     int64_t run_off_cap = 0;
+    unsigned play_wide = lm::kPlayWideBlocks;      // replay workgroups on wide components
+    unsigned play_narrow = lm::kPlayNarrowBlocks;  // replay workgroups on small components (64 each)
 };
 
 template <typename T>
@@ -1142,12 +1144,21 @@ int lm_rps_play(lm_rps_ws* ws, int32_t step, int64_t n, const int8_t* species_in
     LM_REQUIRE(species_in && species_out, LM_ERR_ARG, "NULL species");
     hipStream_t s = (hipStream_t)stream;
     lm::RpsStrides sp{ws->row, ws->pair_stride_used};
-    const unsigned nwide = ws->pair_stride_used > 0 ? lm::kPlayWideBlocks : 0u;
-    const unsigned nnarrow = ws->pair_stride_used > 0 ? lm::kPlayNarrowBlocks : 0u;
+    const unsigned nwide = ws->pair_stride_used > 0 ? ws->play_wide : 0u;
+    const unsigned nnarrow = ws->pair_stride_used > 0 ? ws->play_narrow : 0u;
     const unsigned ncopy = (unsigned)((n + 64 * lm::kCopyItems - 1) / (64 * lm::kCopyItems));
     LM_LAUNCH(lm::K_RPS_PLAY, lm::rps_play_kernel, dim3(nwide + nnarrow + ncopy), dim3(64), 0, s, n,
               (int64_t)step, ws->A, sp, species_in, species_out, nwide, nnarrow);
     LM_LAUNCH_CHECK();
+    return LM_OK;
+}
+
+int lm_rps_ws_set_play_blocks(lm_rps_ws* ws, int32_t wide_blocks, int32_t narrow_blocks) {
+    LM_REQUIRE(ws != nullptr, LM_ERR_ARG, "workspace is NULL");
+    LM_REQUIRE(wide_blocks >= 1 && wide_blocks <= 65536 && narrow_blocks >= 1 && narrow_blocks <= 65536, LM_ERR_ARG,
+               "block counts must be in 1..65536");
+    ws->play_wide = (unsigned)wide_blocks;
+    ws->play_narrow = (unsigned)narrow_blocks;
     return LM_OK;
 }
 

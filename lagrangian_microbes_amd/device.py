@@ -372,6 +372,10 @@ class RPSUpdater:
         check(lib().lm_rps_prepare(self._ws, S, int(n), ptr(pair_i), ptr(pair_j), cap, ptr(npairs), float(pRS),
                                    float(pPR), float(pSP), int(key[0]), int(key[1]), int(step0), stream_ptr()))
 
+    def set_play_blocks(self, wide, narrow):
+        """Launch shape of the replay (see lm_rps_ws_set_play_blocks)."""
+        check(lib().lm_rps_ws_set_play_blocks(self._ws, int(wide), int(narrow)))
+
     def play(self, step, species_in, species_out):
         """Step `step` of the last prepare: species_out = species_in after its pairs."""
         n = species_in.numel()
@@ -396,13 +400,17 @@ class KernelTimer:
     """Context manager: every liblm launch inside is timed with events on its
     own dispatch; ``summary()`` returns {kernel: (launches, total_ms)}."""
 
+    active = False  # launches carry timing events: no graph capture meanwhile
+
     def __enter__(self):
         check(lib().lm_timing_enable(1))
+        KernelTimer.active = True
         return self
 
     def __exit__(self, *exc):
         self._summary = self._read()
         check(lib().lm_timing_enable(0))
+        KernelTimer.active = False
         return False
 
     @staticmethod

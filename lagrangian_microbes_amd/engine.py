@@ -32,7 +32,7 @@ class InteractionEngine:
     """
 
     def __init__(self, n, bbox, radius, norm=2.0, device=None, steps_max=1, initial_capacity=None, nsets=1,
-                 runs=None, cell_scale=1.0):
+                 runs=None, cell_scale=1.0, play_blocks=None, graphs=False):
         self.n = int(n)
         self.radius = float(radius)
         self.norm = float(norm)
@@ -46,6 +46,13 @@ class InteractionEngine:
                             for _ in range(self.nsets)]
         self.pair_sets = [None] * self.nsets
         self.capacity = 0
+        if play_blocks is not None:
+            for r in self.rps_sets:
+                r.set_play_blocks(*play_blocks)
+        # the per-step replay launches of a batch, captured once per (set, steps, buffers)
+        # into a HIP graph and replayed (graphs=True)
+        self.graphs = bool(graphs)
+        self._graphs = {}
         self._grow(initial_capacity or max(2 * self.n, 1024))
 
     # set 0 under the single-batch names
@@ -69,6 +76,7 @@ class InteractionEngine:
         cap = int(cap)
         if cap > self.capacity:
             self.capacity = cap
+            self._graphs = {}  # captured launches address the old buffers
             self.pair_sets = [(torch.empty((self.steps_max, cap), dtype=torch.int32, device=self.device),
                                torch.empty((self.steps_max, cap), dtype=torch.int32, device=self.device))
                               for _ in range(self.nsets)]
@@ -99,12 +107,29 @@ class InteractionEngine:
     def play_set(self, b, S, species, rec_species):
         """Replay set b's S steps: rec_species[s] = species after step s, and
         species ends as the state after the last step (asynchronous)."""
+        if S <= 0:
+            return
+        if self.graphs and not dev.KernelTimer.active:
+            # The launches' arguments are the workspace's arrays, the species
+            # buffers and the pair capacity, fixed for a key; only their contents
+            # change between replays.  The capture stream holds no memset node.
+            key = (b, S, species.data_ptr(), rec_species.data_ptr(), self.capacity)
+            g = self._graphs.get(key)
+            if g is None:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    self._play_steps(b, S, species, rec_species)
+                self._graphs[key] = g
+            g.replay()
+        else:
+            self._play_steps(b, S, species, rec_species)
+        species.copy_(rec_species[S - 1])
+
+    def _play_steps(self, b, S, species, rec_species):
         sp_in = species
         for s in range(S):
             self.rps_sets[b].play(s, sp_in, rec_species[s])
             sp_in = rec_species[s]
-        if S > 0:
-            species.copy_(rec_species[S - 1])
 
     def run_chunk(self, rec_x, rec_y, species, params, key, step0, rec_species=None, npairs=None):
         """Steps s = 0..S-1 over recorded float32 positions rec_x[s], rec_y[s] ([S, n]).

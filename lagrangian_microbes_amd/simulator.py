@@ -38,11 +38,14 @@ class FusedRunner:
     and velocities; float64 positions and pair test); runs: an
     EnsembleRuns splitting the particles into independent runs (then each
     run's own p and key apply and params/key are unused); cell_scale: the
-    pair search's cell side over the radius.
+    pair search's cell side over the radius; play_blocks: (wide, narrow)
+    workgroups of the replay; graphs: replay each chunk's per-step launches
+    from a captured HIP graph.
     """
 
     def __init__(self, lon0, lat0, species0, field, radius, params, key, dt_s, norm=2.0, chunk=100, Kh=0.0,
-                 walk_key=(0, 0x6B68), device=None, bbox=None, precision="f64", runs=None, cell_scale=1.0):
+                 walk_key=(0, 0x6B68), device=None, bbox=None, precision="f64", runs=None, cell_scale=1.0,
+                 play_blocks=None, graphs=False):
         self.device = torch.device(device or "cuda")
         self.n = int(np.asarray(lon0).size)
         self.lon0 = np.asarray(lon0, dtype=np.float64)
@@ -56,7 +59,8 @@ class FusedRunner:
         self.walk_key = tuple(walk_key)
         self.dfield = dev.DeviceField(field, self.device, precision)
         self.engine = InteractionEngine(self.n, bbox or field.bounds, radius, norm, self.device,
-                                        steps_max=self.chunk, nsets=2, runs=runs, cell_scale=cell_scale)
+                                        steps_max=self.chunk, nsets=2, runs=runs, cell_scale=cell_scale,
+                                        play_blocks=play_blocks, graphs=graphs)
         self.rec_x = torch.empty((self.chunk, self.n), dtype=torch.float32, device=self.device)
         self.rec_y = torch.empty((self.chunk, self.n), dtype=torch.float32, device=self.device)
         self.rec_s = torch.empty((self.chunk, self.n), dtype=torch.int8, device=self.device)

@@ -83,3 +83,24 @@ def test_pairs_never_cross_runs(gpu):
                                                                              y[0, :w.n].contiguous(), 0.02)
     m = a < w.n
     assert np.array_equal(a[m], one.cpu().numpy()) and np.array_equal(a[~m], one.cpu().numpy() + w.n)
+
+
+def test_graph_replay_matches_eager(gpu):
+    # the same ensemble with its per-step replay launches captured once per chunk shape and replayed:
+    # 24 steps in chunks of 5 replay the full-chunk graph four times (with host reads in between) and
+    # the last, shorter chunk's once; every step's species must equal the eager run's
+    e = configs.ensemble_small()
+    field = e.velocity_field()
+    lon, lat = e.particles()
+    out = {}
+    for graphs in (False, True):
+        runs = dev.EnsembleRuns([w.n for w in e.runs], [w.params for w in e.runs], [w.key for w in e.runs])
+        r = FusedRunner(lon, lat, e.species(), field, e.runs[0].radius, {"pRS": 0, "pPR": 0, "pSP": 0}, (0, 0),
+                        e.runs[0].dt, chunk=5, precision="f32", runs=runs, graphs=graphs, play_blocks=(64, 32))
+        recs = []
+        r.run(e.steps, lambda k, rx, ry, rs, cnt: recs.append(rs.cpu().numpy().copy()))
+        out[graphs] = np.concatenate(recs)
+        if graphs:
+            # the two buffer sets' 5-step chunks and the last 4-step chunk
+            assert len(r.engine._graphs) == 3
+    assert np.array_equal(out[True], out[False])

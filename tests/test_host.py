@@ -144,3 +144,19 @@ def test_ensemble_runs_layout():
     t = r.params.numpy().view(RUN_PARAMS)
     assert t["key0"].tolist() == [100, 101, 102] and t["pSP"].tolist() == [0.7] * 3
     assert r.slice(2) == slice(3, 5)
+
+
+def test_autotune_picks_fastest_candidate():
+    from lagrangian_microbes_amd import autotune
+    cands = autotune.candidates()
+    assert len(cands) == 27 and all(c["cell_scale"] >= 1.0 for c in cands)
+    seen = []
+
+    def fake_timer(make_runner, c, steps):
+        seen.append(c)
+        # fastest at cell scale 1.5, chunk 12, replay (1024, 1024)
+        return abs(c["cell_scale"] - 1.5) + abs(c["chunk"] - 12) / 10 + (c["play_blocks"] != (1024, 1024))
+
+    best, table = autotune.tune(None, steps=4, cands=cands, timer=fake_timer)
+    assert best == {"cell_scale": 1.5, "chunk": 12, "play_blocks": (1024, 1024)}
+    assert len(table) == 27 and seen == cands
