@@ -236,7 +236,8 @@ def ensemble_leg():
     """The 64-run ensemble (14.7M particles, uneven runs, 240 hourly steps,
     FP32): the launch shape picked by the autotuner, the batched run timed
     from the initial state, then every run alone through the plain FP32 path
-    (one FusedRunner per run, setup outside the timing), for the speedup."""
+    (one FusedRunner per run, setup outside the timing, best of two), for the
+    speedup."""
     import torch
     from lagrangian_microbes_amd import autotune, configs, device as dev
     from lagrangian_microbes_amd.simulator import FusedRunner
@@ -274,12 +275,15 @@ def ensemble_leg():
         rr = FusedRunner(lo, la, w.species(), field, w.radius, w.params, w.key, w.dt, precision="f32")
         rr.run(24)
         torch.cuda.synchronize()
-        rr.reset()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        rr.run(w.steps)
-        torch.cuda.synchronize()
-        per_run.append(time.perf_counter() - t0)
+        best = float("inf")
+        for _ in range(2):  # best of two: a run alone now and then stalls for ~1 s outside its kernels
+            rr.reset()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rr.run(w.steps)
+            torch.cuda.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        per_run.append(best)
         del rr
     torch.cuda.empty_cache()
     seq = float(sum(per_run))
