@@ -14,12 +14,15 @@ particles over 240 hourly steps through 20 float32 field levels
 (lagrangian_microbes_amd.adjoint): the forward with checkpoints every 16
 steps, and loss.backward() of the mean squared great-circle distance to
 targets from a second seeded field (recompute plus reverse sweep);
-``adjoint_forward_ratio`` is backward time over forward time.  The
-``fp32_path`` object runs the FP32 path's workload (4M particles, 2,880
-half-hour steps, 1/12-degree field) in FP32 and FP64 from identical inputs
-and reports both rates and the distance between their positions.  The
-``ensemble`` object runs 64 independent runs (14.7M particles) batched with
-the autotuner's launch shape against the same runs one at a time.
+``adjoint_forward_ratio`` is backward time over forward time.
+
+Opt-in legs (their kernels share names with the 490k run's, so they stay out
+of the default run that the kernel-trace summary profiles): ``--fp32`` adds
+the FP32 path's workload (4M particles, 2,880 half-hour steps, 1/12-degree
+field) in FP32 and FP64 from identical inputs, with both rates and the
+distance between their positions; ``--ensemble`` adds 64 independent runs
+(14.7M particles) batched with the autotuner's launch shape against the same
+runs one at a time.
 """
 import argparse
 import json
@@ -51,8 +54,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint forward/backward timing")
     ap.add_argument("--adjoint-reps", type=int, default=3)
-    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-path run at 4M particles")
-    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-run ensemble")
+    ap.add_argument("--fp32", action="store_true", help="also run the FP32 path at 4M particles (2,880 steps)")
+    ap.add_argument("--ensemble", action="store_true", help="also run the 64-run ensemble")
     ap.add_argument("--fp32-steps", type=int, default=2880)
     ap.add_argument("--cpu-steps", type=int, default=48)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -449,9 +452,9 @@ def main():
         result["cpu_baseline"] = None
     del runner
     torch.cuda.empty_cache()
-    if not args.no_fp32:
+    if args.fp32:
         result["fp32_path"] = fp32_leg(args.fp32_steps)
-    if not args.no_ensemble:
+    if args.ensemble:
         result["ensemble"] = ensemble_leg()
     if not args.no_adjoint:
         result["adjoint"] = adjoint_leg(args.adjoint_reps)
