@@ -1,12 +1,16 @@
 """Short text summaries of rocprofv3 CSV output for liblm kernels.
 
     python -m lagrangian_microbes_amd.profile_summary trace DIR [--title T] [--command C]
+        [--passes KERNEL=N1,N2,...[:NAME1,NAME2,...]]
     python -m lagrangian_microbes_amd.profile_summary traffic FETCH_DIR WRITE_DIR \
         COPY_FETCH_DIR COPY_WRITE_DIR COPY_BYTES [--json OUT] [--command C]
 
 ``trace`` reads the ``*_kernel_stats.csv`` of a ``rocprofv3 --kernel-trace
 --stats --output-format csv`` run and prints a markdown table of the liblm
-kernels (calls, average and total duration).
+kernels (calls, average and total duration).  ``--passes`` splits one
+kernel's dispatches, in launch order (``*_kernel_trace.csv``), into
+consecutive groups of N1, N2, ... launches and prints each group's average:
+the passes of a benchmark run over one workload.
 
 ``traffic`` reads four ``rocprofv3 --pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE``
 runs (``*_counter_collection.csv``): the workload and a plain copy of
@@ -72,6 +76,23 @@ def trace(args):
     for tot, k, calls, avg in sorted(rows, reverse=True):
         out.append(f"| {k} | {TIMER_NAMES.get(k, '')} | {calls} | {avg / 1e3:.2f} | {tot / 1e6:.2f} | "
                    f"{100 * tot / total:.1f}% |")
+    if args.passes:
+        kern, spec = args.passes.split("=", 1)
+        counts, _, names = spec.partition(":")
+        counts = [int(c) for c in counts.split(",")]
+        names = names.split(",") if names else [f"pass {k}" for k in range(len(counts))]
+        rows_k = [r for r in csv.DictReader(open(one_file(args.dir, "*_kernel_trace.csv")))
+                  if short_name(r["Kernel_Name"]) == kern]
+        rows_k.sort(key=lambda r: int(r["Start_Timestamp"]))
+        dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in rows_k]
+        out += ["", f"{kern} by pass ({len(dur)} dispatches in launch order):", "",
+                "| pass | dispatches | avg (us) |", "|---|---:|---:|"]
+        i = 0
+        for c, nm in zip(counts, names):
+            seg = dur[i:i + c]
+            i += c
+            if seg:
+                out.append(f"| {nm} | {len(seg)} | {sum(seg) / len(seg):.2f} |")
     print("\n".join(out))
 
 
@@ -125,6 +146,7 @@ def main():
     t.add_argument("dir")
     t.add_argument("--title", default="Kernel trace")
     t.add_argument("--command", default="")
+    t.add_argument("--passes", default="")
     c = sub.add_parser("traffic")
     for a in ("fetch_dir", "write_dir", "copy_fetch_dir", "copy_write_dir"):
         c.add_argument(a)
