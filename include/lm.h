@@ -260,6 +260,28 @@ int lm_philox_uniforms(int64_t count, uint64_t c0, uint64_t c1, uint64_t c2, uin
                        uint64_t key0, uint64_t key1, int32_t word, double* out, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Reductions for long runs                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* counts[s][c] += number of particles of row s (species + s*stride, int8,
+ * n entries) with code c for c = 1 (rock), 2 (paper), 3 (scissors), and
+ * with any other code for c = 0 (device uint64 [nsteps][4]).  Replaces the
+ * per-time counts of species_count_figure (analysis.py:31-35). */
+int lm_species_counts(const int8_t* species, int64_t n, int32_t nsteps, int64_t stride, uint64_t* counts,
+                      void* stream);
+
+/* hist[b] += number of unordered pairs i < j whose haversine term
+ *   a = sin^2((lat_j - lat_i)/2) + cos(lat_i) cos(lat_j) sin^2((lon_j - lon_i)/2)
+ * (degrees; float32 positions widened to double) has exactly b of the
+ * nbins - 1 increasing thresholds edges[] <= a (device double).  With
+ * thresholds a_b = sin^2(d_b / (2 R)) this is the histogram of the pairwise
+ * great-circle distances over the edges d_b.  Replaces the pairwise distance
+ * histogram of sandbox/pairwise_distance_histogram_gpu.jl:14-43 (whose
+ * binning, round(10 log10(d)), the caller's edges reproduce). */
+int lm_distance_histogram(int64_t n, const float* lon, const float* lat, int32_t nbins, const double* edges,
+                          uint64_t* hist, void* stream);
+
+/* ------------------------------------------------------------------------ */
 /* Adjoint of the RK4 advection with respect to a float32 gridded field and */
 /* the initial positions.  No reference counterpart: the reference only     */
 /* advects forward (AdvectionRK4 through parcels, particle_advecter.py:     */

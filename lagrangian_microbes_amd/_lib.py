@@ -58,6 +58,8 @@ _SIGNATURES = {
     "lm_rps_ws_set_play_blocks": (ctypes.c_int, [c_vp, c_i32, c_i32]),
     "lm_rps_component_stats": (ctypes.c_int, [c_vp, c_i32, c_vp, c_vp]),
     "lm_philox_uniforms": (ctypes.c_int, [c_i64, c_u64, c_u64, c_u64, c_u64, c_u64, c_u64, c_i32, c_vp, c_vp]),
+    "lm_species_counts": (ctypes.c_int, [c_vp, c_i64, c_i32, c_i64, c_vp, c_vp]),
+    "lm_distance_histogram": (ctypes.c_int, [c_i64, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
     "lm_adjoint_forward": (ctypes.c_int, [c_i64, c_vp, c_vp, c_vp, c_i32, ctypes.POINTER(LmGrid), c_vp, c_vp, c_vp,
                                           c_i32, c_dbl, c_i32, c_vp, c_vp, c_i64, c_vp, c_vp]),
     "lm_adjoint_scratch_bytes": (c_i64, [c_i64, c_i32, ctypes.POINTER(LmGrid), c_i32]),

@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblm.so")
-SOURCES = ["lm_core.hip", "lm_advect.hip", "lm_pairs.hip", "lm_rps.hip", "lm_sort.hip", "lm_adjoint.hip"]
+SOURCES = ["lm_core.hip", "lm_advect.hip", "lm_pairs.hip", "lm_rps.hip", "lm_sort.hip", "lm_adjoint.hip", "lm_analysis.hip"]
 ARCH = os.environ.get("LM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

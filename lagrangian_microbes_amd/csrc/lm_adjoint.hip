@@ -51,24 +51,6 @@ namespace lm {
 
 constexpr int kAdjBlock = 256;
 
-// sin(lat * pi / 180) for |lat| <= 90, used only in derivatives: Taylor
-// series through x^17 (truncation below 5e-14 at x = pi/2), Horner in x^2.
-__device__ __forceinline__ double sin_deg(double deg) {
-    const double x = deg * LM_DEG2RAD;
-    const double z = x * x;
-    // s_k = (-1)^k / (2k+1)!, k = 8 .. 0, as exactly rounded hex literals.
-    double p = 0x1.952c77030ad4ap-49;
-    p = fma(p, z, -0x1.ae7f3e733b81fp-41);
-    p = fma(p, z, 0x1.6124613a86d09p-33);
-    p = fma(p, z, -0x1.ae64567f544e4p-26);
-    p = fma(p, z, 0x1.71de3a556c734p-19);
-    p = fma(p, z, -0x1.a01a01a01a01ap-13);
-    p = fma(p, z, 0x1.1111111111111p-7);
-    p = fma(p, z, -0x1.5555555555555p-3);
-    p = fma(p, z, 1.0);
-    return p * x;
-}
-
 // 1 / c by two Newton steps from the hardware reciprocal estimate.
 __device__ __forceinline__ double recip(double c) {
     double r = __builtin_amdgcn_rcp(c);

@@ -83,6 +83,8 @@ enum KernelId {
     K_ADJ_SLAB,
     K_ADVECT_F32,
     K_BLEND_F32,
+    K_SPECIES_COUNT,
+    K_DIST_HIST,
     K_NUM_KERNELS
 };
 bool timing_on();
@@ -212,6 +214,25 @@ __host__ __device__ inline double cos_deg(double lat_deg) {
     p = fma(p, z, 0x1.5555555555555p-5);
     p = fma(p, z, -0.5);
     return fma(p, z, 1.0);
+}
+
+// sin(x * pi / 180) for |x| <= 90: Taylor series through x^17 (truncation
+// below 5e-14 at x = pi/2), Horner in z = x^2 with explicit fma; the same
+// operations on CPU and GPU.
+__host__ __device__ inline double sin_deg(double deg) {
+    const double x = deg * LM_DEG2RAD;
+    const double z = x * x;
+    // s_k = (-1)^k / (2k+1)!, k = 8 .. 0, as exactly rounded hex literals.
+    double p = 0x1.952c77030ad4ap-49;
+    p = fma(p, z, -0x1.ae7f3e733b81fp-41);
+    p = fma(p, z, 0x1.6124613a86d09p-33);
+    p = fma(p, z, -0x1.ae64567f544e4p-26);
+    p = fma(p, z, 0x1.71de3a556c734p-19);
+    p = fma(p, z, -0x1.a01a01a01a01ap-13);
+    p = fma(p, z, 0x1.1111111111111p-7);
+    p = fma(p, z, -0x1.5555555555555p-3);
+    p = fma(p, z, 1.0);
+    return p * x;
 }
 
 }  // namespace lm

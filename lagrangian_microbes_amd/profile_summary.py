@@ -40,7 +40,7 @@ TIMER_NAMES = {
     "advect_rk4_f32_kernel": "advect_rk4_f32", "field_blend_f32_kernel": "field_blend_f32",
     "adj_forward_kernel": "adj_forward", "adj_recompute_kernel": "adj_recompute",
     "adj_backward_kernel": "adj_backward", "adj_blend_kernel": "adj_blend", "adj_slab_kernel": "adj_slab",
-    "adj_zero_kernel": "adj_slab",
+    "adj_zero_kernel": "adj_slab", "species_count_kernel": "species_count", "dist_hist_kernel": "dist_hist",
 }
 
 

@@ -32,6 +32,11 @@ Contents and what pins each function (the checks live in tests/):
   against numpy.cos in float32 rounding, closed-form flows, and its distance
   from ``advect_rk4_f64`` on the same inputs (float32 interpolation error
   only).
+- ``distance_histogram`` / ``sin_deg`` (C): the pairwise haversine term and
+  its binning against thresholds, as the device reduction computes them.
+  Pinned by an independent numpy histogram of round(10 log10(max(1, d)))
+  with numpy's sin/cos/arcsin (equal away from bin edges), the sine
+  polynomial against numpy.sin, and the total n (n - 1) / 2.
 - :mod:`oracle.adjoint_ref` (torch float64 on the CPU): the RK4 advection
   written with differentiable torch operations, whose autograd gradients the
   hand-written adjoint is checked against.  Pinned by its positions against
@@ -75,6 +80,10 @@ def lib():
         L.oracle_advect_rk4_f32.argtypes = [i64, vp, vp, vp, i64, i32, i32, dbl, dbl, dbl, dbl, i32, dbl,
                                             vp, vp, i64, dbl, u64, u64, u64]
         L.oracle_advect_rk4_f32.restype = i64
+        L.oracle_sin_deg.argtypes = [dbl]
+        L.oracle_sin_deg.restype = dbl
+        L.oracle_distance_histogram.argtypes = [i64, vp, vp, i32, vp, vp]
+        L.oracle_distance_histogram.restype = None
         L.oracle_rps_sequential.argtypes = [i64, vp, vp, vp, i64, dbl, dbl, dbl, u64, u64, u64]
         L.oracle_rps_sequential.restype = None
         L.oracle_pairs_bruteforce.argtypes = [i64, vp, vp, dbl, dbl, vp, vp, i64]
@@ -174,6 +183,21 @@ def advect_rk4_f32(lon, lat, stage_fields, grid, nsteps, dt, steady=False, recor
         float(idy), int(nsteps), float(dt), _ptr(rl) if record else None, _ptr(rt) if record else None,
         n, float(walk_amp), int(key[0]), int(key[1]), int(step0))
     return lon, lat, rl, rt, int(oob)
+
+
+def sin_deg(x):
+    return lib().oracle_sin_deg(float(x))
+
+
+def distance_histogram(lon, lat, edges):
+    """Counts of pairs i < j by the number of thresholds <= their haversine
+    term (float32 positions).  O(n^2)."""
+    lon = np.ascontiguousarray(lon, dtype=np.float32)
+    lat = np.ascontiguousarray(lat, dtype=np.float32)
+    edges = np.ascontiguousarray(edges, dtype=np.float64)
+    hist = np.zeros(edges.size + 1, dtype=np.uint64)
+    lib().oracle_distance_histogram(lon.size, _ptr(lon), _ptr(lat), edges.size + 1, _ptr(edges), _ptr(hist))
+    return hist
 
 
 def rps_sequential(species, pair_i, pair_j, pRS, pPR, pSP, key, step):

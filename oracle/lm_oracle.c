@@ -283,6 +283,43 @@ int64_t oracle_advect_rk4_f32(int64_t n, double* lon, double* lat, const float* 
 }
 
 /* ------------------------------------------------------------------------ */
+/* pairwise distance histogram (the device reduction's restatement)          */
+/* ------------------------------------------------------------------------ */
+
+/* sin(x * pi/180), |x| <= 90: Taylor series through x^17, Horner in x^2. */
+double oracle_sin_deg(double deg) {
+    const double x = deg * 0.017453292519943295;
+    const double z = x * x;
+    double p = 0x1.952c77030ad4ap-49;
+    p = fma(p, z, -0x1.ae7f3e733b81fp-41);
+    p = fma(p, z, 0x1.6124613a86d09p-33);
+    p = fma(p, z, -0x1.ae64567f544e4p-26);
+    p = fma(p, z, 0x1.71de3a556c734p-19);
+    p = fma(p, z, -0x1.a01a01a01a01ap-13);
+    p = fma(p, z, 0x1.1111111111111p-7);
+    p = fma(p, z, -0x1.5555555555555p-3);
+    p = fma(p, z, 1.0);
+    return p * x;
+}
+
+/* hist[b] += pairs i < j whose haversine term a has b thresholds <= a. */
+void oracle_distance_histogram(int64_t n, const float* lon, const float* lat, int32_t nbins, const double* edges,
+                               uint64_t* hist) {
+    for (int64_t i = 0; i < n; ++i) {
+        const double xi = (double)lon[i], yi = (double)lat[i], ci = oracle_cos_deg(yi);
+        for (int64_t j = i + 1; j < n; ++j) {
+            const double yj = (double)lat[j];
+            const double s1 = oracle_sin_deg(0.5 * (yj - yi));
+            const double s2 = oracle_sin_deg(0.5 * ((double)lon[j] - xi));
+            const double a = s1 * s1 + (ci * oracle_cos_deg(yj)) * (s2 * s2);
+            int b = 0;
+            while (b < nbins - 1 && edges[b] <= a) ++b;
+            ++hist[b];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* rock-paper-scissors, sequential over the sorted pair list                 */
 /* ------------------------------------------------------------------------ */
 enum { ROCK = 1, PAPER = 2, SCISSORS = 3 };
