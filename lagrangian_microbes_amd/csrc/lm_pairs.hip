@@ -121,14 +121,17 @@ __device__ __forceinline__ void for_stencil(const CellGrid& g, const uint32_t* _
 __global__ void __launch_bounds__(kBlock)
 bin_keys_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y,
                 const uint16_t* __restrict__ run, CellGrid g, Strides st, uint32_t* __restrict__ keys,
-                uint32_t* __restrict__ vals) {
+                uint32_t* __restrict__ vals, float2* __restrict__ xy) {
     const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const int64_t s = blockIdx.y;
+    const float xf = x[s * st.pos + i], yf = y[s * st.pos + i];
     int cx, cy;
-    cell_of(g, x[s * st.pos + i], y[s * st.pos + i], &cx, &cy);
+    cell_of(g, xf, yf, &cx, &cy);
     keys[s * st.rec + i] = bucket_of(g, cx, cy, run ? (uint32_t)run[i] : 0u);
     vals[s * st.rec + i] = (uint32_t)i;
+    // interleaved copy: the binning gathers a particle's position with one 8-byte load
+    xy[s * st.rec + i] = make_float2(xf, yf);
 }
 
 // Second level of the binning: one block per super-bucket (the particles
@@ -143,8 +146,8 @@ constexpr uint32_t kFineMaxWide = 8192;
 
 template <uint32_t FINE_MAX>
 __global__ void __launch_bounds__(kBlock)
-bin_table_kernel(int64_t n, const float* __restrict__ x, const float* __restrict__ y,
-                 const uint16_t* __restrict__ run, CellGrid g, Strides st, const uint32_t* __restrict__ keys_all,
+bin_table_kernel(int64_t n, const float2* __restrict__ xy, const uint16_t* __restrict__ run, CellGrid g, Strides st,
+                 const uint32_t* __restrict__ keys_all,
                  const uint32_t* __restrict__ vals_all, const uint32_t* __restrict__ dstart, int64_t drow,
                  int64_t dstride, uint32_t fine, uint32_t* __restrict__ start_all, float4* __restrict__ recs_all) {
     __shared__ uint32_t cnt[FINE_MAX];
@@ -188,8 +191,11 @@ bin_table_kernel(int64_t n, const float* __restrict__ x, const float* __restrict
         const uint32_t i = vals[e];
         const uint32_t pos = a + atomicAdd(&cnt[keys[e] & fmask], 1u);
         if (i < nn && pos < b)
-            recs_all[s * st.rec + pos] = make_float4(x[s * st.pos + i], y[s * st.pos + i], __uint_as_float(i),
-                                                     __uint_as_float(run ? (uint32_t)run[i] : 0u));
+        {
+            const float2 p = xy[s * st.rec + i];
+            recs_all[s * st.rec + pos] =
+                make_float4(p.x, p.y, __uint_as_float(i), __uint_as_float(run ? (uint32_t)run[i] : 0u));
+        }
     }
 }
 
@@ -397,6 +403,7 @@ struct lm_pairs_ws {
     uint32_t* start = nullptr;     // [steps][tab]
     uint32_t* scan_tmp = nullptr;  // block sums
     float4* recs = nullptr;        // [steps][n_max]
+    float2* xy = nullptr;          // [steps][n_max] positions interleaved, in index order
     uint32_t* nbr_cnt = nullptr;   // [steps][row]
     uint32_t* nbr = nullptr;       // [steps][n_max * kNbr]
     uint32_t* row_off = nullptr;   // [steps][row]
@@ -413,6 +420,7 @@ static void pairs_ws_free(lm_pairs_ws* w) {
     (void)hipFree(w->start);
     (void)hipFree(w->scan_tmp);
     (void)hipFree(w->recs);
+    (void)hipFree(w->xy);
     (void)hipFree(w->nbr_cnt);
     (void)hipFree(w->nbr);
     (void)hipFree(w->row_off);
@@ -500,6 +508,7 @@ int lm_pairs_ws_create_ex(int64_t n_max, int32_t steps_max, double lon_min, doub
         (rc = lm::sort_scratch_alloc(&w->sort, n_max, steps_max)) ||
         (rc = ws_alloc(&w->start, S * w->tab_stride, &w->bytes)) ||
         (rc = ws_alloc(&w->scan_tmp, S * nb, &w->bytes)) || (rc = ws_alloc(&w->recs, S * n_max, &w->bytes)) ||
+        (rc = ws_alloc(&w->xy, S * n_max, &w->bytes)) ||
         (rc = ws_alloc(&w->nbr_cnt, S * w->row_stride, &w->bytes)) ||
         (rc = ws_alloc(&w->nbr, S * n_max * lm::kNbr, &w->bytes)) ||
         (rc = ws_alloc(&w->row_off, S * w->row_stride, &w->bytes))) {
@@ -577,7 +586,7 @@ int lm_query_pairs_runs(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float*
     st.pair = pair_stride;
     const dim3 grid(lm::grid_for(n), (unsigned)nsteps);
     LM_LAUNCH(lm::K_BIN_KEYS, lm::bin_keys_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, run, g, st,
-              ws->keys[0], ws->vals[0]);
+              ws->keys[0], ws->vals[0], ws->xy);
     LM_LAUNCH_CHECK();
     // two-level binning: one radix pass on the high bits of the bucket, then
     // a block per super-bucket sorts its fine buckets in LDS
@@ -596,11 +605,11 @@ int lm_query_pairs_runs(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float*
     }
     if (fine_bits <= 12) {
         LM_LAUNCH(lm::K_BIN_TABLE, lm::bin_table_kernel<lm::kFineMax>, dim3(1u << super_bits, (unsigned)nsteps),
-                  dim3(lm::kBlock), 0, s, n, lon, lat, run, g, st, (const uint32_t*)ws->keys[res],
+                  dim3(lm::kBlock), 0, s, n, (const float2*)ws->xy, run, g, st, (const uint32_t*)ws->keys[res],
                   (const uint32_t*)ws->vals[res], dstart, ws->sort.row, dstride, 1u << fine_bits, ws->start, ws->recs);
     } else {
         LM_LAUNCH(lm::K_BIN_TABLE, lm::bin_table_kernel<lm::kFineMaxWide>, dim3(1u << super_bits, (unsigned)nsteps),
-                  dim3(lm::kBlock), 0, s, n, lon, lat, run, g, st, (const uint32_t*)ws->keys[res],
+                  dim3(lm::kBlock), 0, s, n, (const float2*)ws->xy, run, g, st, (const uint32_t*)ws->keys[res],
                   (const uint32_t*)ws->vals[res], dstart, ws->sort.row, dstride, 1u << fine_bits, ws->start, ws->recs);
     }
     LM_LAUNCH_CHECK();

@@ -76,6 +76,7 @@ struct RpsArrays {
     uint32_t *rcnt, *roff;                // per sorted position: row length, run offset
     uint2* info;                          // per particle: {run position - pair rank, slot}
     uint8_t* touched;                     // per particle
+    uint2* rows;                          // per particle: {rowbeg, rowend | touched << 31} (one gather)
     uint32_t *pkey, *pval, *ptk, *ptv;    // particle sort buffers
     const uint32_t *spk, *ids;            // sorted roots / particle indices (two of the above)
     uint4 *narrow, *wide;                 // component lists (per particle)
@@ -260,7 +261,10 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st) {
     const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (v < n) {
         // an untouched particle is keyed by itself: a run of one, never listed
-        A.pkey[ro + v] = A.touched[ro + v] ? find_root(A.parent + ro, (uint32_t)v) : (uint32_t)v;
+        const uint8_t t = A.touched[ro + v];
+        A.pkey[ro + v] = t ? find_root(A.parent + ro, (uint32_t)v) : (uint32_t)v;
+        // what the grouping gathers per particle, in one 8-byte record (row ends stay below 2^29)
+        A.rows[ro + v] = make_uint2(A.rowbeg[ro + v], A.rowend[ro + v] | (t ? 0x80000000u : 0u));
         A.pval[ro + v] = (uint32_t)v;
     }
 }
@@ -348,9 +352,10 @@ rps_group_kernel(int64_t n, RpsArrays A, RpsStrides st, const uint32_t* __restri
 #pragma unroll
         for (uint32_t r = 0; r < kGroupRounds; ++r) {
             const bool ok = v[r] < nn;
-            tch[r] = ok ? A.touched[ro + v[r]] : (uint8_t)0;
-            rb[r] = ok ? A.rowbeg[ro + v[r]] : 0u;
-            re[r] = ok ? A.rowend[ro + v[r]] : 0u;
+            const uint2 q = ok ? A.rows[ro + v[r]] : make_uint2(0u, 0u);
+            tch[r] = (uint8_t)(q.y >> 31);
+            rb[r] = q.x;
+            re[r] = q.y & 0x7fffffffu;
         }
 #pragma unroll
         for (uint32_t r = 0; r < kGroupRounds; ++r) {
@@ -920,6 +925,7 @@ static void rps_ws_free(lm_rps_ws* w) {
                         A.lists, w->scan_tmp})
         (void)hipFree(p);
     (void)hipFree(A.info);
+    (void)hipFree(A.rows);
     (void)hipFree(A.touched);
     (void)hipFree(A.narrow);
     (void)hipFree(A.wide);
@@ -979,6 +985,7 @@ int lm_rps_ws_create(int64_t n_max, int32_t steps_max, lm_rps_ws** out) {
         (rc = rws_alloc(&A.rowbeg, R, &w->bytes)) ||
         (rc = rws_alloc(&A.rowend, R, &w->bytes)) || (rc = rws_alloc(&A.rcnt, R, &w->bytes)) ||
         (rc = rws_alloc(&A.roff, R, &w->bytes)) || (rc = rws_alloc(&A.info, R, &w->bytes)) ||
+        (rc = rws_alloc(&A.rows, R, &w->bytes)) ||
         (rc = rws_alloc(&A.touched, R, &w->bytes)) || (rc = rws_alloc(&A.pkey, R, &w->bytes)) ||
         (rc = rws_alloc(&A.pval, R, &w->bytes)) || (rc = rws_alloc(&A.ptk, R, &w->bytes)) ||
         (rc = rws_alloc(&A.ptv, R, &w->bytes)) || (rc = rws_alloc(&A.narrow, R, &w->bytes)) ||
