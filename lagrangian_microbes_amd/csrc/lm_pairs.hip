@@ -81,15 +81,16 @@ __device__ __forceinline__ uint32_t bucket_of(const CellGrid& g, int cx, int cy,
     return (run * g.cells + (uint32_t)cy * (uint32_t)g.nx + (uint32_t)cx) & g.mask;
 }
 
+// The metric is a template argument of the kernels that test candidates, so
+// each instance holds only its own test (the general-p form inlines pow).
+template <int KIND>
 __device__ __forceinline__ bool within(const Metric& m, float xa, float ya, float xb, float yb) {
     const double dx = (double)xb - (double)xa;
     const double dy = (double)yb - (double)ya;
-    switch (m.kind) {
-        case kL2: return dx * dx + dy * dy <= m.r2;
-        case kL1: return fabs(dx) + fabs(dy) <= m.r;
-        case kLinf: return fmax(fabs(dx), fabs(dy)) <= m.r;
-        default: return pow(fabs(dx), m.p) + pow(fabs(dy), m.p) <= m.r2;
-    }
+    if constexpr (KIND == kL2) return dx * dx + dy * dy <= m.r2;
+    else if constexpr (KIND == kL1) return fabs(dx) + fabs(dy) <= m.r;
+    else if constexpr (KIND == kLinf) return fmax(fabs(dx), fabs(dy)) <= m.r;
+    else return pow(fabs(dx), m.p) + pow(fabs(dy), m.p) <= m.r2;
 }
 
 // Visit the record slots of the (up to) 3x3 cells around (cx, cy).  Slot
@@ -224,8 +225,17 @@ __device__ __forceinline__ void stencil_ranges(const CellGrid& g, const uint32_t
         on[2 * r + 1] = valid && wrap;
     }
     uint32_t sb[6], se[6];
+    // the three main segments load unconditionally (index 0 for a row off the
+    // grid, then discarded); the wrapped second segments are rare
 #pragma unroll
-    for (int r = 0; r < 6; ++r) {
+    for (int r = 0; r < 6; r += 2) {
+        sb[r] = start[on[r] ? lo[r] : 0u];
+        se[r] = start[on[r] ? hi[r] : 0u];
+        sb[r] = on[r] ? sb[r] : 0u;
+        se[r] = on[r] ? se[r] : 0u;
+    }
+#pragma unroll
+    for (int r = 1; r < 6; r += 2) {
         sb[r] = on[r] ? start[lo[r]] : 0u;
         se[r] = on[r] ? start[hi[r]] : 0u;
     }
@@ -238,6 +248,7 @@ __device__ __forceinline__ void stencil_ranges(const CellGrid& g, const uint32_t
 
 constexpr int kCand = 8;  // candidate records in flight per thread
 
+template <int KIND>
 __global__ void __launch_bounds__(kBlock)
 pair_count_kernel(int64_t n, CellGrid g, Metric m, Strides st, const uint32_t* __restrict__ start_all,
                   const float4* __restrict__ recs_all, uint32_t* __restrict__ nbr_cnt,
@@ -270,23 +281,25 @@ pair_count_kernel(int64_t n, CellGrid g, Metric m, Strides st, const uint32_t* _
         float4 o[kCand];
 #pragma unroll
         for (int u = 0; u < kCand; ++u) {
-            const uint32_t t = t0 + u;
+            // past the end: the last candidate again (a valid slot), discarded below
+            const uint32_t t = min(t0 + u, total - 1u);
             uint32_t q = rb[0] + t;
 #pragma unroll
             for (int r = 1; r < 6; ++r) q = t >= pre[r] ? rb[r] + (t - pre[r]) : q;
-            o[u] = t < total ? recs[q] : make_float4(0.f, 0.f, __uint_as_float(0u), 0.f);
+            o[u] = recs[q];
         }
 #pragma unroll
         for (int u = 0; u < kCand; ++u) {
             const uint32_t j = __float_as_uint(o[u].z);
             // |dx| <= r and |dy| <= r hold for every p >= 1; the float32
             // differences of nearby positions are exact, and m.box is above
-            // r, so the prefilter only drops pairs the exact test drops
-            if (t0 + u < total && j > i && __float_as_uint(o[u].w) == my_run && fabsf(o[u].x - me.x) <= m.box &&
-                fabsf(o[u].y - me.y) <= m.box && within(m, me.x, me.y, o[u].x, o[u].y)) {
-                if (cnt < kNbr) row[cnt] = j;
-                ++cnt;
-            }
+            // r, so the prefilter only drops pairs the exact test drops.
+            // One predicate, evaluated without branches; one branch for the rare store.
+            const bool cand = (t0 + u < total) & (j > i) & (__float_as_uint(o[u].w) == my_run) &
+                              (fabsf(o[u].x - me.x) <= m.box) & (fabsf(o[u].y - me.y) <= m.box);
+            const bool hit = cand & within<KIND>(m, me.x, me.y, o[u].x, o[u].y);
+            if (hit & (cnt < kNbr)) row[cnt] = j;
+            cnt += hit ? 1u : 0u;
         }
     }
     nbr_cnt[s * st.row + i] = cnt;
@@ -316,6 +329,7 @@ __device__ __forceinline__ void emit_sorted(const uint32_t* row, uint32_t c, uin
         }
 }
 
+template <int KIND>
 __global__ void __launch_bounds__(kBlock)
 pair_emit_kernel(int64_t n, const float* __restrict__ x_all, const float* __restrict__ y_all,
                  const uint16_t* __restrict__ run, CellGrid g, Metric m, Strides st, const uint32_t* __restrict__ start_all, const float4* __restrict__ recs_all,
@@ -363,7 +377,7 @@ pair_emit_kernel(int64_t n, const float* __restrict__ x_all, const float* __rest
         for_stencil(g, start, (uint32_t)n, cx, cy, my_run, [&](uint32_t q0) {
             const float4 rec = recs[q0];
             uint32_t j = __float_as_uint(rec.z);
-            if (j > last && j < v[kSel - 1] && __float_as_uint(rec.w) == my_run && within(m, xf, yf, rec.x, rec.y)) {
+            if (j > last && j < v[kSel - 1] && __float_as_uint(rec.w) == my_run && within<KIND>(m, xf, yf, rec.x, rec.y)) {
 #pragma unroll
                 for (int q = 0; q < kSel; ++q) {  // insert into the sorted register array
                     const uint32_t lo = min(v[q], j), hi = max(v[q], j);
@@ -444,6 +458,29 @@ static uint32_t next_pow2_u32(uint64_t v) {
     while (p < v) p <<= 1;
     return (uint32_t)p;
 }
+
+namespace lm {
+
+// Phases 4-6 with the distance test of metric KIND.
+template <int KIND>
+static int count_and_emit(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float* lon, const float* lat,
+                          const uint16_t* run, const CellGrid& g, const Metric& m, const Strides& st, int64_t* npairs,
+                          int32_t* pair_i, int32_t* pair_j, hipStream_t s) {
+    const dim3 grid(grid_for(n), (unsigned)nsteps);
+    LM_LAUNCH(K_PAIR_COUNT, pair_count_kernel<KIND>, grid, dim3(kBlock), 0, s, n, g, m, st,
+              (const uint32_t*)ws->start, (const float4*)ws->recs, ws->nbr_cnt, ws->nbr);
+    LM_LAUNCH_CHECK();
+    const int rc = exclusive_scan_u32_batched(ws->nbr_cnt, n, st.row, ws->row_off, st.row, nsteps, ws->scan_tmp,
+                                              npairs, s);
+    if (rc) return rc;
+    LM_LAUNCH(K_PAIR_EMIT, pair_emit_kernel<KIND>, grid, dim3(kBlock), 0, s, n, lon, lat, run, g, m, st,
+              (const uint32_t*)ws->start, (const float4*)ws->recs, (const uint32_t*)ws->nbr_cnt,
+              (const uint32_t*)ws->nbr, (const uint32_t*)ws->row_off, pair_i, pair_j);
+    LM_LAUNCH_CHECK();
+    return LM_OK;
+}
+
+}  // namespace lm
 
 extern "C" {
 
@@ -613,17 +650,15 @@ int lm_query_pairs_runs(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float*
                   (const uint32_t*)ws->vals[res], dstart, ws->sort.row, dstride, 1u << fine_bits, ws->start, ws->recs);
     }
     LM_LAUNCH_CHECK();
-    LM_LAUNCH(lm::K_PAIR_COUNT, lm::pair_count_kernel, grid, dim3(lm::kBlock), 0, s, n, g, m, st,
-              (const uint32_t*)ws->start, (const float4*)ws->recs, ws->nbr_cnt, ws->nbr);
-    LM_LAUNCH_CHECK();
-    rc = lm::exclusive_scan_u32_batched(ws->nbr_cnt, n, st.row, ws->row_off, st.row, nsteps, ws->scan_tmp, npairs,
-                                        s);
-    if (rc) return rc;
-    LM_LAUNCH(lm::K_PAIR_EMIT, lm::pair_emit_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, run, g, m, st,
-              (const uint32_t*)ws->start, (const float4*)ws->recs, (const uint32_t*)ws->nbr_cnt,
-              (const uint32_t*)ws->nbr, (const uint32_t*)ws->row_off, pair_i, pair_j);
-    LM_LAUNCH_CHECK();
-    return LM_OK;
+    switch (m.kind) {
+        case lm::kL2: rc = lm::count_and_emit<lm::kL2>(ws, nsteps, n, lon, lat, run, g, m, st, npairs, pair_i, pair_j, s); break;
+        case lm::kL1: rc = lm::count_and_emit<lm::kL1>(ws, nsteps, n, lon, lat, run, g, m, st, npairs, pair_i, pair_j, s); break;
+        case lm::kLinf:
+            rc = lm::count_and_emit<lm::kLinf>(ws, nsteps, n, lon, lat, run, g, m, st, npairs, pair_i, pair_j, s);
+            break;
+        default: rc = lm::count_and_emit<lm::kLp>(ws, nsteps, n, lon, lat, run, g, m, st, npairs, pair_i, pair_j, s); break;
+    }
+    return rc;
 }
 
 int lm_query_pairs(lm_pairs_ws* ws, int64_t n, const float* lon, const float* lat, double radius, double p,
