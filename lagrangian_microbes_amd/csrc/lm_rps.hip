@@ -405,20 +405,42 @@ rps_comps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
     __syncthreads();
     uint4 d[kCompItems];
     uint32_t at[kCompItems];  // list (bit 31: wide) and local index, or kUnset
+    // Every load below is unconditional, from an index clamped into its array
+    // (results for items that turn out invalid are discarded), so the
+    // dependent gathers of all items are in flight together.
+    const uint32_t nn = (uint32_t)n, last = nn - 1u;
+    uint32_t L[kCompItems], x[kCompItems];
+    bool ok[kCompItems];
+#pragma unroll
+    for (int r = 0; r < kCompItems; ++r) {
+        const int64_t e = ((int64_t)blockIdx.x * kCompItems + r) * kBlock + threadIdx.x;
+        const int64_t ec = e < n ? e : (int64_t)last;
+        L[r] = spk[ec];
+        x[r] = A.ids[ro + ec];
+        ok[r] = e < n && L[r] < nn && x[r] < nn;
+        L[r] = L[r] < nn ? L[r] : 0u;
+        x[r] = x[r] < nn ? x[r] : 0u;
+    }
+    uint32_t lc[kCompItems], pb[kCompItems], rbg[kCompItems], ro_e[kCompItems];
+#pragma unroll
+    for (int r = 0; r < kCompItems; ++r) {
+        const int64_t e = ((int64_t)blockIdx.x * kCompItems + r) * kBlock + threadIdx.x;
+        const int64_t ec = e < n ? e : (int64_t)last;
+        lc[r] = A.loc[ro + x[r]];
+        pb[r] = A.pstart[ro + L[r]];
+        rbg[r] = A.rowbeg[ro + x[r]];
+        ro_e[r] = roff[ec];
+    }
 #pragma unroll
     for (int r = 0; r < kCompItems; ++r) {
         const int64_t e = ((int64_t)blockIdx.x * kCompItems + r) * kBlock + threadIdx.x;
         at[r] = kUnset;
         d[r] = make_uint4(0u, 0u, 0u, 0u);
-        if (e >= n) continue;
-        const uint32_t L = spk[e];
-        const uint32_t x = A.ids[ro + e];
-        if (L >= (uint64_t)n || x >= (uint64_t)n || A.loc[ro + x] == kUnset) continue;  // untouched
-        const uint32_t pb = A.pstart[ro + L];
-        A.info[ro + x] = make_uint2(roff[e] - A.rowbeg[ro + x], (uint32_t)e - pb);
-        if ((uint32_t)e == pb) {
-            const uint32_t pe = A.pend[ro + L];
-            const uint32_t qb = min(roff[e], P), qe = min(roff[pe], P);
+        if (!ok[r] || lc[r] == kUnset) continue;  // past the end or untouched
+        A.info[ro + x[r]] = make_uint2(ro_e[r] - rbg[r], (uint32_t)e - pb[r]);
+        if ((uint32_t)e == pb[r]) {
+            const uint32_t pe = A.pend[ro + L[r]];
+            const uint32_t qb = min(ro_e[r], P), qe = min(roff[pe], P);
             d[r] = make_uint4(qb, qe > qb ? qe - qb : 0u, (uint32_t)e, pe - (uint32_t)e);
             const uint32_t wide = d[r].y > kNarrowPairs ? 1u : 0u;
             at[r] = (wide << 31) | atomicAdd(&cnt[wide], 1u);
