@@ -130,7 +130,7 @@ bin_keys_kernel(int64_t n, const float* __restrict__ x, const float* __restrict_
     int cx, cy;
     cell_of(g, xf, yf, &cx, &cy);
     keys[s * st.rec + i] = bucket_of(g, cx, cy, run ? (uint32_t)run[i] : 0u);
-    vals[s * st.rec + i] = (uint32_t)i;
+    if (vals) vals[s * st.rec + i] = (uint32_t)i;  // NULL when the radix pass generates the indices
     // interleaved copy: the binning gathers a particle's position with one 8-byte load
     xy[s * st.rec + i] = make_float2(xf, yf);
 }
@@ -621,22 +621,23 @@ int lm_query_pairs_runs(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float*
     st.row = ws->row_stride;
     st.nbr = ws->n_max * lm::kNbr;
     st.pair = pair_stride;
-    const dim3 grid(lm::grid_for(n), (unsigned)nsteps);
-    LM_LAUNCH(lm::K_BIN_KEYS, lm::bin_keys_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, run, g, st,
-              ws->keys[0], ws->vals[0], ws->xy);
-    LM_LAUNCH_CHECK();
     // two-level binning: one radix pass on the high bits of the bucket, then
     // a block per super-bucket sorts its fine buckets in LDS
     // 12 fine bits in LDS (13 above 2^22 buckets), the rest in one radix pass
     const int fine_bits = ws->key_bits <= 12 ? ws->key_bits : ws->key_bits <= 22 ? 12 : ws->key_bits - lm::kMaxDigitBits;
     const int super_bits = ws->key_bits - fine_bits;
+    const dim3 grid(lm::grid_for(n), (unsigned)nsteps);
+    // the particle indices are written only when no radix pass generates them
+    LM_LAUNCH(lm::K_BIN_KEYS, lm::bin_keys_kernel, grid, dim3(lm::kBlock), 0, s, n, lon, lat, run, g, st,
+              ws->keys[0], super_bits > 0 ? nullptr : ws->vals[0], ws->xy);
+    LM_LAUNCH_CHECK();
     const uint32_t* dstart = nullptr;
     int64_t dstride = 0;
     int rc = LM_OK;
     int res = 0;
     if (super_bits > 0) {
         rc = lm::radix_partition_u32(ws->keys, ws->vals, st.rec, nsteps, n, fine_bits, super_bits, &ws->sort, s,
-                                     &dstart, &dstride);
+                                     &dstart, &dstride, true);
         if (rc) return rc;
         res = 1;
     }

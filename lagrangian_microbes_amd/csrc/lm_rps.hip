@@ -256,7 +256,7 @@ rps_run_offsets_kernel(int nsteps, int nruns, const int64_t* __restrict__ run_st
 }
 
 __global__ void __launch_bounds__(kBlock)
-rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st) {
+rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
     const int64_t s = blockIdx.y, ro = s * st.row;
     const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (v < n) {
@@ -265,7 +265,7 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st) {
         A.pkey[ro + v] = t ? find_root(A.parent + ro, (uint32_t)v) : (uint32_t)v;
         // what the grouping gathers per particle, in one 8-byte record (row ends stay below 2^29)
         A.rows[ro + v] = make_uint2(A.rowbeg[ro + v], A.rowend[ro + v] | (t ? 0x80000000u : 0u));
-        A.pval[ro + v] = (uint32_t)v;
+        if (write_vals) A.pval[ro + v] = (uint32_t)v;  // else the radix pass generates the indices
     }
 }
 
@@ -1086,8 +1086,6 @@ static int rps_prepare_impl(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int3
                       pair_j, npairs, A, sp, pRS, pPR, pSP, key0, key1, step0, R);
         }
         LM_LAUNCH_CHECK();
-        LM_LAUNCH(lm::K_RPS_KEYS, lm::rps_keys_kernel, dim3(nbp, S), dim3(lm::kBlock), 0, s, n, A, sp);
-        LM_LAUNCH_CHECK();
         // grouping by root in two levels: a radix pass on the root's high
         // bits, then a block per super-bucket counts its roots in LDS
         const int kb = key_bits_for(n);
@@ -1098,6 +1096,8 @@ static int rps_prepare_impl(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int3
         const int super_bits = kb - fine_bits;
         LM_REQUIRE(super_bits <= lm::kMaxDigitBits, LM_ERR_ARG, "at most 2^%d particles",
                    (int)lm::kGroupFineBitsHuge + lm::kMaxDigitBits);
+        LM_LAUNCH(lm::K_RPS_KEYS, lm::rps_keys_kernel, dim3(nbp, S), dim3(lm::kBlock), 0, s, n, A, sp, super_bits == 0);
+        LM_LAUNCH_CHECK();
         uint32_t* pk[2] = {A.pkey, A.ptk};
         uint32_t* pv[2] = {A.pval, A.ptv};
         const uint32_t* dstart = nullptr;
@@ -1105,7 +1105,7 @@ static int rps_prepare_impl(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int3
         int in = 0;
         if (super_bits > 0) {
             if ((rc = lm::radix_partition_u32(pk, pv, ws->row, nsteps, n, fine_bits, super_bits, &ws->sort, s, &dstart,
-                                              &dstride)))
+                                              &dstride, true)))
                 return rc;
             in = 1;
         }

@@ -104,7 +104,8 @@ radix_scatter_kernel(const uint32_t* __restrict__ kin, const uint32_t* __restric
     const int64_t group0 = (int64_t)blockIdx.x * kSortGroup * kSortTile;
     if (group0 >= len) return;  // uniform over the block
     const uint32_t* ki = kin + (int64_t)g * stride;
-    const uint32_t* vi = vin + (int64_t)g * stride;
+    // vin == NULL: each element's value is its index in the segment
+    const uint32_t* vi = vin ? vin + (int64_t)g * stride : nullptr;
     uint32_t* ko = kout + (int64_t)g * stride;
     uint32_t* vo = vout + (int64_t)g * stride;
     uint4 gb[PER];  // output offsets of my digits in the group's tiles
@@ -118,7 +119,7 @@ radix_scatter_kernel(const uint32_t* __restrict__ kin, const uint32_t* __restric
     for (int r = 0; r < kSortItems; ++r) {
         const int64_t e = group0 + wave * kWaveChunk + r * 64 + lane;
         key[r] = e < len ? ki[e] : 0u;
-        val[r] = e < len ? vi[e] : 0u;
+        val[r] = e < len ? (vi ? vi[e] : (uint32_t)e) : 0u;
     }
 #pragma unroll
     for (int j = 0; j < kSortGroup; ++j) {
@@ -148,7 +149,7 @@ radix_scatter_kernel(const uint32_t* __restrict__ kin, const uint32_t* __restric
             const int64_t e = base + kSortTile + r * 64 + lane;
             const bool ok = j + 1 < kSortGroup && e < len;
             nkey[r] = ok ? ki[e] : 0u;
-            nval[r] = ok ? vi[e] : 0u;
+            nval[r] = ok ? (vi ? vi[e] : (uint32_t)e) : 0u;
         }
         __syncthreads();
         // per digit: offsets of the wavefronts' runs, tile-local start, global start
@@ -254,7 +255,8 @@ void sort_scratch_free(SortScratch* s) {
 // kMaxDigitBits wide) from keys/vals[cur] into keys/vals[1 - cur].
 static int radix_pass(uint32_t* keys[2], uint32_t* vals[2], int cur, int64_t stride, int32_t nseg,
                       const int64_t* seg_len, int64_t uniform_len, int64_t max_len, int shift, int bits,
-                      SortScratch* sc, hipStream_t stream) {
+                      SortScratch* sc, hipStream_t stream, bool index_vals = false) {
+    const uint32_t* vin = index_vals ? nullptr : (const uint32_t*)vals[cur];
     const int64_t ntp = sort_padded_tiles(max_len);
     const dim3 grid((unsigned)(ntp / kSortGroup), (unsigned)nseg);
     const uint32_t dmask = (uint32_t)((1ull << bits) - 1ull);
@@ -272,11 +274,11 @@ static int radix_pass(uint32_t* keys[2], uint32_t* vals[2], int cur, int64_t str
     if (rc) return rc;
     if (wide)
         LM_LAUNCH(K_SORT_SCATTER, radix_scatter_kernel<kMaxDigitBits>, grid, dim3(kSortThreads), 0, stream,
-                  (const uint32_t*)keys[cur], (const uint32_t*)vals[cur], keys[1 - cur], vals[1 - cur], stride,
+                  (const uint32_t*)keys[cur], vin, keys[1 - cur], vals[1 - cur], stride,
                   seg_len, uniform_len, max_len, shift, dmask, (const uint32_t*)sc->offs, sc->row, ntp);
     else
         LM_LAUNCH(K_SORT_SCATTER, radix_scatter_kernel<8>, grid, dim3(kSortThreads), 0, stream,
-                  (const uint32_t*)keys[cur], (const uint32_t*)vals[cur], keys[1 - cur], vals[1 - cur], stride,
+                  (const uint32_t*)keys[cur], vin, keys[1 - cur], vals[1 - cur], stride,
                   seg_len, uniform_len, max_len, shift, dmask, (const uint32_t*)sc->offs, sc->row, ntp);
     LM_LAUNCH_CHECK();
     return LM_OK;
@@ -308,13 +310,13 @@ int radix_sort_u32(uint32_t* keys[2], uint32_t* vals[2], int64_t stride, int32_t
 
 int radix_partition_u32(uint32_t* keys[2], uint32_t* vals[2], int64_t stride, int32_t nseg, int64_t len,
                         int shift, int bits, SortScratch* sc, hipStream_t stream, const uint32_t** digit_start,
-                        int64_t* digit_stride) {
+                        int64_t* digit_stride, bool index_vals) {
     LM_REQUIRE(nseg >= 1 && nseg <= sc->nseg_max && len <= sc->max_len && stride >= len, LM_ERR_ARG,
                "partition request exceeds its scratch");
     LM_REQUIRE(bits >= 1 && bits <= kMaxDigitBits && shift >= 0 && shift + bits <= 32, LM_ERR_ARG,
                "partition digit out of range");
     LM_REQUIRE(len >= 1, LM_ERR_ARG, "empty partition");
-    int rc = radix_pass(keys, vals, 0, stride, nseg, nullptr, len, len, shift, bits, sc, stream);
+    int rc = radix_pass(keys, vals, 0, stride, nseg, nullptr, len, len, shift, bits, sc, stream, index_vals);
     if (rc) return rc;
     *digit_start = sc->offs;
     *digit_stride = sort_padded_tiles(len);
