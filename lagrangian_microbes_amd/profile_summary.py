@@ -53,10 +53,11 @@ def short_name(full):
 
 
 def one_file(d, pattern):
-    files = sorted(glob.glob(os.path.join(d, "**", pattern), recursive=True))
+    """The newest file matching pattern under d (a directory may hold several runs)."""
+    files = glob.glob(os.path.join(d, "**", pattern), recursive=True)
     if not files:
         sys.exit(f"no {pattern} under {d}")
-    return files[-1]
+    return max(files, key=os.path.getmtime)
 
 
 def trace(args):
