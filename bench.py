@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=24)
     ap.add_argument("--workload", default="rps_490k_30day")
     ap.add_argument("--chunk", type=int, default=100)
+    ap.add_argument("--cell-scale", type=float, default=1.0,
+                    help="pair-search cell side in radii (changes speed, never the pairs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint forward/backward timing")
     ap.add_argument("--adjoint-reps", type=int, default=3)
@@ -371,7 +373,7 @@ def main():
     lon, lat = w.particles()
     log(f"[bench] workload {w.name}: {w.n} particles, {w.steps} steps available; building runner")
     runner = FusedRunner(lon, lat, w.species(), w.velocity_field(), w.radius, w.params, w.key, w.dt, w.norm,
-                         chunk=args.chunk)
+                         chunk=args.chunk, cell_scale=args.cell_scale)
 
     # warm-up (compiles nothing, but touches every kernel and sizes buffers)
     run_steps(runner, args.warmup, w.steps)
@@ -433,7 +435,7 @@ def main():
         "config": {"workload": w.name, "particles": w.n, "lattice": "700x700", "dt_s": w.dt,
                    "radius_deg": w.radius, "p": w.p, "field": w.describe()["field"],
                    "mean_pairs_per_step": round(P, 1), "mean_particles_with_pairs": round(stats["U"], 1),
-                   "chunk": args.chunk},
+                   "chunk": args.chunk, "cell_scale": args.cell_scale},
         "roofline": roofline,
         "kernels": {k: {"launches": v["launches"], "avg_us": round(v["avg_us"], 2), "algorithmic_gbs": v["gbs"],
                         "hbm_gbs": v["hbm_gbs"]} for k, v in kernels.items()},
