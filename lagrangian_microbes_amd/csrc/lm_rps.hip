@@ -389,6 +389,143 @@ rps_group_kernel(int64_t n, RpsArrays A, RpsStrides st, const uint32_t* __restri
     }
 }
 
+// The same grouping with four wavefronts per super-bucket, for the wide
+// fine-bit widths (12 and 14 bits: 16 and 64 KB of counters, which leave room
+// for only a few one-wavefront blocks per CU).  Counting and the offset scan
+// use the whole block.  The placement stays stable: a pass takes
+// 4 x kGroupRounds x 64 consecutive particles, wavefront w the w-th quarter;
+// every wavefront ranks its particles (ballots) at once, then the wavefronts
+// read and advance the LDS cursors in turn, w = 0..3 (the only serialised
+// part), and write their particles at once.
+constexpr int kGroupWaves = 4;
+
+template <uint32_t FB>
+__global__ void __launch_bounds__(64 * kGroupWaves)
+rps_group_blk_kernel(int64_t n, RpsArrays A, RpsStrides st, const uint32_t* __restrict__ kin,
+                     const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                     const uint32_t* __restrict__ dstart, int64_t drow, int64_t dstride, uint32_t fine) {
+    static_assert(64 * kGroupWaves == kBlock, "the block scan assumes kBlock threads");
+    constexpr uint32_t NT = 64 * kGroupWaves;
+    __shared__ uint32_t cnt[1u << FB];
+    __shared__ uint32_t scan_lds[kGroupWaves];
+    const uint32_t d = blockIdx.x, nsuper = gridDim.x, t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    const int64_t s = blockIdx.y, ro = s * st.row;
+    const uint32_t nn = (uint32_t)n;
+    kin += ro;
+    vin += ro;
+    kout += ro;
+    vout += ro;
+    uint32_t a = 0, b = nn;
+    if (dstart != nullptr) {
+        a = min(dstart[s * drow + (int64_t)d * dstride], nn);
+        b = d + 1 < nsuper ? min(dstart[s * drow + (int64_t)(d + 1) * dstride], nn) : nn;
+        b = max(a, b);
+    }
+    const uint32_t fmask = fine - 1u;
+    for (uint32_t f = t; f < fine; f += NT) cnt[f] = 0;
+    __syncthreads();
+    for (uint32_t c0 = a; c0 < b; c0 += NT * kGroupRounds) {
+        uint32_t k[kGroupRounds];
+#pragma unroll
+        for (uint32_t r = 0; r < kGroupRounds; ++r) {
+            const uint32_t e = c0 + NT * r + t;
+            k[r] = e < b ? kin[e] : 0u;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kGroupRounds; ++r)
+            if (c0 + NT * r + t < b) atomicAdd(&cnt[k[r] & fmask], 1u);
+    }
+    __syncthreads();
+    // exclusive offsets: thread t owns `per` consecutive roots; runs and their ends
+    const uint32_t per = (fine + NT - 1) / NT;
+    const uint32_t f0 = min(t * per, fine), f1 = min(f0 + per, fine);
+    uint32_t sum = 0;
+    for (uint32_t f = f0; f < f1; ++f) sum += cnt[f];
+    uint32_t all;
+    uint32_t acc = block_exclusive_scan(sum, scan_lds, &all);
+    const uint64_t base = (uint64_t)d * fine;
+    for (uint32_t f = f0; f < f1; ++f) {
+        const uint32_t c = cnt[f];
+        cnt[f] = acc;
+        if (base + f < (uint64_t)nn) {
+            A.pstart[ro + base + f] = a + acc;
+            A.pend[ro + base + f] = a + acc + c;
+        }
+        acc += c;
+    }
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (uint32_t c0 = a; c0 < b; c0 += NT * kGroupRounds) {  // uniform over the block
+        const uint32_t w0 = c0 + 64u * kGroupRounds * wave;     // this wavefront's quarter
+        uint32_t k[kGroupRounds], v[kGroupRounds];
+#pragma unroll
+        for (uint32_t r = 0; r < kGroupRounds; ++r) {
+            const uint32_t e = w0 + 64 * r + lane;
+            k[r] = e < b ? kin[e] : 0u;
+            v[r] = e < b ? vin[e] : kUnset;
+        }
+        uint8_t tch[kGroupRounds];
+        uint32_t rb[kGroupRounds], re[kGroupRounds];
+#pragma unroll
+        for (uint32_t r = 0; r < kGroupRounds; ++r) {
+            const bool ok = v[r] < nn;
+            const uint2 q = ok ? A.rows[ro + v[r]] : make_uint2(0u, 0u);
+            tch[r] = (uint8_t)(q.y >> 31);
+            rb[r] = q.x;
+            re[r] = q.y & 0x7fffffffu;
+        }
+        // ranks among equal roots within the wavefront's rounds (no LDS)
+        uint32_t rank[kGroupRounds], tot[kGroupRounds];
+#pragma unroll
+        for (uint32_t r = 0; r < kGroupRounds; ++r) {
+            const bool valid = w0 + 64 * r + lane < b;
+            const uint32_t f = k[r] & fmask;
+            unsigned long long m = __ballot(valid);
+#pragma unroll
+            for (uint32_t bit = 0; bit < FB; ++bit) {
+                const bool on = (f >> bit) & 1u;
+                const unsigned long long bm = __ballot(valid && on);
+                m &= on ? bm : ~bm;
+            }
+            rank[r] = valid ? (uint32_t)__popcll(m & below) : 0u;
+            tot[r] = valid ? (uint32_t)__popcll(m) : 0u;
+        }
+        // cursors in wavefront order, rounds in order within a wavefront
+        uint32_t start[kGroupRounds];
+        for (uint32_t w = 0; w < (uint32_t)kGroupWaves; ++w) {
+            if (wave == w) {
+#pragma unroll
+                for (uint32_t r = 0; r < kGroupRounds; ++r) {
+                    const bool valid = w0 + 64 * r + lane < b;
+                    const uint32_t f = k[r] & fmask;
+                    start[r] = valid ? cnt[f] : 0u;
+                    // every lane has read the cursor before its leader advances it; the
+                    // wavefront's LDS operations complete in program order
+                    if (valid && rank[r] == 0) cnt[f] = start[r] + tot[r];
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kGroupRounds; ++r) {
+            const bool valid = w0 + 64 * r + lane < b;
+            const uint32_t pos = a + start[r] + rank[r];
+            if (valid && pos < b) {
+                const uint32_t x = v[r];
+                kout[pos] = k[r];
+                vout[pos] = x;
+                uint32_t len = 0;
+                if (tch[r] && k[r] < nn && x < nn) {
+                    A.loc[ro + x] = pos;
+                    len = re[r] > rb[r] ? re[r] - rb[r] : 0u;
+                }
+                A.rcnt[ro + pos] = len;
+            }
+        }
+    }
+}
+
 // Over the sorted particles: slot and pair shift per particle; every run start
 // lists its component.  A block takes kCompItems positions per thread and
 // reserves its list entries with one global add per list.
@@ -1114,13 +1251,13 @@ static int rps_prepare_impl(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int3
                       0, s, n, A, sp, (const uint32_t*)pk[in], (const uint32_t*)pv[in], pk[1 - in], pv[1 - in],
                       dstart, ws->sort.row, dstride, 1u << fine_bits);
         } else if (fb == (int)lm::kGroupFineBitsWide) {
-            LM_LAUNCH(lm::K_RPS_GROUP, lm::rps_group_kernel<lm::kGroupFineBitsWide>, dim3(1u << super_bits, S),
-                      dim3(64), 0, s, n, A, sp, (const uint32_t*)pk[in], (const uint32_t*)pv[in], pk[1 - in],
-                      pv[1 - in], dstart, ws->sort.row, dstride, 1u << fine_bits);
+            LM_LAUNCH(lm::K_RPS_GROUP, lm::rps_group_blk_kernel<lm::kGroupFineBitsWide>, dim3(1u << super_bits, S),
+                      dim3(64 * lm::kGroupWaves), 0, s, n, A, sp, (const uint32_t*)pk[in], (const uint32_t*)pv[in],
+                      pk[1 - in], pv[1 - in], dstart, ws->sort.row, dstride, 1u << fine_bits);
         } else {
-            LM_LAUNCH(lm::K_RPS_GROUP, lm::rps_group_kernel<lm::kGroupFineBitsHuge>, dim3(1u << super_bits, S),
-                      dim3(64), 0, s, n, A, sp, (const uint32_t*)pk[in], (const uint32_t*)pv[in], pk[1 - in],
-                      pv[1 - in], dstart, ws->sort.row, dstride, 1u << fine_bits);
+            LM_LAUNCH(lm::K_RPS_GROUP, lm::rps_group_blk_kernel<lm::kGroupFineBitsHuge>, dim3(1u << super_bits, S),
+                      dim3(64 * lm::kGroupWaves), 0, s, n, A, sp, (const uint32_t*)pk[in], (const uint32_t*)pv[in],
+                      pk[1 - in], pv[1 - in], dstart, ws->sort.row, dstride, 1u << fine_bits);
         }
         LM_LAUNCH_CHECK();
         A.spk = pk[1 - in];
