@@ -4,6 +4,7 @@ These are thin: validate on the host, hand raw pointers and the current
 stream to liblm, and return.  Nothing here synchronises with the host except
 where a data-dependent size must come back (PairSearch.query_sync).
 """
+import atexit
 import collections
 import ctypes
 
@@ -12,6 +13,11 @@ import torch
 
 from . import _lib
 from ._lib import LmGrid, c_vp, check, ptr, require, stream_ptr
+
+# Set once the interpreter starts to exit (atexit runs before module globals
+# are cleared); workspace destructors skip freeing device memory from then on.
+_EXITING = [False]
+atexit.register(_EXITING.__setitem__, 0, True)
 
 
 def lib():
@@ -194,6 +200,8 @@ class PairSearch:
         h = c_vp()
         check(lib().lm_pairs_ws_create_ex(self.n_max, self.steps_max, self.bbox[0], self.bbox[1], self.bbox[2],
                                           self.bbox[3], self.max_radius, self.cell_scale, ctypes.byref(h)))
+        self._destroy = lib().lm_pairs_ws_destroy  # kept: module globals may be gone when __del__ runs at exit
+        self._exiting = _EXITING
         self._ws = h
         self.device = torch.device(device or "cuda")
         self.pair_i = None
@@ -201,9 +209,9 @@ class PairSearch:
         self.npairs = torch.zeros(1, dtype=torch.int64, device=self.device)
 
     def __del__(self):
-        ws = getattr(self, "_ws", None)
-        if ws and _lib._lib is not None:
-            _lib._lib.lm_pairs_ws_destroy(ws)
+        ws, destroy = getattr(self, "_ws", None), getattr(self, "_destroy", None)
+        if ws and destroy is not None and not self._exiting[0]:  # at exit the process frees everything
+            destroy(ws)
             self._ws = None
 
     @property
@@ -326,14 +334,16 @@ class RPSUpdater:
         self.steps_max = int(steps_max)
         h = c_vp()
         check(lib().lm_rps_ws_create(self.n_max, self.steps_max, ctypes.byref(h)))
+        self._destroy = lib().lm_rps_ws_destroy  # kept: module globals may be gone when __del__ runs at exit
+        self._exiting = _EXITING
         self._ws = h
         self.device = torch.device(device or "cuda")
         self._stats = torch.zeros(8, dtype=torch.int64, device=self.device)
 
     def __del__(self):
-        ws = getattr(self, "_ws", None)
-        if ws and _lib._lib is not None:
-            _lib._lib.lm_rps_ws_destroy(ws)
+        ws, destroy = getattr(self, "_ws", None), getattr(self, "_destroy", None)
+        if ws and destroy is not None and not self._exiting[0]:  # at exit the process frees everything
+            destroy(ws)
             self._ws = None
 
     @property

@@ -17,7 +17,7 @@ from lagrangian_microbes_amd.simulator import FusedRunner
 pytestmark = pytest.mark.gpu
 
 
-def run_alone(w, field, chunk):
+def run_alone(w, field, chunk, steps=None):
     lon, lat = w.particles()
     r = FusedRunner(lon, lat, w.species(), field, w.radius, w.params, w.key, w.dt, chunk=chunk, precision="f32")
     recs = []
@@ -25,7 +25,7 @@ def run_alone(w, field, chunk):
     def keep(k, rx, ry, rs, cnt):
         recs.append((rx.cpu().numpy().copy(), rs.cpu().numpy().copy(), cnt.cpu().numpy().copy()))
 
-    r.run(w.steps, keep)
+    r.run(w.steps if steps is None else steps, keep)
     r.check_domain()
     rx = np.concatenate([a for a, _, _ in recs])
     rs = np.concatenate([b for _, b, _ in recs])
@@ -104,3 +104,47 @@ def test_graph_replay_matches_eager(gpu):
             # the two buffer sets' 5-step chunks and the last 4-step chunk
             assert len(r.engine._graphs) == 3
     assert np.array_equal(out[True], out[False])
+
+
+def test_full_size_ensemble_matches_runs_alone(gpu):
+    # The 64-run ensemble at full size (14.7M particles) takes the paths only
+    # sizes above 2^23 particles reach: 2^23 buckets with 13 bits counted in
+    # LDS per binning block, and the grouping by root with 14 fine bits per
+    # block.  Its smallest, median and largest runs, each alone (2^19
+    # particles or fewer: the small-size paths), must match it step by step
+    # over the first two days (the lattices start wider apart than the
+    # radius; pairs form as the flow converges).
+    e = configs.ensemble_64runs()
+    field = e.velocity_field()
+    lon, lat = e.particles()
+    steps, chunk = 48, 24
+    print(f"[full-size ensemble] {e.n} particles, {len(e.runs)} runs, {steps} steps", flush=True)
+    assert e.n > 1 << 23  # bucket table capped at 2^23; 24 root bits
+    runs = dev.EnsembleRuns([w.n for w in e.runs], [w.params for w in e.runs], [w.key for w in e.runs])
+    r = FusedRunner(lon, lat, e.species(), field, e.runs[0].radius, {"pRS": 0, "pPR": 0, "pSP": 0}, (0, 0),
+                    e.runs[0].dt, chunk=chunk, precision="f32", runs=runs)
+    sizes = np.array([w.n for w in e.runs])
+    order = np.argsort(sizes, kind="stable")
+    picks = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    sel = {q: ([], []) for q in picks}
+
+    def keep(k, rx, ry, rs, cnt):
+        for q in picks:
+            sl = runs.slice(q)
+            sel[q][0].append(rx[:, sl].cpu().numpy())
+            sel[q][1].append(rs[:, sl].cpu().numpy())
+
+    r.run(steps, keep)
+    r.check_domain()
+    del r
+    torch.cuda.empty_cache()
+    pairs = changed = 0
+    for q in picks:
+        w = e.runs[q]
+        rx, rs, cnt = run_alone(w, field, chunk, steps)
+        assert np.array_equal(np.concatenate(sel[q][0]), rx), q
+        assert np.array_equal(np.concatenate(sel[q][1]), rs), q
+        pairs += int(cnt.sum())
+        changed += int(np.sum(rs[-1] != w.species()))
+    print(f"[full-size ensemble] runs {picks}: {pairs} pairs, {changed} species changed", flush=True)
+    assert pairs > 0 and changed > 0  # the runs do interact
