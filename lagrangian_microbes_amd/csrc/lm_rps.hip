@@ -269,137 +269,28 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
     }
 }
 
-// Second level of the grouping by root: one wavefront per super-bucket (the
-// particles whose root has the same high bits, made contiguous, in index
-// order, by one stable radix pass) counts its roots' particles in LDS and
-// writes each root's run [pstart, pend), every touched particle's position in
-// the run order (loc) and the row lengths in that order.  The placement is
-// stable -- a run keeps index order, which the pair order relies on: 64
-// particles at a time in order, a lane's rank among equal roots from one
-// ballot per root bit, a running count per root in LDS.  Loads go kGroupRounds
-// rounds at a time.  FB fine bits per block: 9 up to 2^19 particles (one
-// radix pass of at most 10 bits covers the rest), 12 up to 2^22, 14 up to
-// 2^24 (ensembles).
+// Second level of the grouping by root: one block of four wavefronts per
+// super-bucket (the particles whose root has the same high bits, made
+// contiguous, in index order, by one stable radix pass) counts its roots'
+// particles in LDS and writes each root's run [pstart, pend), every touched
+// particle's position in the run order (loc) and the row lengths in that
+// order.  Counting and the offset scan use the whole block.  The placement is
+// stable -- a run keeps index order, which the pair order relies on: a pass
+// takes 4 x R x 64 consecutive particles, wavefront w the w-th quarter in R
+// rounds of 64; a lane's rank among equal roots in its round comes from one
+// ballot per root bit, every wavefront ranking at once; then the wavefronts
+// read and advance the per-root LDS cursors in turn, w = 0..3, rounds in
+// order (the only serialised part), and write their particles at once.
+// FB fine bits per block: 9 up to 2^19 particles (one radix pass of at most
+// 10 bits covers the rest; R = 2, as a super-bucket holds ~2^9 particles),
+// 12 up to 2^22, 14 up to 2^24 (ensembles; 64 KB of counters).
 constexpr uint32_t kGroupFineBits = 9;
 constexpr uint32_t kGroupFineBitsWide = 12;
 constexpr uint32_t kGroupFineBitsHuge = 14;
 constexpr uint32_t kGroupRounds = 8;
-
-template <uint32_t FB>
-__global__ void __launch_bounds__(64)
-rps_group_kernel(int64_t n, RpsArrays A, RpsStrides st, const uint32_t* __restrict__ kin,
-                 const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                 const uint32_t* __restrict__ dstart, int64_t drow, int64_t dstride, uint32_t fine) {
-    __shared__ uint32_t cnt[1u << FB];
-    const uint32_t d = blockIdx.x, nsuper = gridDim.x, lane = threadIdx.x;
-    const int64_t s = blockIdx.y, ro = s * st.row;
-    const uint32_t nn = (uint32_t)n;
-    kin += ro;
-    vin += ro;
-    kout += ro;
-    vout += ro;
-    uint32_t a = 0, b = nn;
-    if (dstart != nullptr) {
-        a = min(dstart[s * drow + (int64_t)d * dstride], nn);
-        b = d + 1 < nsuper ? min(dstart[s * drow + (int64_t)(d + 1) * dstride], nn) : nn;
-        b = max(a, b);
-    }
-    const uint32_t fmask = fine - 1u;
-    for (uint32_t f = lane; f < fine; f += 64) cnt[f] = 0;
-    __syncthreads();
-    for (uint32_t c0 = a; c0 < b; c0 += 64 * kGroupRounds) {
-        uint32_t k[kGroupRounds];
-#pragma unroll
-        for (uint32_t r = 0; r < kGroupRounds; ++r) {
-            const uint32_t e = c0 + 64 * r + lane;
-            k[r] = e < b ? kin[e] : 0u;
-        }
-#pragma unroll
-        for (uint32_t r = 0; r < kGroupRounds; ++r)
-            if (c0 + 64 * r + lane < b) atomicAdd(&cnt[k[r] & fmask], 1u);
-    }
-    __syncthreads();
-    // exclusive offsets, 64 roots at a time; runs and their ends
-    const uint64_t base = (uint64_t)d * fine;
-    uint32_t carry = 0;
-    for (uint32_t f0 = 0; f0 < fine; f0 += 64) {
-        const uint32_t f = f0 + lane;
-        const uint32_t c = f < fine ? cnt[f] : 0u;
-        const uint32_t inc = wave_inclusive_scan(c);
-        const uint32_t off = carry + inc - c;
-        if (f < fine) {
-            cnt[f] = off;
-            if (base + f < (uint64_t)nn) {
-                A.pstart[ro + base + f] = a + off;
-                A.pend[ro + base + f] = a + off + c;
-            }
-        }
-        carry += __shfl(inc, 63, 64);
-    }
-    __syncthreads();
-    const unsigned long long below = (1ull << lane) - 1ull;
-    for (uint32_t c0 = a; c0 < b; c0 += 64 * kGroupRounds) {
-        uint32_t k[kGroupRounds], v[kGroupRounds];
-#pragma unroll
-        for (uint32_t r = 0; r < kGroupRounds; ++r) {
-            const uint32_t e = c0 + 64 * r + lane;
-            k[r] = e < b ? kin[e] : 0u;
-            v[r] = e < b ? vin[e] : kUnset;
-        }
-        // everything the placement writes, loaded before the ranking rounds
-        uint8_t tch[kGroupRounds];
-        uint32_t rb[kGroupRounds], re[kGroupRounds];
-#pragma unroll
-        for (uint32_t r = 0; r < kGroupRounds; ++r) {
-            const bool ok = v[r] < nn;
-            const uint2 q = ok ? A.rows[ro + v[r]] : make_uint2(0u, 0u);
-            tch[r] = (uint8_t)(q.y >> 31);
-            rb[r] = q.x;
-            re[r] = q.y & 0x7fffffffu;
-        }
-#pragma unroll
-        for (uint32_t r = 0; r < kGroupRounds; ++r) {
-            const bool valid = c0 + 64 * r + lane < b;
-            if (__ballot(valid) == 0ull) break;  // uniform
-            const uint32_t f = k[r] & fmask;
-            unsigned long long m = __ballot(valid);
-#pragma unroll
-            for (uint32_t bit = 0; bit < FB; ++bit) {
-                const bool on = (f >> bit) & 1u;
-                const unsigned long long bm = __ballot(valid && on);
-                m &= on ? bm : ~bm;
-            }
-            const uint32_t rank = (uint32_t)__popcll(m & below);
-            const uint32_t start = valid ? cnt[f] : 0u;
-            if (valid && rank == 0) cnt[f] = start + (uint32_t)__popcll(m);
-            __builtin_amdgcn_wave_barrier();
-            const uint32_t pos = a + start + rank;
-            if (valid && pos < b) {
-                const uint32_t x = v[r];
-                kout[pos] = k[r];
-                vout[pos] = x;
-                uint32_t len = 0;
-                if (tch[r] && k[r] < nn && x < nn) {
-                    A.loc[ro + x] = pos;
-                    len = re[r] > rb[r] ? re[r] - rb[r] : 0u;
-                }
-                A.rcnt[ro + pos] = len;
-            }
-        }
-    }
-}
-
-// The same grouping with four wavefronts per super-bucket, for the wide
-// fine-bit widths (12 and 14 bits: 16 and 64 KB of counters, which leave room
-// for only a few one-wavefront blocks per CU).  Counting and the offset scan
-// use the whole block.  The placement stays stable: a pass takes
-// 4 x kGroupRounds x 64 consecutive particles, wavefront w the w-th quarter;
-// every wavefront ranks its particles (ballots) at once, then the wavefronts
-// read and advance the LDS cursors in turn, w = 0..3 (the only serialised
-// part), and write their particles at once.
 constexpr int kGroupWaves = 4;
 
-template <uint32_t FB>
+template <uint32_t FB, uint32_t R = kGroupRounds>
 __global__ void __launch_bounds__(64 * kGroupWaves)
 rps_group_blk_kernel(int64_t n, RpsArrays A, RpsStrides st, const uint32_t* __restrict__ kin,
                      const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
@@ -424,15 +315,15 @@ rps_group_blk_kernel(int64_t n, RpsArrays A, RpsStrides st, const uint32_t* __re
     const uint32_t fmask = fine - 1u;
     for (uint32_t f = t; f < fine; f += NT) cnt[f] = 0;
     __syncthreads();
-    for (uint32_t c0 = a; c0 < b; c0 += NT * kGroupRounds) {
-        uint32_t k[kGroupRounds];
+    for (uint32_t c0 = a; c0 < b; c0 += NT * R) {
+        uint32_t k[R];
 #pragma unroll
-        for (uint32_t r = 0; r < kGroupRounds; ++r) {
+        for (uint32_t r = 0; r < R; ++r) {
             const uint32_t e = c0 + NT * r + t;
             k[r] = e < b ? kin[e] : 0u;
         }
 #pragma unroll
-        for (uint32_t r = 0; r < kGroupRounds; ++r)
+        for (uint32_t r = 0; r < R; ++r)
             if (c0 + NT * r + t < b) atomicAdd(&cnt[k[r] & fmask], 1u);
     }
     __syncthreads();
@@ -455,19 +346,19 @@ rps_group_blk_kernel(int64_t n, RpsArrays A, RpsStrides st, const uint32_t* __re
     }
     __syncthreads();
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (uint32_t c0 = a; c0 < b; c0 += NT * kGroupRounds) {  // uniform over the block
-        const uint32_t w0 = c0 + 64u * kGroupRounds * wave;     // this wavefront's quarter
-        uint32_t k[kGroupRounds], v[kGroupRounds];
+    for (uint32_t c0 = a; c0 < b; c0 += NT * R) {  // uniform over the block
+        const uint32_t w0 = c0 + 64u * R * wave;     // this wavefront's quarter
+        uint32_t k[R], v[R];
 #pragma unroll
-        for (uint32_t r = 0; r < kGroupRounds; ++r) {
+        for (uint32_t r = 0; r < R; ++r) {
             const uint32_t e = w0 + 64 * r + lane;
             k[r] = e < b ? kin[e] : 0u;
             v[r] = e < b ? vin[e] : kUnset;
         }
-        uint8_t tch[kGroupRounds];
-        uint32_t rb[kGroupRounds], re[kGroupRounds];
+        uint8_t tch[R];
+        uint32_t rb[R], re[R];
 #pragma unroll
-        for (uint32_t r = 0; r < kGroupRounds; ++r) {
+        for (uint32_t r = 0; r < R; ++r) {
             const bool ok = v[r] < nn;
             const uint2 q = ok ? A.rows[ro + v[r]] : make_uint2(0u, 0u);
             tch[r] = (uint8_t)(q.y >> 31);
@@ -475,9 +366,9 @@ rps_group_blk_kernel(int64_t n, RpsArrays A, RpsStrides st, const uint32_t* __re
             re[r] = q.y & 0x7fffffffu;
         }
         // ranks among equal roots within the wavefront's rounds (no LDS)
-        uint32_t rank[kGroupRounds], tot[kGroupRounds];
+        uint32_t rank[R], tot[R];
 #pragma unroll
-        for (uint32_t r = 0; r < kGroupRounds; ++r) {
+        for (uint32_t r = 0; r < R; ++r) {
             const bool valid = w0 + 64 * r + lane < b;
             const uint32_t f = k[r] & fmask;
             unsigned long long m = __ballot(valid);
@@ -491,11 +382,11 @@ rps_group_blk_kernel(int64_t n, RpsArrays A, RpsStrides st, const uint32_t* __re
             tot[r] = valid ? (uint32_t)__popcll(m) : 0u;
         }
         // cursors in wavefront order, rounds in order within a wavefront
-        uint32_t start[kGroupRounds];
+        uint32_t start[R];
         for (uint32_t w = 0; w < (uint32_t)kGroupWaves; ++w) {
             if (wave == w) {
 #pragma unroll
-                for (uint32_t r = 0; r < kGroupRounds; ++r) {
+                for (uint32_t r = 0; r < R; ++r) {
                     const bool valid = w0 + 64 * r + lane < b;
                     const uint32_t f = k[r] & fmask;
                     start[r] = valid ? cnt[f] : 0u;
@@ -508,7 +399,7 @@ rps_group_blk_kernel(int64_t n, RpsArrays A, RpsStrides st, const uint32_t* __re
             __syncthreads();
         }
 #pragma unroll
-        for (uint32_t r = 0; r < kGroupRounds; ++r) {
+        for (uint32_t r = 0; r < R; ++r) {
             const bool valid = w0 + 64 * r + lane < b;
             const uint32_t pos = a + start[r] + rank[r];
             if (valid && pos < b) {
@@ -1247,9 +1138,9 @@ static int rps_prepare_impl(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int3
             in = 1;
         }
         if (fb == (int)lm::kGroupFineBits) {
-            LM_LAUNCH(lm::K_RPS_GROUP, lm::rps_group_kernel<lm::kGroupFineBits>, dim3(1u << super_bits, S), dim3(64),
-                      0, s, n, A, sp, (const uint32_t*)pk[in], (const uint32_t*)pv[in], pk[1 - in], pv[1 - in],
-                      dstart, ws->sort.row, dstride, 1u << fine_bits);
+            LM_LAUNCH(lm::K_RPS_GROUP, (lm::rps_group_blk_kernel<lm::kGroupFineBits, 2>), dim3(1u << super_bits, S),
+                      dim3(64 * lm::kGroupWaves), 0, s, n, A, sp, (const uint32_t*)pk[in], (const uint32_t*)pv[in],
+                      pk[1 - in], pv[1 - in], dstart, ws->sort.row, dstride, 1u << fine_bits);
         } else if (fb == (int)lm::kGroupFineBitsWide) {
             LM_LAUNCH(lm::K_RPS_GROUP, lm::rps_group_blk_kernel<lm::kGroupFineBitsWide>, dim3(1u << super_bits, S),
                       dim3(64 * lm::kGroupWaves), 0, s, n, A, sp, (const uint32_t*)pk[in], (const uint32_t*)pv[in],
