@@ -538,11 +538,20 @@ rps_deps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
 // the masks of rps_deps, so it is exactly the longest-path depth), then the
 // pairs counting-sorted by level.  Pairs of one level share no particle, so a
 // replay may run each level in parallel, levels in order.
+//
+// LDS: 16 KB, so that many blocks stay resident.  The per-slot last level
+// (uint16) is dead once the levels are known, so its words then hold the
+// per-level counts and cursors as packed uint16 pairs (level l in the
+// (l & 1)-th half of word l >> 1; every count and cursor is <= kLevelPairs,
+// so an add never carries into the other half).
+__device__ __forceinline__ uint32_t half_of(uint32_t word, uint32_t l) { return (word >> (16u * (l & 1u))) & 0xffffu; }
+
 __global__ void __launch_bounds__(64)
 rps_levels_kernel(int64_t n, RpsArrays A, RpsStrides st) {
-    __shared__ uint16_t last[kLevelSlots];
+    static_assert(kLevelPairs < 65536u, "level counts must fit 16 bits");
+    __shared__ uint32_t words[(kLevelSlots + 1) / 2];  // last[] during the level phase, then the packed counts
     __shared__ uint16_t lvl[kLevelPairs];
-    __shared__ uint32_t cnt[kLevelPairs];
+    uint16_t* last = reinterpret_cast<uint16_t*>(words);
     const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
     const uint32_t lane = threadIdx.x;
     const uint32_t ncomp = A.lists[4 * s + 1];
@@ -556,7 +565,6 @@ rps_levels_kernel(int64_t n, RpsArrays A, RpsStrides st) {
             continue;
         }
         for (uint32_t t = lane; t < psz; t += 64) last[t] = 0;
-        for (uint32_t t = lane; t < np; t += 64) cnt[t] = 0;
         __syncthreads();
         uint32_t dmax = 0;
         for (uint32_t w = 0; w < np; w += 64) {
@@ -584,23 +592,32 @@ rps_levels_kernel(int64_t n, RpsArrays A, RpsStrides st) {
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) dmax = max(dmax, (uint32_t)__shfl_xor(dmax, d, 64));
         __syncthreads();
-        for (uint32_t q = lane; q < np; q += 64) atomicAdd(&cnt[lvl[q] - 1u], 1u);
+        // last[] is dead: its words become the per-level counts (dmax <= np <= kLevelPairs levels)
+        for (uint32_t t = lane; t < (dmax + 1u) / 2u; t += 64) words[t] = 0u;
         __syncthreads();
-        // exclusive offsets per level (kept in cnt as cursors) and level ends
+        for (uint32_t q = lane; q < np; q += 64) {
+            const uint32_t l = lvl[q] - 1u;
+            atomicAdd(&words[l >> 1], 1u << (16u * (l & 1u)));
+        }
+        __syncthreads();
+        // exclusive offsets per level (kept as cursors in the same halves) and level ends
         uint32_t carry = 0;
         for (uint32_t l0 = 0; l0 < dmax; l0 += 64) {
-            const bool in = l0 + lane < dmax;
-            const uint32_t v = in ? cnt[l0 + lane] : 0u;
+            const uint32_t l = l0 + lane;
+            const bool in = l < dmax;
+            const uint32_t v = in ? half_of(words[l >> 1], l) : 0u;
             const uint32_t inc = wave_inclusive_scan(v);
+            __builtin_amdgcn_wave_barrier();  // every lane has read its half before any half is rewritten
             if (in) {
-                cnt[l0 + lane] = carry + inc - v;
-                A.lend[po + qb + l0 + lane] = carry + inc;
+                last[l] = (uint16_t)(carry + inc - v);
+                A.lend[po + qb + l] = carry + inc;
             }
             carry += __shfl(inc, 63, 64);
         }
         __syncthreads();
         for (uint32_t q = lane; q < np; q += 64) {
-            const uint32_t pos = atomicAdd(&cnt[lvl[q] - 1u], 1u);
+            const uint32_t l = lvl[q] - 1u;
+            const uint32_t pos = half_of(atomicAdd(&words[l >> 1], 1u << (16u * (l & 1u))), l);
             if (pos < np) A.lrec[po + qb + pos] = rec[qb + q];
         }
         if (lane == 0) A.depth[ro + c] = dmax;
