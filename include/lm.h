@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define LM_ABI_VERSION 5
+#define LM_ABI_VERSION 6
 
 #define LM_OK 0
 #define LM_ERR_ARG (-1)      /* invalid argument                             */
@@ -215,6 +215,12 @@ int lm_rps_prepare(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair
  * the same result (defaults 2048 and 2048).  No reference counterpart (set by
  * the ensemble's autotuner). */
 int lm_rps_ws_set_play_blocks(lm_rps_ws* ws, int32_t wide_blocks, int32_t narrow_blocks);
+
+/* Size the workspace's per-pair arrays for pairs_per_step pairs per step now
+ * (they are grown, never shrunk), so that a later prepare with that pair
+ * capacity does not synchronise `stream` to grow them.  No reference
+ * counterpart (buffer management of a pipelined caller). */
+int lm_rps_ws_reserve(lm_rps_ws* ws, int64_t pairs_per_step, void* stream);
 
 /* Per-run parameters of an ensemble (a device array, one entry per run): the
  * Philox key and the forward probabilities. */

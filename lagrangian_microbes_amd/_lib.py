@@ -56,6 +56,7 @@ _SIGNATURES = {
                                            c_u64, c_vp]),
     "lm_rps_play": (ctypes.c_int, [c_vp, c_i32, c_i64, c_vp, c_vp, c_vp]),
     "lm_rps_ws_set_play_blocks": (ctypes.c_int, [c_vp, c_i32, c_i32]),
+    "lm_rps_ws_reserve": (ctypes.c_int, [c_vp, c_i64, c_vp]),
     "lm_rps_component_stats": (ctypes.c_int, [c_vp, c_i32, c_vp, c_vp]),
     "lm_philox_uniforms": (ctypes.c_int, [c_i64, c_u64, c_u64, c_u64, c_u64, c_u64, c_u64, c_i32, c_vp, c_vp]),
     "lm_species_counts": (ctypes.c_int, [c_vp, c_i64, c_i32, c_i64, c_vp, c_vp]),
@@ -90,7 +91,7 @@ def load(path=LIB_PATH):
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        if L.lm_abi_version() != 5:
+        if L.lm_abi_version() != 6:
             raise RuntimeError("liblm.so ABI version mismatch; rebuild it")
         _lib = L
     return _lib

@@ -1227,6 +1227,12 @@ int lm_rps_play(lm_rps_ws* ws, int32_t step, int64_t n, const int8_t* species_in
     return LM_OK;
 }
 
+int lm_rps_ws_reserve(lm_rps_ws* ws, int64_t pairs_per_step, void* stream) {
+    LM_REQUIRE(ws != nullptr, LM_ERR_ARG, "workspace is NULL");
+    LM_REQUIRE(pairs_per_step >= 0, LM_ERR_ARG, "negative pair capacity");
+    return rps_ensure_pairs(ws, pairs_per_step, (hipStream_t)stream);
+}
+
 int lm_rps_ws_set_play_blocks(lm_rps_ws* ws, int32_t wide_blocks, int32_t narrow_blocks) {
     LM_REQUIRE(ws != nullptr, LM_ERR_ARG, "workspace is NULL");
     LM_REQUIRE(wide_blocks >= 1 && wide_blocks <= 65536 && narrow_blocks >= 1 && narrow_blocks <= 65536, LM_ERR_ARG,

@@ -386,6 +386,11 @@ class RPSUpdater:
         """Launch shape of the replay (see lm_rps_ws_set_play_blocks)."""
         check(lib().lm_rps_ws_set_play_blocks(self._ws, int(wide), int(narrow)))
 
+    def reserve(self, pairs_per_step):
+        """Size the per-pair arrays for pairs_per_step pairs per step now, so
+        that a prepare at that capacity does not stop the stream to grow them."""
+        check(lib().lm_rps_ws_reserve(self._ws, int(pairs_per_step), stream_ptr()))
+
     def play(self, step, species_in, species_out):
         """Step `step` of the last prepare: species_out = species_in after its pairs."""
         n = species_in.numel()

@@ -80,6 +80,10 @@ class InteractionEngine:
             self.pair_sets = [(torch.empty((self.steps_max, cap), dtype=torch.int32, device=self.device),
                                torch.empty((self.steps_max, cap), dtype=torch.int32, device=self.device))
                               for _ in range(self.nsets)]
+            # every set's per-pair RPS arrays too: growing them inside a pipelined
+            # run would stop the stream that prepares the next chunk
+            for r in self.rps_sets:
+                r.reserve(cap)
 
     def prepare_set(self, b, rec_x, rec_y, params, key, step0):
         """Pair search and grouping of S steps into set b (asynchronous)."""

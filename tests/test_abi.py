@@ -43,7 +43,7 @@ def test_binding_covers_header(liblm):
 
 
 def test_abi_version_and_argument_errors(liblm):
-    assert liblm.lm_abi_version() == 5
+    assert liblm.lm_abi_version() == 6
     # argument validation runs on the host before any launch: no GPU is touched
     rc = liblm.lm_advect_rk4_f64(-1, None, None, None, 0, None, 0, 1.0, None, None, 0, 0.0, 0, 0, 0, None, None)
     assert rc != 0 and b"negative" in liblm.lm_last_error()
