@@ -32,7 +32,9 @@ __device__ __forceinline__ bool sample_uv(const double* __restrict__ f, const lm
     j = j > g.ny - 2 ? g.ny - 2 : j;
     const double xi = fx - (double)i;
     const double eta = fy - (double)j;
-    const double2* r0 = reinterpret_cast<const double2*>(f) + ((size_t)j * (size_t)g.nx + (size_t)i);
+    // node index in 32 bits (the entry points require nx * ny < 2^31): one
+    // integer multiply-add instead of a 64-bit product per sample
+    const double2* r0 = reinterpret_cast<const double2*>(f) + ((uint32_t)j * (uint32_t)g.nx + (uint32_t)i);
     const double2* r1 = r0 + g.nx;
     const double2 a = r0[0], b = r0[1], c = r1[0], d = r1[1];
     const double ub = fma(xi, b.x - a.x, a.x);
@@ -142,7 +144,7 @@ __device__ __forceinline__ bool sample_uv_f32(const float* __restrict__ f, const
     j = j > g.ny - 2 ? g.ny - 2 : j;
     const float xi = (float)(fx - (double)i);
     const float eta = (float)(fy - (double)j);
-    const float2* r0 = reinterpret_cast<const float2*>(f) + ((size_t)j * (size_t)g.nx + (size_t)i);
+    const float2* r0 = reinterpret_cast<const float2*>(f) + ((uint32_t)j * (uint32_t)g.nx + (uint32_t)i);
     const float2* r1 = r0 + g.nx;
     const float2 a = r0[0], b = r0[1], c = r1[0], d = r1[1];
     const float ub = fmaf(xi, b.x - a.x, a.x);
@@ -236,6 +238,7 @@ int lm_advect_rk4_f64(int64_t n, double* lon, double* lat, const double* stage_f
     LM_REQUIRE(grid != nullptr, LM_ERR_ARG, "grid is NULL");
     LM_REQUIRE(grid->nx >= 2 && grid->ny >= 2, LM_ERR_ARG, "grid needs at least 2x2 nodes (got %dx%d)",
                grid->nx, grid->ny);
+    LM_REQUIRE((int64_t)grid->nx * grid->ny < ((int64_t)1 << 31), LM_ERR_ARG, "grid has 2^31 nodes or more");
     LM_REQUIRE(stage_stride >= 0, LM_ERR_ARG, "negative stage stride");
     LM_REQUIRE(stage_stride == 0 || stage_stride >= 2 * (int64_t)grid->nx * grid->ny, LM_ERR_ARG,
                "stage stride smaller than one field");
@@ -276,6 +279,7 @@ int lm_advect_rk4_f32(int64_t n, double* lon, double* lat, const float* stage_fi
     LM_REQUIRE(grid != nullptr, LM_ERR_ARG, "grid is NULL");
     LM_REQUIRE(grid->nx >= 2 && grid->ny >= 2, LM_ERR_ARG, "grid needs at least 2x2 nodes (got %dx%d)",
                grid->nx, grid->ny);
+    LM_REQUIRE((int64_t)grid->nx * grid->ny < ((int64_t)1 << 31), LM_ERR_ARG, "grid has 2^31 nodes or more");
     LM_REQUIRE(stage_stride >= 0, LM_ERR_ARG, "negative stage stride");
     LM_REQUIRE(stage_stride == 0 || stage_stride >= 2 * (int64_t)grid->nx * grid->ny, LM_ERR_ARG,
                "stage stride smaller than one field");
