@@ -12,6 +12,11 @@ kernel's dispatches, in launch order (``*_kernel_trace.csv``), into
 consecutive groups of N1, N2, ... launches and prints each group's average:
 the passes of a benchmark run over one workload.
 
+    python -m lagrangian_microbes_amd.profile_summary legs BENCH_JSON OUT_DIR [--command C]
+
+``legs`` renders the ``fp32_path`` and ``ensemble`` objects of a ``bench.py
+--fp32 --ensemble`` JSON line as ``bench_<workload>.md`` files in OUT_DIR.
+
 ``traffic`` reads four ``rocprofv3 --pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE``
 runs (``*_counter_collection.csv``): the workload and a plain copy of
 COPY_BYTES bytes each way (``bench.py --copy-calibration COPY_BYTES``).  Each
@@ -33,7 +38,7 @@ TIMER_NAMES = {
     "bin_keys_kernel": "bin_keys", "bin_table_kernel": "bin_table", "pair_count_kernel": "pair_count",
     "pair_emit_kernel": "pair_emit", "radix_hist_kernel": "sort_hist", "radix_scatter_kernel": "sort_scatter",
     "scan_reduce_kernel": "scan_reduce", "scan_apply_kernel": "scan_apply", "rps_init_kernel": "rps_init",
-    "rps_union_kernel": "rps_union", "rps_keys_kernel": "rps_keys", "rps_group_kernel": "rps_group",
+    "rps_union_kernel": "rps_union", "rps_keys_kernel": "rps_keys", "rps_group_blk_kernel": "rps_group",
     "rps_comps_kernel": "rps_comps", "rps_fill_kernel": "rps_fill", "rps_deps_kernel": "rps_deps",
     "rps_levels_kernel": "rps_levels", "rps_play_kernel": "rps_play", "copy16_kernel": "copy16",
     "philox_kernel": "philox", "seg_copy_kernel": "copy16",
@@ -140,6 +145,57 @@ def traffic(args):
             json.dump({"copy_bytes": nb, "fetch_scale": sf, "write_scale": sw, "kernels": res}, fh, indent=1)
 
 
+def _fmt_rate(x):
+    return f"{x:.2e}".replace("e+0", "e+")
+
+
+def legs(args):
+    d = json.load(open(args.bench_json))
+    cmd = ["Command:", "", "```", args.command, "```", ""] if args.command else []
+    f = d.get("fp32_path")
+    if f:
+        out = [f"# FP32 path: {f['workload']} ({f['particles']:,} particles, {f['steps']:,} steps of "
+               f"{f['dt_s']:g} s; field: {f['field']})", ""] + cmd
+        out += ["Fused run (advection, pair search, RPS, records) from identical inputs in both precisions, "
+                "chunks pipelined:", "", "| precision | ms/step | particle-steps/s | steps outside the field |",
+                "|---|---:|---:|---:|"]
+        for prec in ("f32", "f64"):
+            r = f[prec]
+            out.append(f"| {prec} | {r['ms_per_step']:.4f} | {_fmt_rate(r['particle_steps_per_s'])} | "
+                       f"{r['particle_steps_outside_field']} |")
+        out += ["", "Distance between the FP32 and FP64 positions (degrees):", "",
+                "| after | median | 99th percentile | max |", "|---|---:|---:|---:|"]
+        for k, v in f["f32_vs_f64_position_error"].items():
+            out.append(f"| {k.replace('after_', '').replace('_', ' ')} | {v['median_deg']:.3g} | "
+                       f"{v['p99_deg']:.3g} | {v['max_deg']:.3g} |")
+        out += ["", "JSON object:", "", "```", json.dumps(f, indent=1), "```"]
+        with open(os.path.join(args.out_dir, f"bench_{f['workload']}.md"), "w") as fh:
+            fh.write("\n".join(out) + "\n")
+    e = d.get("ensemble")
+    if e:
+        out = [f"# Ensemble: {e['workload']} ({e['runs']} runs, {e['particles']:,} particles, "
+               f"{e['steps']} steps; field: {e['field']}; FP32)", ""] + cmd
+        a = e["autotuned"]
+        out += [f"Autotuner choice: cell side {a['cell_scale']} radius, chunk {a['chunk']} steps, "
+                f"replay workgroups {a['play_blocks']}.", "",
+                f"| | seconds for {e['steps']} steps | particle-steps/s |", "|---|---:|---:|",
+                f"| batched ensemble (best of two) | {e['ms_per_step'] * e['steps'] / 1e3:.3f} | "
+                f"{_fmt_rate(e['particle_steps_per_s'])} |",
+                f"| the {e['runs']} runs alone, one after another (best of two each) | {e['runs_alone_s']:.3f} | "
+                f"{_fmt_rate(e['runs_alone_particle_steps_per_s'])} |", "",
+                f"Speedup {e['speedup_vs_runs_alone']:.2f}x. A run alone takes "
+                f"{e['runs_alone_ms_per_step']['min']}-{e['runs_alone_ms_per_step']['max']} ms per step "
+                f"(median {e['runs_alone_ms_per_step']['median']}).", "",
+                "Autotuner table (cell scale, chunk, replay workgroups, ms/step over 24 steps), fastest first:", "",
+                "| cell scale | chunk | replay (wide, narrow) | ms/step |", "|---:|---:|---|---:|"]
+        for c, k, pb, t in sorted(e["tuning_ms_per_step"], key=lambda r: r[3]):
+            out.append(f"| {c} | {k} | ({pb[0]}, {pb[1]}) | {t} |")
+        slim = {k: v for k, v in e.items() if k != "tuning_ms_per_step"}
+        out += ["", "JSON object (without the tuning table):", "", "```", json.dumps(slim, indent=1), "```"]
+        with open(os.path.join(args.out_dir, f"bench_{e['workload']}.md"), "w") as fh:
+            fh.write("\n".join(out) + "\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -154,8 +210,12 @@ def main():
     c.add_argument("copy_bytes", type=int)
     c.add_argument("--json")
     c.add_argument("--command", default="")
+    g = sub.add_parser("legs")
+    g.add_argument("bench_json")
+    g.add_argument("out_dir")
+    g.add_argument("--command", default="")
     args = ap.parse_args()
-    (trace if args.cmd == "trace" else traffic)(args)
+    {"trace": trace, "traffic": traffic, "legs": legs}[args.cmd](args)
 
 
 if __name__ == "__main__":
