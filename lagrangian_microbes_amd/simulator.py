@@ -28,6 +28,26 @@ from .velocity_fields import VelocityField
 logger = logging.getLogger(__name__)
 
 
+# Smallest chunk the tapered end of a pipelined run uses.
+TAPER_MIN = 8
+
+
+def chunk_schedule(total, chunk, taper_min=TAPER_MIN):
+    """Chunk lengths of a pipelined run of `total` steps: full chunks, then
+    the last (up to) two chunks' worth of steps in halving chunks down to
+    taper_min.  Each chunk's replay overlaps the next chunk's preparation, so
+    only the last replay is exposed; a short last chunk keeps it short."""
+    sizes, left = [], int(total)
+    while left > 2 * chunk:
+        sizes.append(chunk)
+        left -= chunk
+    while left > 0:
+        k = min(chunk, left if left <= taper_min else max(taper_min, (left + 1) // 2))
+        sizes.append(k)
+        left -= k
+    return sizes
+
+
 class FusedRunner:
     """Device state and buffers for a fused run.
 
@@ -113,9 +133,9 @@ class FusedRunner:
         eng = self.engine
         main = torch.cuda.current_stream(self.device)
         side = self._side
-        sizes = []
-        while sum(sizes) < total:
-            sizes.append(min(self.chunk, total - sum(sizes)))
+        sizes = chunk_schedule(total, self.chunk)
+        if max(sizes) > self.chunk:  # the chunk buffers hold self.chunk steps
+            raise RuntimeError(f"chunk schedule exceeds the buffers: {sizes}")
         step0 = [self.step + sum(sizes[:c]) for c in range(len(sizes))]
         clock0 = [self.clock_steps + sum(sizes[:c]) for c in range(len(sizes))]
         prepared = [None] * len(sizes)   # event after chunk c's preparation

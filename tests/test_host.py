@@ -160,3 +160,14 @@ def test_autotune_picks_fastest_candidate():
     best, table = autotune.tune(None, steps=4, cands=cands, timer=fake_timer)
     assert best == {"cell_scale": 1.5, "chunk": 12, "play_blocks": (1024, 1024)}
     assert len(table) == 27 and seen == cands
+
+
+def test_chunk_schedule_tapers_the_end():
+    from lagrangian_microbes_amd.simulator import chunk_schedule
+    assert chunk_schedule(720, 100) == [100] * 6 + [60, 30, 15, 8, 7]
+    assert chunk_schedule(24, 5) == [5, 5, 5, 5, 4]  # chunks at or below the taper floor are kept
+    assert chunk_schedule(0, 100) == []
+    for total in range(0, 400, 7):
+        for chunk in (1, 8, 24, 100):
+            s = chunk_schedule(total, chunk)
+            assert sum(s) == total and all(1 <= k <= chunk for k in s)
