@@ -295,9 +295,11 @@ pair_count_kernel(int64_t n, CellGrid g, Metric m, Strides st, const uint32_t* _
             // differences of nearby positions are exact, and m.box is above
             // r, so the prefilter only drops pairs the exact test drops.
             // One predicate, evaluated without branches; one branch for the rare store.
-            const bool cand = (t0 + u < total) & (j > i) & (__float_as_uint(o[u].w) == my_run) &
-                              (fabsf(o[u].x - me.x) <= m.box) & (fabsf(o[u].y - me.y) <= m.box);
-            const bool hit = cand & within<KIND>(m, me.x, me.y, o[u].x, o[u].y);
+            const float ax = fabsf(o[u].x - me.x), ay = fabsf(o[u].y - me.y);
+            const bool cand = (t0 + u < total) & (j > i) & (__float_as_uint(o[u].w) == my_run) & (ax <= m.box) &
+                              (ay <= m.box);
+            const bool near = within<KIND>(m, me.x, me.y, o[u].x, o[u].y);
+            const bool hit = cand & near;
             if (hit & (cnt < kNbr)) row[cnt] = j;
             cnt += hit ? 1u : 0u;
         }

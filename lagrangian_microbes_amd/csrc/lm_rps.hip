@@ -873,24 +873,25 @@ rps_play_kernel(int64_t n, int64_t s, RpsArrays A, RpsStrides st, const int8_t* 
             play_narrow(n, A.ids + ro, A.rec + po, A.narrow[ro + c], sp_in, sp_out, my);
         return;
     }
-    // particles without pairs (loc unset) keep their species; the others are
-    // written by their component
+    // particles without pairs (touched 0, the same particles whose loc is
+    // unset) keep their species; the others are written by their component
     if (sp_in == sp_out) return;
-    const uint32_t* loc = A.loc + ro;
+    const uint8_t* touched = A.touched + ro;
     const int64_t x0 = (int64_t)kCopyItems * ((int64_t)(bid - nwide - nnarrow) * 64 + lane);
     if (x0 + kCopyItems <= n) {
-        const uint4 l0 = *reinterpret_cast<const uint4*>(loc + x0);  // rows are 16-byte aligned
-        const uint4 l1 = *reinterpret_cast<const uint4*>(loc + x0 + 4);
+        static_assert(kCopyItems == 8, "two 4-byte words of flags");
+        // step rows are multiples of 4 bytes and x0 of 8, so both words are aligned
+        const uint32_t t0 = *reinterpret_cast<const uint32_t*>(touched + x0);
+        const uint32_t t1 = *reinterpret_cast<const uint32_t*>(touched + x0 + 4);
         int8_t v[kCopyItems];
 #pragma unroll
         for (int q = 0; q < kCopyItems; ++q) v[q] = sp_in[x0 + q];
-        const uint32_t lv[kCopyItems] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
 #pragma unroll
         for (int q = 0; q < kCopyItems; ++q)
-            if (lv[q] == kUnset) sp_out[x0 + q] = v[q];
+            if ((((q < 4 ? t0 : t1) >> (8 * (q & 3))) & 0xffu) == 0u) sp_out[x0 + q] = v[q];
     } else {
         for (int64_t x = x0; x < n; ++x)
-            if (loc[x] == kUnset) sp_out[x] = sp_in[x];
+            if (touched[x] == 0) sp_out[x] = sp_in[x];
     }
 }
 
