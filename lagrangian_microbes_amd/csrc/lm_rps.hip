@@ -72,11 +72,10 @@ struct RpsStrides {
 struct RpsArrays {
     uint32_t *parent, *loc;               // per particle
     uint32_t *pstart, *pend;              // per root: its run of particles [pstart, pend)
-    uint32_t *rowbeg, *rowend;            // per particle: its row [rowbeg, rowend) of pairs
     uint32_t *rcnt, *roff;                // per sorted position: row length, run offset
     uint2* info;                          // per particle: {run position - pair rank, slot}
     uint8_t* touched;                     // per particle
-    uint2* rows;                          // per particle: {rowbeg, rowend | touched << 31} (one gather)
+    uint2* rows;                          // per particle: its row [x, y) of pairs; y | touched << 31 once keyed
     uint32_t *pkey, *pval, *ptk, *ptv;    // particle sort buffers
     const uint32_t *spk, *ids;            // sorted roots / particle indices (two of the above)
     uint4 *narrow, *wide;                 // component lists (per particle)
@@ -175,8 +174,7 @@ rps_init_kernel(int64_t n, RpsArrays A, RpsStrides st) {
         A.parent[ro + v] = (uint32_t)v;
         A.loc[ro + v] = kUnset;
         A.touched[ro + v] = 0;
-        A.rowbeg[ro + v] = 0;
-        A.rowend[ro + v] = 0;
+        A.rows[ro + v] = make_uint2(0u, 0u);
     }
     if (v < 4) A.lists[4 * s + v] = 0;
 }
@@ -222,8 +220,9 @@ rps_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* _
         const double u = u01_from_u64(philox4x64_10(rank, step0 + (uint64_t)s, kStreamPairs, 0, q0, q1).v[0]);
         bits[k] = (uint8_t)((u < a_rs ? 1u : 0u) | (u < a_pr ? 2u : 0u) | (u < a_sp ? 4u : 0u));
         if (a < (uint64_t)n) {
-            if (k == 0 || (uint32_t)pi[k - 1] != a) A.rowbeg[ro + a] = (uint32_t)k;
-            if (k + 1 == P || (uint32_t)pi[k + 1] != a) A.rowend[ro + a] = (uint32_t)(k + 1);
+            uint32_t* row = reinterpret_cast<uint32_t*>(A.rows + ro + a);  // {begin, end}
+            if (k == 0 || (uint32_t)pi[k - 1] != a) row[0] = (uint32_t)k;
+            if (k + 1 == P || (uint32_t)pi[k + 1] != a) row[1] = (uint32_t)(k + 1);
             touched[a] = 1;
             if (b < (uint64_t)n) {
                 touched[b] = 1;
@@ -263,8 +262,9 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
         // an untouched particle is keyed by itself: a run of one, never listed
         const uint8_t t = A.touched[ro + v];
         A.pkey[ro + v] = t ? find_root(A.parent + ro, (uint32_t)v) : (uint32_t)v;
-        // what the grouping gathers per particle, in one 8-byte record (row ends stay below 2^29)
-        A.rows[ro + v] = make_uint2(A.rowbeg[ro + v], A.rowend[ro + v] | (t ? 0x80000000u : 0u));
+        // the touched flag joins the row record, so the grouping gathers both in one 8-byte load
+        // (row ends stay below 2^29)
+        if (t) reinterpret_cast<uint32_t*>(A.rows + ro + v)[1] |= 0x80000000u;
         if (write_vals) A.pval[ro + v] = (uint32_t)v;  // else the radix pass generates the indices
     }
 }
@@ -456,7 +456,7 @@ rps_comps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
         const int64_t ec = e < n ? e : (int64_t)last;
         lc[r] = A.loc[ro + x[r]];
         pb[r] = A.pstart[ro + L[r]];
-        rbg[r] = A.rowbeg[ro + x[r]];
+        rbg[r] = reinterpret_cast<const uint32_t*>(A.rows + ro + x[r])[0];  // row begin
         ro_e[r] = roff[ec];
     }
 #pragma unroll
@@ -989,7 +989,7 @@ static void rps_free_pairs(lm_rps_ws* w) {
 static void rps_ws_free(lm_rps_ws* w) {
     if (!w) return;
     lm::RpsArrays& A = w->A;
-    for (uint32_t* p : {A.parent, A.loc, A.pstart, A.pend, A.rowbeg, A.rowend, A.rcnt, A.roff, A.pkey, A.pval, A.ptk, A.ptv,
+    for (uint32_t* p : {A.parent, A.loc, A.pstart, A.pend, A.rcnt, A.roff, A.pkey, A.pval, A.ptk, A.ptv,
                         A.lists, w->scan_tmp})
         (void)hipFree(p);
     (void)hipFree(A.info);
@@ -1050,8 +1050,7 @@ int lm_rps_ws_create(int64_t n_max, int32_t steps_max, lm_rps_ws** out) {
     int rc;
     if ((rc = rws_alloc(&A.parent, R, &w->bytes)) || (rc = rws_alloc(&A.loc, R, &w->bytes)) ||
         (rc = rws_alloc(&A.pend, R, &w->bytes)) || (rc = rws_alloc(&A.pstart, R, &w->bytes)) ||
-        (rc = rws_alloc(&A.rowbeg, R, &w->bytes)) ||
-        (rc = rws_alloc(&A.rowend, R, &w->bytes)) || (rc = rws_alloc(&A.rcnt, R, &w->bytes)) ||
+        (rc = rws_alloc(&A.rcnt, R, &w->bytes)) ||
         (rc = rws_alloc(&A.roff, R, &w->bytes)) || (rc = rws_alloc(&A.info, R, &w->bytes)) ||
         (rc = rws_alloc(&A.rows, R, &w->bytes)) ||
         (rc = rws_alloc(&A.touched, R, &w->bytes)) || (rc = rws_alloc(&A.pkey, R, &w->bytes)) ||
