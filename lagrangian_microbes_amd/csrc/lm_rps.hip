@@ -7,7 +7,7 @@
 //
 // Everything that depends only on the pairs runs once for a batch of steps
 // (blockIdx.y = step), with the parallelism of all steps at once:
-//   prepare 1. init      parent[v] = v; touched marks, slots and rows cleared
+//   prepare 1. init      parent[v] = v; touched marks and rows cleared
 //           2. union     lock-free union-find, hooking the larger root under the
 //                        smaller, so every root is the smallest index of its
 //                        component; the same pass draws each pair's uniform
@@ -70,7 +70,7 @@ struct RpsStrides {
 // per-pair array at s * pair.  A component is described by a uint4
 // {first pair, pairs, first particle, particles} of its runs.
 struct RpsArrays {
-    uint32_t *parent, *loc;               // per particle
+    uint32_t* parent;                     // per particle
     uint32_t *pstart, *pend;              // per root: its run of particles [pstart, pend)
     uint32_t *rcnt, *roff;                // per sorted position: row length, run offset
     uint2* info;                          // per particle: {run position - pair rank, slot}
@@ -172,7 +172,6 @@ rps_init_kernel(int64_t n, RpsArrays A, RpsStrides st) {
     const int64_t s = blockIdx.y, ro = s * st.row;
     if (v < n) {
         A.parent[ro + v] = (uint32_t)v;
-        A.loc[ro + v] = kUnset;
         A.touched[ro + v] = 0;
         A.rows[ro + v] = make_uint2(0u, 0u);
     }
@@ -272,9 +271,8 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
 // Second level of the grouping by root: one block of four wavefronts per
 // super-bucket (the particles whose root has the same high bits, made
 // contiguous, in index order, by one stable radix pass) counts its roots'
-// particles in LDS and writes each root's run [pstart, pend), every touched
-// particle's position in the run order (loc) and the row lengths in that
-// order.  Counting and the offset scan use the whole block.  The placement is
+// particles in LDS and writes each root's run [pstart, pend) and the row
+// lengths in that order.  Counting and the offset scan use the whole block.  The placement is
 // stable -- a run keeps index order, which the pair order relies on: a pass
 // takes 4 x R x 64 consecutive particles, wavefront w the w-th quarter in R
 // rounds of 64; a lane's rank among equal roots in its round comes from one
@@ -406,11 +404,7 @@ rps_group_blk_kernel(int64_t n, RpsArrays A, RpsStrides st, const uint32_t* __re
                 const uint32_t x = v[r];
                 kout[pos] = k[r];
                 vout[pos] = x;
-                uint32_t len = 0;
-                if (tch[r] && k[r] < nn && x < nn) {
-                    A.loc[ro + x] = pos;
-                    len = re[r] > rb[r] ? re[r] - rb[r] : 0u;
-                }
+                const uint32_t len = tch[r] && k[r] < nn && x < nn && re[r] > rb[r] ? re[r] - rb[r] : 0u;
                 A.rcnt[ro + pos] = len;
             }
         }
@@ -449,14 +443,15 @@ rps_comps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
         L[r] = L[r] < nn ? L[r] : 0u;
         x[r] = x[r] < nn ? x[r] : 0u;
     }
-    uint32_t lc[kCompItems], pb[kCompItems], rbg[kCompItems], ro_e[kCompItems];
+    uint32_t tch[kCompItems], pb[kCompItems], rbg[kCompItems], ro_e[kCompItems];
 #pragma unroll
     for (int r = 0; r < kCompItems; ++r) {
         const int64_t e = ((int64_t)blockIdx.x * kCompItems + r) * kBlock + threadIdx.x;
         const int64_t ec = e < n ? e : (int64_t)last;
-        lc[r] = A.loc[ro + x[r]];
+        const uint2 rw = A.rows[ro + x[r]];  // {row begin, row end | touched << 31}
+        tch[r] = rw.y >> 31;
+        rbg[r] = rw.x;
         pb[r] = A.pstart[ro + L[r]];
-        rbg[r] = reinterpret_cast<const uint32_t*>(A.rows + ro + x[r])[0];  // row begin
         ro_e[r] = roff[ec];
     }
 #pragma unroll
@@ -464,7 +459,7 @@ rps_comps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
         const int64_t e = ((int64_t)blockIdx.x * kCompItems + r) * kBlock + threadIdx.x;
         at[r] = kUnset;
         d[r] = make_uint4(0u, 0u, 0u, 0u);
-        if (!ok[r] || lc[r] == kUnset) continue;  // past the end or untouched
+        if (!ok[r] || !tch[r]) continue;  // past the end or untouched
         A.info[ro + x[r]] = make_uint2(ro_e[r] - rbg[r], (uint32_t)e - pb[r]);
         if ((uint32_t)e == pb[r]) {
             const uint32_t pe = A.pend[ro + L[r]];
@@ -873,8 +868,8 @@ rps_play_kernel(int64_t n, int64_t s, RpsArrays A, RpsStrides st, const int8_t* 
             play_narrow(n, A.ids + ro, A.rec + po, A.narrow[ro + c], sp_in, sp_out, my);
         return;
     }
-    // particles without pairs (touched 0, the same particles whose loc is
-    // unset) keep their species; the others are written by their component
+    // particles without pairs (touched 0) keep their species; the others
+    // are written by their component
     if (sp_in == sp_out) return;
     const uint8_t* touched = A.touched + ro;
     const int64_t x0 = (int64_t)kCopyItems * ((int64_t)(bid - nwide - nnarrow) * 64 + lane);
@@ -989,7 +984,7 @@ static void rps_free_pairs(lm_rps_ws* w) {
 static void rps_ws_free(lm_rps_ws* w) {
     if (!w) return;
     lm::RpsArrays& A = w->A;
-    for (uint32_t* p : {A.parent, A.loc, A.pstart, A.pend, A.rcnt, A.roff, A.pkey, A.pval, A.ptk, A.ptv,
+    for (uint32_t* p : {A.parent, A.pstart, A.pend, A.rcnt, A.roff, A.pkey, A.pval, A.ptk, A.ptv,
                         A.lists, w->scan_tmp})
         (void)hipFree(p);
     (void)hipFree(A.info);
@@ -1048,7 +1043,7 @@ int lm_rps_ws_create(int64_t n_max, int32_t steps_max, lm_rps_ws** out) {
     const int64_t S = steps_max, R = S * w->row;
     lm::RpsArrays& A = w->A;
     int rc;
-    if ((rc = rws_alloc(&A.parent, R, &w->bytes)) || (rc = rws_alloc(&A.loc, R, &w->bytes)) ||
+    if ((rc = rws_alloc(&A.parent, R, &w->bytes)) ||
         (rc = rws_alloc(&A.pend, R, &w->bytes)) || (rc = rws_alloc(&A.pstart, R, &w->bytes)) ||
         (rc = rws_alloc(&A.rcnt, R, &w->bytes)) ||
         (rc = rws_alloc(&A.roff, R, &w->bytes)) || (rc = rws_alloc(&A.info, R, &w->bytes)) ||
