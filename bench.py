@@ -124,7 +124,7 @@ def step_statistics(runner, steps, total, chunk):
 
     def collect(k, rx, ry, rs, cnt):
         counts = cnt.cpu().numpy()
-        rps = runner.engine.rps_sets[chunks[0] % 2]
+        rps = runner.engine.rps_sets[chunks[0] % runner.PIPELINE_SETS]
         for s in range(k):
             st = rps.component_stats(s)
             rows.append((counts[s], st.touched, st.wide_pairs, st.wide_particles))
@@ -279,7 +279,7 @@ def ensemble_leg():
     best, table = autotune.tune(make, steps=24)
     log(f"[bench] ensemble autotuner picked {best}")
     r = make(**best)
-    r.run(2 * best["chunk"])  # warm-up: both chunk buffer sets allocated, their replay graphs captured
+    r.run(FusedRunner.PIPELINE_SETS * best["chunk"])  # warm-up: every chunk buffer set's replay graph captured
     torch.cuda.synchronize()
     batched = float("inf")
     for _ in range(2):  # best of two, as for the runs alone below
@@ -296,7 +296,7 @@ def ensemble_leg():
     for w in e.runs:
         lo, la = w.particles()
         rr = FusedRunner(lo, la, w.species(), field, w.radius, w.params, w.key, w.dt, precision="f32")
-        rr.run(2 * rr.chunk)  # warm-up: both chunk buffer sets
+        rr.run(FusedRunner.PIPELINE_SETS * rr.chunk)  # warm-up: every chunk buffer set
         torch.cuda.synchronize()
         t_run = float("inf")
         for _ in range(2):  # best of two: a run alone now and then stalls for ~1 s outside its kernels

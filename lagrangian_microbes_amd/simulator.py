@@ -63,6 +63,8 @@ class FusedRunner:
     from a captured HIP graph.
     """
 
+    PIPELINE_SETS = 3  # chunk buffer sets of run(): replay, preparation, advection
+
     def __init__(self, lon0, lat0, species0, field, radius, params, key, dt_s, norm=2.0, chunk=100, Kh=0.0,
                  walk_key=(0, 0x6B68), device=None, bbox=None, precision="f64", runs=None, cell_scale=1.0,
                  play_blocks=None, graphs=False):
@@ -79,7 +81,8 @@ class FusedRunner:
         self.walk_key = tuple(walk_key)
         self.dfield = dev.DeviceField(field, self.device, precision)
         self.engine = InteractionEngine(self.n, bbox or field.bounds, radius, norm, self.device,
-                                        steps_max=self.chunk, nsets=2, runs=runs, cell_scale=cell_scale,
+                                        steps_max=self.chunk, nsets=self.PIPELINE_SETS, runs=runs,
+                                        cell_scale=cell_scale,
                                         play_blocks=play_blocks, graphs=graphs)
         self.rec_x = torch.empty((self.chunk, self.n), dtype=torch.float32, device=self.device)
         self.rec_y = torch.empty((self.chunk, self.n), dtype=torch.float32, device=self.device)
@@ -118,10 +121,13 @@ class FusedRunner:
     def run(self, nsteps, on_chunk=None):
         """Run nsteps steps with consecutive chunks overlapped on the device.
 
-        Advection, pair search and grouping need positions only, so chunk
-        c + 1 is prepared on a side stream while chunk c replays the species
-        on the caller's stream (two sets of chunk buffers alternate).  After
-        each chunk, ``on_chunk(k, rec_x, rec_y, rec_s, npairs)`` receives that
+        Three streams, three sets of chunk buffers (set c % 3 for chunk c):
+        the advection of chunk c + 2 (FP64 arithmetic) runs on its own stream
+        beside the pair search and grouping of chunk c + 1 (memory-latency
+        bound) on a side stream, while chunk c replays the species on the
+        caller's stream.  The advection of a chunk waits for the replay that
+        last used its set; the grouping waits for its advection.  After each
+        chunk, ``on_chunk(k, rec_x, rec_y, rec_s, npairs)`` receives that
         chunk's records (device views, valid until the callback returns).
         The state and step counters end as after ``advance`` calls over the
         same steps; returns the number of steps done.
@@ -131,40 +137,50 @@ class FusedRunner:
             return 0
         self._ensure_pipeline()
         eng = self.engine
+        nsets = self.PIPELINE_SETS
         main = torch.cuda.current_stream(self.device)
-        side = self._side
+        side, adv = self._side, self._adv
         sizes = chunk_schedule(total, self.chunk)
         if max(sizes) > self.chunk:  # the chunk buffers hold self.chunk steps
             raise RuntimeError(f"chunk schedule exceeds the buffers: {sizes}")
         step0 = [self.step + sum(sizes[:c]) for c in range(len(sizes))]
         clock0 = [self.clock_steps + sum(sizes[:c]) for c in range(len(sizes))]
+        advected = [None] * len(sizes)   # event after chunk c's advection
         prepared = [None] * len(sizes)   # event after chunk c's preparation
-        played = [None, None]            # event after the replay that last used set b
+        played = [None] * nsets          # event after the replay that last used set b
         side.wait_stream(main)           # the state as the caller left it
+        adv.wait_stream(main)
+
+        def advect(c):
+            b, k = c % nsets, sizes[c]
+            with torch.cuda.stream(adv):
+                if played[b] is not None:
+                    adv.wait_event(played[b])
+                self.dfield.advect(self.lon, self.lat, clock0[c] * self.dt_s, self.dt_s, k, self._rec_sets[b][0],
+                                   self._rec_sets[b][1], self.n, self.walk_amp, self.walk_key, step0[c], self.oob)
+                ev = torch.cuda.Event()
+                ev.record(adv)
+                advected[c] = ev
 
         def group(c):
-            b, k = c % 2, sizes[c]
+            b, k = c % nsets, sizes[c]
             with torch.cuda.stream(side):
+                side.wait_event(advected[c])
                 eng.prepare_set(b, self._rec_sets[b][0][:k], self._rec_sets[b][1][:k], self.params, self.key,
                                 step0[c])
                 ev = torch.cuda.Event()
                 ev.record(side)
                 prepared[c] = ev
 
-        def prepare(c):
-            b, k = c % 2, sizes[c]
-            with torch.cuda.stream(side):
-                if played[b] is not None:
-                    side.wait_event(played[b])
-                self.dfield.advect(self.lon, self.lat, clock0[c] * self.dt_s, self.dt_s, k, self._rec_sets[b][0],
-                                   self._rec_sets[b][1], self.n, self.walk_amp, self.walk_key, step0[c], self.oob)
-            group(c)
-
-        prepare(0)
+        for c in range(min(2, len(sizes))):
+            advect(c)
+        group(0)
         for c, k in enumerate(sizes):
-            b = c % 2
+            b = c % nsets
             if c + 1 < len(sizes):
-                prepare(c + 1)
+                group(c + 1)
+            if c + 2 < len(sizes):
+                advect(c + 2)
             main.wait_event(prepared[c])
             worst = eng.worst_count(b, k)
             if worst > eng.capacity:
@@ -186,13 +202,16 @@ class FusedRunner:
             self.step += k
             self.clock_steps += k
         main.wait_stream(side)
+        main.wait_stream(adv)
         return total
 
     def _ensure_pipeline(self):
         if getattr(self, "_side", None) is None:
-            second = tuple(torch.empty_like(t) for t in (self.rec_x, self.rec_y, self.rec_s))
-            self._rec_sets = [(self.rec_x, self.rec_y, self.rec_s), second]
+            extra = [tuple(torch.empty_like(t) for t in (self.rec_x, self.rec_y, self.rec_s))
+                     for _ in range(self.PIPELINE_SETS - 1)]
+            self._rec_sets = [(self.rec_x, self.rec_y, self.rec_s)] + extra
             self._side = torch.cuda.Stream(self.device)
+            self._adv = torch.cuda.Stream(self.device)
 
     def check_domain(self):
         n_oob = int(self.oob.item())

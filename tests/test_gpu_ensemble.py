@@ -101,8 +101,8 @@ def test_graph_replay_matches_eager(gpu):
         r.run(e.steps, lambda k, rx, ry, rs, cnt: recs.append(rs.cpu().numpy().copy()))
         out[graphs] = np.concatenate(recs)
         if graphs:
-            # the two buffer sets' 5-step chunks and the last 4-step chunk
-            assert len(r.engine._graphs) == 3
+            # the three buffer sets' 5-step chunks and the last 4-step chunk
+            assert len(r.engine._graphs) == 4
     assert np.array_equal(out[True], out[False])
 
 
