@@ -51,6 +51,8 @@ def test_abi_version_and_argument_errors(liblm):
     assert rc != 0 and b"word" in liblm.lm_last_error()
     rc = liblm.lm_query_pairs(None, 0, None, None, 0.1, 2.0, None, None, 0, None, None)
     assert rc != 0 and b"workspace" in liblm.lm_last_error()
+    rc = liblm.lm_rps_ws_reserve(None, 1000, None)
+    assert rc != 0 and b"workspace" in liblm.lm_last_error()
 
 
 def test_adjoint_argument_errors(liblm):
