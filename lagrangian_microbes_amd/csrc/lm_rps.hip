@@ -501,26 +501,57 @@ rps_fill_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __
 }
 
 // 64-lane blocks, one wavefront per wide component: for each window of 64
-// pairs, the bitmask of earlier lanes that share a particle slot.
+// pairs, the bitmask of earlier lanes that share a particle slot.  For a
+// component of up to kDepsTable particles, an LDS table of one 64-bit mask
+// per slot: every lane ORs its bit into the masks of its two slots, then
+// reads them back and keeps the earlier lanes' bits, then clears them (the
+// wavefront's LDS operations complete in program order, and every window
+// leaves the table zero).  Larger components compare each lane with the 63
+// before it by shuffles.
+constexpr uint32_t kDepsTable = 2048;
+
 __global__ void __launch_bounds__(64)
 rps_deps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
+    __shared__ unsigned long long mask[kDepsTable];
     const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
     const uint32_t lane = threadIdx.x;
     const uint32_t cnt = A.lists[4 * s + 1];
+    if (blockIdx.x >= cnt) return;  // uniform: no component for this block
     const uint2* rec = A.rec + po;
     unsigned long long* deps = A.deps + po;
+    for (uint32_t t = lane; t < kDepsTable; t += 64) mask[t] = 0ull;
+    __syncthreads();
+    const unsigned long long bit = 1ull << lane;
     for (uint32_t c = blockIdx.x; c < cnt; c += gridDim.x) {
         const uint4 dsc = A.wide[ro + c];
-        const uint32_t qb = dsc.x, np = dsc.y;
+        const uint32_t qb = dsc.x, np = dsc.y, psz = dsc.w;
+        const bool table = psz <= kDepsTable;  // uniform over the wavefront
         for (uint32_t w = 0; w < np; w += 64) {
             const bool live = w + lane < np;
             const uint2 r = live ? rec[qb + w + lane] : make_uint2(kUnset, kUnset - 1u);
             const uint32_t a = live ? (r.x & kSlotMask) : kUnset, b = live ? r.y : kUnset - 1u;
             unsigned long long d = 0ull;
-            for (uint32_t e = 1; e < 64; ++e) {
-                const uint32_t src = lane >= e ? lane - e : lane;
-                const uint32_t a2 = __shfl(a, src, 64), b2 = __shfl(b, src, 64);
-                if (live && lane >= e && (a2 == a || a2 == b || b2 == a || b2 == b)) d |= 1ull << (lane - e);
+            if (table) {
+                // slots are < psz <= kDepsTable; the clamp only keeps a bad record inside the table
+                const uint32_t ta = min(a, kDepsTable - 1u), tb = min(b, kDepsTable - 1u);
+                if (live) {
+                    atomicOr(&mask[ta], bit);
+                    atomicOr(&mask[tb], bit);
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (live) d = (mask[ta] | mask[tb]) & (bit - 1ull);
+                __builtin_amdgcn_wave_barrier();
+                if (live) {
+                    mask[ta] = 0ull;
+                    mask[tb] = 0ull;
+                }
+                __builtin_amdgcn_wave_barrier();
+            } else {
+                for (uint32_t e = 1; e < 64; ++e) {
+                    const uint32_t src = lane >= e ? lane - e : lane;
+                    const uint32_t a2 = __shfl(a, src, 64), b2 = __shfl(b, src, 64);
+                    if (live && lane >= e && (a2 == a || a2 == b || b2 == a || b2 == b)) d |= 1ull << (lane - e);
+                }
             }
             if (live) deps[qb + w + lane] = d;
         }
