@@ -279,10 +279,12 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
 // ballot per root bit, every wavefront ranking at once; then the wavefronts
 // read and advance the per-root LDS cursors in turn, w = 0..3, rounds in
 // order (the only serialised part), and write their particles at once.
-// FB fine bits per block: 9 up to 2^19 particles (one radix pass of at most
-// 10 bits covers the rest; R = 2, as a super-bucket holds ~2^9 particles),
+// FB fine bits per block: 10 up to 2^20 particles (one radix pass of at most
+// 10 bits covers the rest: 9 at 490k particles, whose per-tile digit
+// counters cost half the 10-bit pass's; R = 2, as a super-bucket holds
+// ~2^10 particles),
 // 12 up to 2^22, 14 up to 2^24 (ensembles; 64 KB of counters).
-constexpr uint32_t kGroupFineBits = 9;
+constexpr uint32_t kGroupFineBits = 10;
 constexpr uint32_t kGroupFineBitsWide = 12;
 constexpr uint32_t kGroupFineBitsHuge = 14;
 constexpr uint32_t kGroupRounds = 8;

@@ -260,26 +260,33 @@ static int radix_pass(uint32_t* keys[2], uint32_t* vals[2], int cur, int64_t str
     const int64_t ntp = sort_padded_tiles(max_len);
     const dim3 grid((unsigned)(ntp / kSortGroup), (unsigned)nseg);
     const uint32_t dmask = (uint32_t)((1ull << bits) - 1ull);
-    const bool wide = bits > 8;
-    const int64_t n = (int64_t)(wide ? (1 << kMaxDigitBits) : 256) * ntp;
-    if (wide)
-        LM_LAUNCH(K_SORT_HIST, radix_hist_kernel<kMaxDigitBits>, grid, dim3(kSortThreads), 0, stream,
-                  (const uint32_t*)keys[cur], stride, seg_len, uniform_len, max_len, shift, dmask, sc->hist, sc->row,
-                  ntp);
+    // the narrowest kernel that holds the digit: 8, 9 or kMaxDigitBits bits
+    const int tbits = bits <= 8 ? 8 : bits == 9 ? 9 : kMaxDigitBits;
+    const int64_t n = (int64_t)(1 << tbits) * ntp;
+    const uint32_t* kin = (const uint32_t*)keys[cur];
+    if (tbits == 8)
+        LM_LAUNCH(K_SORT_HIST, radix_hist_kernel<8>, grid, dim3(kSortThreads), 0, stream, kin, stride, seg_len,
+                  uniform_len, max_len, shift, dmask, sc->hist, sc->row, ntp);
+    else if (tbits == 9)
+        LM_LAUNCH(K_SORT_HIST, radix_hist_kernel<9>, grid, dim3(kSortThreads), 0, stream, kin, stride, seg_len,
+                  uniform_len, max_len, shift, dmask, sc->hist, sc->row, ntp);
     else
-        LM_LAUNCH(K_SORT_HIST, radix_hist_kernel<8>, grid, dim3(kSortThreads), 0, stream, (const uint32_t*)keys[cur],
-                  stride, seg_len, uniform_len, max_len, shift, dmask, sc->hist, sc->row, ntp);
+        LM_LAUNCH(K_SORT_HIST, radix_hist_kernel<kMaxDigitBits>, grid, dim3(kSortThreads), 0, stream, kin, stride,
+                  seg_len, uniform_len, max_len, shift, dmask, sc->hist, sc->row, ntp);
     LM_LAUNCH_CHECK();
     int rc = exclusive_scan_u32_batched(sc->hist, n, sc->row, sc->offs, sc->row, nseg, sc->bsums, nullptr, stream);
     if (rc) return rc;
-    if (wide)
-        LM_LAUNCH(K_SORT_SCATTER, radix_scatter_kernel<kMaxDigitBits>, grid, dim3(kSortThreads), 0, stream,
-                  (const uint32_t*)keys[cur], vin, keys[1 - cur], vals[1 - cur], stride,
-                  seg_len, uniform_len, max_len, shift, dmask, (const uint32_t*)sc->offs, sc->row, ntp);
+    const uint32_t* offs = (const uint32_t*)sc->offs;
+    if (tbits == 8)
+        LM_LAUNCH(K_SORT_SCATTER, radix_scatter_kernel<8>, grid, dim3(kSortThreads), 0, stream, kin, vin,
+                  keys[1 - cur], vals[1 - cur], stride, seg_len, uniform_len, max_len, shift, dmask, offs, sc->row, ntp);
+    else if (tbits == 9)
+        LM_LAUNCH(K_SORT_SCATTER, radix_scatter_kernel<9>, grid, dim3(kSortThreads), 0, stream, kin, vin,
+                  keys[1 - cur], vals[1 - cur], stride, seg_len, uniform_len, max_len, shift, dmask, offs, sc->row, ntp);
     else
-        LM_LAUNCH(K_SORT_SCATTER, radix_scatter_kernel<8>, grid, dim3(kSortThreads), 0, stream,
-                  (const uint32_t*)keys[cur], vin, keys[1 - cur], vals[1 - cur], stride,
-                  seg_len, uniform_len, max_len, shift, dmask, (const uint32_t*)sc->offs, sc->row, ntp);
+        LM_LAUNCH(K_SORT_SCATTER, radix_scatter_kernel<kMaxDigitBits>, grid, dim3(kSortThreads), 0, stream, kin,
+                  vin, keys[1 - cur], vals[1 - cur], stride, seg_len, uniform_len, max_len, shift, dmask, offs,
+                  sc->row, ntp);
     LM_LAUNCH_CHECK();
     return LM_OK;
 }
