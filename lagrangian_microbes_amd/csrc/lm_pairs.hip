@@ -26,9 +26,10 @@
 // runs whose cells alias into the same buckets are dropped: pairs never cross
 // runs.
 // Distances use the float32 positions widened to double, with the same
-// operations cKDTree applies: dx*dx + dy*dy <= r*r for p = 2.  Since a
-// difference of two nearby floats is exact in double and its square is exact,
-// the test is independent of FMA contraction.
+// operations cKDTree applies: dx*dx + dy*dy <= r*r for p = 2, evaluated as
+// written (the library is built without FMA contraction).  For positions of
+// similar magnitude the double difference of two floats is exact and so is
+// its square.
 #include "lm_common.h"
 #include "lm_scan.h"
 #include "lm_sort.h"
@@ -291,9 +292,10 @@ pair_count_kernel(int64_t n, CellGrid g, Metric m, Strides st, const uint32_t* _
 #pragma unroll
         for (int u = 0; u < kCand; ++u) {
             const uint32_t j = __float_as_uint(o[u].z);
-            // |dx| <= r and |dy| <= r hold for every p >= 1; the float32
-            // differences of nearby positions are exact, and m.box is above
-            // r, so the prefilter only drops pairs the exact test drops.
+            // |dx| <= r and |dy| <= r hold for every p >= 1; a float32
+            // difference is within one rounding (relative 2^-24) of the exact
+            // one and m.box lies above r by more than that, so the prefilter
+            // only drops pairs the exact test drops.
             // One predicate, evaluated without branches; one branch for the rare store.
             const float ax = fabsf(o[u].x - me.x), ay = fabsf(o[u].y - me.y);
             const bool cand = (t0 + u < total) & (j > i) & (__float_as_uint(o[u].w) == my_run) & (ax <= m.box) &
