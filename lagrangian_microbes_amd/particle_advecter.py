@@ -9,8 +9,9 @@ HIP RK4 kernel (lm_advect_rk4_f64) that advances all particles through every
 step of the chunk and records the post-step float32 positions.
 
 Differences, all deliberate:
-- ``velocity_field`` may be a VelocityField (the only usable choice offline;
-  "OSCAR" needs the network, velocity_fields.py:21-32).
+- ``velocity_field`` may be a VelocityField; "OSCAR" reads the local
+  oscar_vel<year>.nc in the working directory (NetCDF-3 only) and, where the
+  reference would download it (velocity_fields.py:21-32), raises.
 - Particle state is float64 on the device (parcels keeps float32,
   particle_advecter.py:208-209); records stay float32 as in the reference.
 - The Kh random walk draws keyed Philox uniforms (particle, step) instead of
@@ -114,12 +115,15 @@ class ParticleAdvecter:
         self.walk_key = (int(seed), 0x6B68)  # key word 1 separates the walk from other streams
         self.device = torch.device(device or "cuda")
         self._dev_field = None
+        self._oscar = None  # (year, VelocityField) read from the local OSCAR file
 
     # -- helpers -----------------------------------------------------------
     def _field(self, start_time):
         field = self.velocity_field
-        if isinstance(field, str):
-            field = oscar_dataset(start_time.year)  # raises offline
+        if isinstance(field, str):  # "OSCAR": the local oscar_vel<year>.nc, read once per year
+            if self._oscar is None or self._oscar[0] != start_time.year:
+                self._oscar = (start_time.year, oscar_dataset(start_time.year))
+            field = self._oscar[1]
         if self._dev_field is None or self._dev_field.field is not field:
             self._dev_field = dev.DeviceField(field, self.device)
         return self._dev_field

@@ -5,7 +5,8 @@ The reference reads OSCAR surface currents over OPeNDAP
 rectilinear lon/lat grid with times in seconds since the first level
 (particle_advecter.py:163-180).  ``VelocityField`` is that contract as an
 object; ``synthetic_fourier_field`` builds seeded eddy fields for offline runs;
-``oscar_dataset`` keeps the reference name but cannot fetch anything offline.
+``oscar_dataset`` keeps the reference name and reads a local OSCAR file
+(NetCDF-3), but cannot fetch anything offline.
 """
 import os
 
@@ -134,13 +135,97 @@ def oscar_dataset_filename(year):
     return "oscar_vel" + str(year) + ".nc"
 
 
-def oscar_dataset(year):
-    """The reference downloads OSCAR over OPeNDAP here (velocity_fields.py:21-32).
+_TIME_UNIT_SECONDS = {"day": 86400.0, "hour": 3600.0, "minute": 60.0, "min": 60.0, "second": 1.0, "sec": 1.0,
+                      "s": 1.0}
 
-    This build has no network access and no HDF5 reader, so a
-    VelocityField must be passed instead of "OSCAR".
+
+def _text(value):
+    return value.decode("ascii", "replace") if isinstance(value, bytes) else str(value)
+
+
+def _seconds_since_first(var):
+    """CF time values ("<unit> since <epoch>") as seconds since the first one,
+    the particle_advecter.py:170-173 conversion of the decoded datetimes."""
+    units = _text(getattr(var, "units", "")).strip().lower()
+    unit = units.split(" since ")[0].strip() if " since " in units else ""
+    scale = _TIME_UNIT_SECONDS.get(unit[:-1] if unit.endswith("s") and unit[:-1] in _TIME_UNIT_SECONDS else unit)
+    if scale is None:
+        raise RuntimeError(f"time units {units!r} are not '<days|hours|minutes|seconds> since <epoch>'")
+    t = np.ma.filled(np.ma.asarray(var[:], dtype=np.float64), np.nan).ravel()
+    if t.size == 0 or not np.all(np.isfinite(t)):
+        raise RuntimeError("the time axis is empty or has missing values")
+    return (t - t[0]) * scale
+
+
+def _axis_of(dims, names):
+    for k, d in enumerate(dims):
+        if _text(d).lower() in names:
+            return k
+    return None
+
+
+def read_oscar_file(path):
+    """The VelocityField parcels is handed from an OSCAR file.
+
+    Follows particle_advecter.py:160-184: the first (nominal) depth only,
+    u and v as [time, latitude, longitude], times in seconds since the first
+    level.  What xarray and parcels do to the arrays on the way, restated:
+    missing and fill values become NaN (xarray's CF decoding, scale_factor and
+    add_offset applied), parcels' Field then sets NaN (land) to 0, and a
+    decreasing latitude axis (OSCAR runs 80N to 80S) is flipped with the data.
+    OSCAR's longitudes run 20 to 420 E on a uniform 1/3 degree grid and are
+    kept as they are.
+
+    Only NetCDF-3 files (classic or 64-bit offset) are readable here, through
+    scipy.io.netcdf_file: there is no HDF5 reader for NetCDF-4.
     """
-    path = oscar_dataset_filename(year)
-    raise RuntimeError(
-        f"OSCAR {year} is not available offline ({os.path.abspath(path)}); construct a "
-        "lagrangian_microbes_amd.VelocityField from local arrays and pass it as velocity_field")
+    with open(path, "rb") as fh:
+        magic = fh.read(4)
+    if magic[:3] != b"CDF" or magic[3:4] not in (b"\x01", b"\x02"):
+        kind = "a NetCDF-4 (HDF5) file" if magic == b"\x89HDF" else "not a NetCDF-3 file"
+        raise RuntimeError(f"{path} is {kind}: only NetCDF-3 (classic or 64-bit offset) is readable without "
+                           "netCDF4/h5py; convert it with `nccopy -k classic` or pass a VelocityField")
+    from scipy.io import netcdf_file
+
+    with netcdf_file(path, "r", mmap=False, maskandscale=True) as nc:
+        var = nc.variables
+        missing = [k for k in ("time", "latitude", "longitude", "u", "v") if k not in var]
+        if missing:
+            raise RuntimeError(f"{path} has no variable(s) {missing} (OSCAR layout: time, depth, latitude, "
+                               "longitude, u, v)")
+        times = _seconds_since_first(var["time"])
+        lats = np.asarray(var["latitude"][:], dtype=np.float64).ravel()
+        lons = np.asarray(var["longitude"][:], dtype=np.float64).ravel()
+        fields = []
+        for name in ("u", "v"):
+            x = var[name]
+            dims = tuple(x.dimensions)
+            data = np.ma.filled(np.ma.asarray(x[:], dtype=np.float64), np.nan)
+            kt = _axis_of(dims, ("time",))
+            ky = _axis_of(dims, ("latitude", "lat"))
+            kx = _axis_of(dims, ("longitude", "lon"))
+            kz = _axis_of(dims, ("depth", "z", "lev"))
+            if None in (kt, ky, kx) or data.ndim != 3 + (kz is not None):
+                raise RuntimeError(f"{name}{dims}: expected (time[, depth], latitude, longitude)")
+            if kz is not None:
+                data = np.take(data, 0, axis=kz)  # the nominal depth, particle_advecter.py:163-164
+                kt, ky, kx = (k - (k > kz) for k in (kt, ky, kx))
+            data = np.transpose(data, (kt, ky, kx))
+            data[np.isnan(data)] = 0.0
+            fields.append(data)
+    return VelocityField(lons, lats, fields[0], fields[1], times if times.size > 1 else None)
+
+
+def oscar_dataset(year, directory="."):
+    """OSCAR surface currents of ``year`` from the local file
+    ``oscar_vel{year}.nc`` in ``directory`` (the file velocity_fields.py:21-32
+    caches and opens; the reference's working directory by default), as a
+    VelocityField (``read_oscar_file``).  The reference's OPeNDAP download,
+    used when that file is absent, needs the network: then this raises.
+    """
+    path = os.path.join(directory, oscar_dataset_filename(year))
+    if not os.path.isfile(path):
+        raise RuntimeError(
+            f"OSCAR {year} is not available offline ({os.path.abspath(path)}): the reference downloads it over "
+            "OPeNDAP; place the file there or pass a lagrangian_microbes_amd.VelocityField as velocity_field")
+    return read_oscar_file(path)

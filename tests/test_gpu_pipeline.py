@@ -86,6 +86,37 @@ def test_dropin_two_pass_c1(gpu, c1_oracle, tmp_path):
     assert list(isim.pairs_per_step) == [len(p[0]) for p in c1_oracle["pairs"]]
 
 
+def test_oscar_file_dropin_c1(gpu, c1_oracle, tmp_path, monkeypatch):
+    # velocity_field="OSCAR" reads oscar_vel<year>.nc from the working
+    # directory: the C1 field written in OSCAR's layout (latitude decreasing,
+    # a depth axis, CF time) advects exactly as the field object does
+    from scipy.io import netcdf_file
+    w = configs.rps_10k_steady()
+    f = w.velocity_field()
+    with netcdf_file(str(tmp_path / "oscar_vel2017.nc"), "w", version=2) as nc:
+        for name, size in (("time", 1), ("depth", 1), ("latitude", f.ny), ("longitude", f.nx)):
+            nc.createDimension(name, size)
+        t = nc.createVariable("time", "i4", ("time",))
+        t[:] = [8858]
+        t.units = "day since 1992-10-05 00:00:00"
+        nc.createVariable("depth", "f4", ("depth",))[:] = [15.0]
+        nc.createVariable("latitude", "f8", ("latitude",))[:] = f.lats[::-1]
+        nc.createVariable("longitude", "f8", ("longitude",))[:] = f.lons
+        for name, data in (("u", f.u), ("v", f.v)):
+            nc.createVariable(name, "f8", ("time", "depth", "latitude", "longitude"))[:] = \
+                data[:, None, ::-1, :]
+    monkeypatch.chdir(tmp_path)
+    lon, lat = w.particles()
+    start, dt = datetime(2017, 1, 1), timedelta(hours=1)
+    pa = ParticleAdvecter(lon, lat, N_procs=1, velocity_field="OSCAR", output_dir=str(tmp_path / "adv"),
+                          output_chunk_iters=24)
+    pa.time_step(start, start + w.steps * dt, dt)
+    pa.create_netcdf_file(start, start + w.steps * dt, dt)
+    data, _, _ = read_particle_dataset(str(tmp_path / "adv" / "particle_data.nc"))
+    assert np.array_equal(data["longitude"], c1_oracle["rec_lon"].T)
+    assert np.array_equal(data["latitude"], c1_oracle["rec_lat"].T)
+
+
 def test_generic_callback_path(gpu, c1_oracle, tmp_path):
     # a non-RPS callback sees every pair once, in sorted (i, j) order
     w = configs.rps_10k_steady()
