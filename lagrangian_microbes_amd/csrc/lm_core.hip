@@ -26,7 +26,8 @@ static const char* const kKernelNames[K_NUM_KERNELS] = {
     "advect_rk4_f64", "field_blend_f64", "bin_keys", "bin_table", "pair_count", "pair_emit",
     "scan_reduce",    "scan_apply",      "rps_init",  "rps_union",   "rps_keys",   "rps_group", "rps_comps",
     "rps_play",       "rps_fill",  "rps_deps",    "rps_levels",    "philox",
-    "copy16", "sort_hist", "sort_scatter", "adj_forward", "adj_recompute", "adj_backward", "adj_blend", "adj_slab", "advect_rk4_f32", "field_blend_f32", "species_count", "dist_hist"};
+    "copy16", "sort_hist", "sort_scatter", "adj_forward", "adj_recompute", "adj_backward", "adj_blend", "adj_slab", "advect_rk4_f32", "field_blend_f32", "species_count", "dist_hist",
+    "rps_hook"};
 
 struct TimedLaunch {
     int kid;

@@ -8,9 +8,14 @@
 // Everything that depends only on the pairs runs once for a batch of steps
 // (blockIdx.y = step), with the parallelism of all steps at once:
 //   prepare 1. init      parent[v] = v; touched marks and rows cleared
+//           1b. hook     parent[j] = i for every pair (i < j), plain stores: when
+//                        several pairs share j one of them lands, any of them a
+//                        smaller member of j's component
 //           2. union     lock-free union-find, hooking the larger root under the
 //                        smaller, so every root is the smallest index of its
-//                        component; the same pass draws each pair's uniform
+//                        component; a pair whose j still points at its i is
+//                        already joined and skips it; the same pass draws each
+//                        pair's uniform
 //                        (Philox keyed by pair rank and step), keeps its three
 //                        comparisons with pRS, pPR, pSP as bits, marks both
 //                        particles as touched and records where each row (the
@@ -178,6 +183,27 @@ rps_init_kernel(int64_t n, RpsArrays A, RpsStrides st) {
     if (v < 4) A.lists[4 * s + v] = 0;
 }
 
+// Before the union-find: every pair (i, j), i < j, points j at i with a plain
+// store.  Pairs sharing a j race and one of their stores lands; each is a
+// smaller particle of j's component, so the forest stays valid (parents only
+// point down to smaller indices of the same component) and most pairs are
+// joined before the union-find runs.  Stores rather than atomics: at a few
+// hundred thousand pairs per step, a device-scope atomic per pair costs about
+// as much as the union-find work it saves.
+__global__ void __launch_bounds__(kBlock)
+rps_hook_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __restrict__ pj_all,
+                const int64_t* __restrict__ npairs, RpsArrays A, RpsStrides st) {
+    const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
+    const int64_t P = step_pairs(npairs, s, st.pair);
+    const int32_t* pi = pi_all + po;
+    const int32_t* pj = pj_all + po;
+    uint32_t* parent = A.parent + ro;
+    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
+        const uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
+        if (a < (uint64_t)n && b < (uint64_t)n && a != b) parent[max(a, b)] = min(a, b);
+    }
+}
+
 // Runs: an ensemble of independent runs sharing the index range (run r holds
 // particles [run_start[r], run_start[r+1]) and never pairs across runs).  A
 // pair of run r draws its uniform as run r would alone: counter (its rank
@@ -225,7 +251,8 @@ rps_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* _
             touched[a] = 1;
             if (b < (uint64_t)n) {
                 touched[b] = 1;
-                if (a != b) unite(parent, a, b);
+                // parent[b] == a: b hangs below a (sets only merge), nothing to join
+                if (a != b && parent[max(a, b)] != min(a, b)) unite(parent, a, b);
             }
         }
     }
@@ -1138,6 +1165,9 @@ static int rps_prepare_impl(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int3
     if (pair_stride > 0) {
         // one thread per pair slot of the capacity (threads past a step's count exit)
         const unsigned npb = lm::grid_for(pair_stride);
+        LM_LAUNCH(lm::K_RPS_HOOK, lm::rps_hook_kernel, dim3(npb, S), dim3(lm::kBlock), 0, s, n, pair_i, pair_j,
+                  npairs, A, sp);
+        LM_LAUNCH_CHECK();
         lm::RunsView R{run, params, nullptr, nruns};
         if (run != nullptr) {
             const int64_t need = (int64_t)ws->steps_max * nruns;

@@ -430,7 +430,7 @@ class KernelTimer:
 
     @staticmethod
     def _read():
-        nk = 32
+        nk = 64
         counts = np.zeros(nk, dtype=np.int64)
         ms = np.zeros(nk, dtype=np.float64)
         got = lib().lm_timing_summary(counts.ctypes.data_as(c_vp), ms.ctypes.data_as(c_vp), nk)

@@ -193,8 +193,18 @@ def test_rps_matches_sequential_oracle(gpu, seed, n, m, clustered):
     g, up = gpu_rps(sp, pi, pj, *args)
     o = oracle.rps_sequential(sp, pi, pj, *args)
     assert np.array_equal(g, o)
+    # the union-find's partition against scipy's connected components: species
+    # alone would not show two components merged (their pairs replay in order)
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    _, lab = connected_components(coo_matrix((np.ones(m), (pi, pj)), shape=(n, n)), directed=False)
+    touched = np.unique(np.concatenate([pi, pj]))
+    comp_pairs = np.bincount(lab[pi], minlength=n)
+    comp_parts = np.bincount(lab[touched], minlength=n)
     st = up.component_stats()
-    assert st.pairs == m and st.components >= 1 and st.touched <= n
+    assert st.pairs == m and st.touched == touched.size
+    assert st.components == np.count_nonzero(comp_pairs)
+    assert st.max_pairs == comp_pairs.max() and st.max_particles == comp_parts.max()
 
 
 @pytest.mark.parametrize("n", [10000, 20000])
