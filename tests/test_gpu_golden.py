@@ -66,8 +66,8 @@ def test_c1_fused_runner_fixture(gpu):
 
 @pytest.mark.parametrize("chunk,grow", [(7, False), (10, True), (24, False)])
 def test_c1_pipelined_run_fixture(gpu, chunk, grow):
-    """Chunks overlapped on two streams give the same records and end state,
-    also when the pair buffers must grow in the middle of the run."""
+    """Chunks overlapped on three streams give the same records and end
+    state, also when the pair buffers must grow in the middle of the run."""
     g = load("c1_24h")
     w = configs.rps_10k_steady()
     lon, lat = w.particles()
