@@ -17,11 +17,13 @@
 //                        already joined and skips it; the same pass draws each
 //                        pair's uniform
 //                        (Philox keyed by pair rank and step), keeps its three
-//                        comparisons with pRS, pPR, pSP as bits, marks both
-//                        particles as touched and records where each row (the
-//                        run of pairs with one i) begins and ends
-//           3. keys      each touched particle's root (an untouched particle:
-//                        its own index, a run of one that is never listed)
+//                        comparisons with pRS, pPR, pSP as bits and records
+//                        where each row (the run of pairs with one i) begins
+//                        and ends
+//           3. keys      which particles have pairs (a row of their own, or a
+//                        parent other than themselves: every j was hooked) and
+//                        each such particle's root (any other particle: its
+//                        own index, a run of one that is never listed)
 //           4. group     particles grouped by root, stably, in two levels (a
 //                        radix pass on the root's high bits, then counting in
 //                        LDS per super-bucket): each component's particles one
@@ -226,7 +228,6 @@ rps_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* _
     const int32_t* pi = pi_all + po;
     const int32_t* pj = pj_all + po;
     uint32_t* parent = A.parent + ro;
-    uint8_t* touched = A.touched + ro;
     uint8_t* bits = A.bits + po;
     for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
         const uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
@@ -248,9 +249,11 @@ rps_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* _
             uint32_t* row = reinterpret_cast<uint32_t*>(A.rows + ro + a);  // {begin, end}
             if (k == 0 || (uint32_t)pi[k - 1] != a) row[0] = (uint32_t)k;
             if (k + 1 == P || (uint32_t)pi[k + 1] != a) row[1] = (uint32_t)(k + 1);
-            touched[a] = 1;
+            // a has its row and the hook pointed b below a, which is how the
+            // keys pass tells both have pairs; only a j below its i (never
+            // in a list from the pair search) is marked here
             if (b < (uint64_t)n) {
-                touched[b] = 1;
+                if (b < a) A.touched[ro + b] = 1;
                 // parent[b] == a: b hangs below a (sets only merge), nothing to join
                 if (a != b && parent[max(a, b)] != min(a, b)) unite(parent, a, b);
             }
@@ -285,8 +288,12 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
     const int64_t s = blockIdx.y, ro = s * st.row;
     const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (v < n) {
-        // an untouched particle is keyed by itself: a run of one, never listed
-        const uint8_t t = A.touched[ro + v];
+        // a particle has pairs if it has a row (as some pair's i) or a parent
+        // below it (as some pair's j: hooked, and parents only move down);
+        // one without pairs is keyed by itself: a run of one, never listed
+        const uint32_t p = A.parent[ro + v];
+        const uint8_t t = (p != (uint32_t)v) | (A.rows[ro + v].y != 0u) | (A.touched[ro + v] != 0);
+        A.touched[ro + v] = t;  // for the replay's copy role
         A.pkey[ro + v] = t ? find_root(A.parent + ro, (uint32_t)v) : (uint32_t)v;
         // the touched flag joins the row record, so the grouping gathers both in one 8-byte load
         // (row ends stay below 2^29)
