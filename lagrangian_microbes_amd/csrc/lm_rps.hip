@@ -294,7 +294,16 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
         const uint32_t p = A.parent[ro + v];
         const uint8_t t = (p != (uint32_t)v) | (A.rows[ro + v].y != 0u) | (A.touched[ro + v] != 0);
         A.touched[ro + v] = t;  // for the replay's copy role
-        A.pkey[ro + v] = t ? find_root(A.parent + ro, (uint32_t)v) : (uint32_t)v;
+        // the root by a read-only walk (parents point to smaller indices, so
+        // it ends): the trees are shallow after the hook pass and the
+        // union-find, and path halving here would add scattered stores
+        uint32_t x = (uint32_t)v, q = p;
+        if (t)
+            while (q != x) {
+                x = q;
+                q = A.parent[ro + x];
+            }
+        A.pkey[ro + v] = x;
         // the touched flag joins the row record, so the grouping gathers both in one 8-byte load
         // (row ends stay below 2^29)
         if (t) reinterpret_cast<uint32_t*>(A.rows + ro + v)[1] |= 0x80000000u;
