@@ -185,8 +185,9 @@ rps_init_kernel(int64_t n, RpsArrays A, RpsStrides st) {
     if (v < 4) A.lists[4 * s + v] = 0;
 }
 
-// Before the union-find: every pair (i, j), i < j, points j at i with a plain
-// store.  Pairs sharing a j race and one of their stores lands; each is a
+// Before the union-find: every pair points its larger index at its smaller
+// one (j at i, as lists have i < j) with a plain store.  Pairs sharing a j
+// race and one of their stores lands; each is a
 // smaller particle of j's component, so the forest stays valid (parents only
 // point down to smaller indices of the same component) and most pairs are
 // joined before the union-find runs.  Stores rather than atomics: at a few
