@@ -284,30 +284,61 @@ rps_run_offsets_kernel(int nsteps, int nruns, const int64_t* __restrict__ run_st
     off[t] = (uint32_t)lo;
 }
 
+// Particles per thread of the keys pass: their root walks run in lockstep,
+// so a lane keeps that many dependent loads in flight instead of one.
+constexpr int kKeyItems = 4;
+
 __global__ void __launch_bounds__(kBlock)
 rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
     const int64_t s = blockIdx.y, ro = s * st.row;
-    const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (v < n) {
+    const int64_t v0 = (int64_t)blockIdx.x * kBlock * kKeyItems + threadIdx.x;
+    uint32_t x[kKeyItems], q[kKeyItems];
+    bool t[kKeyItems], act[kKeyItems];
+#pragma unroll
+    for (int k = 0; k < kKeyItems; ++k) {
+        const int64_t v = v0 + (int64_t)k * kBlock;
+        const bool live = v < n;
         // a particle has pairs if it has a row (as some pair's i) or a parent
         // below it (as some pair's j: hooked, and parents only move down);
         // one without pairs is keyed by itself: a run of one, never listed
-        const uint32_t p = A.parent[ro + v];
-        const uint8_t t = (p != (uint32_t)v) | (A.rows[ro + v].y != 0u) | (A.touched[ro + v] != 0);
-        A.touched[ro + v] = t;  // for the replay's copy role
-        // the root by a read-only walk (parents point to smaller indices, so
-        // it ends): the trees are shallow after the hook pass and the
-        // union-find, and path halving here would add scattered stores
-        uint32_t x = (uint32_t)v, q = p;
-        if (t)
-            while (q != x) {
-                x = q;
-                q = A.parent[ro + x];
+        const uint32_t p = live ? A.parent[ro + v] : 0u;
+        t[k] = live && ((p != (uint32_t)v) | (A.rows[ro + v].y != 0u) | (A.touched[ro + v] != 0));
+        x[k] = (uint32_t)v;
+        q[k] = p;
+        act[k] = t[k] && p != (uint32_t)v;
+    }
+#pragma unroll
+    for (int k = 0; k < kKeyItems; ++k) {
+        const int64_t v = v0 + (int64_t)k * kBlock;
+        if (v < n) A.touched[ro + v] = t[k];  // for the replay's copy role
+    }
+    // the roots by read-only walks (parents point to smaller indices, so they
+    // end), all of a thread's at once: the trees are shallow after the hook
+    // pass and the union-find, and path halving here would add scattered stores
+    for (;;) {
+        uint32_t nq[kKeyItems];
+#pragma unroll
+        for (int k = 0; k < kKeyItems; ++k) nq[k] = act[k] ? A.parent[ro + q[k]] : 0u;
+        bool more = false;
+#pragma unroll
+        for (int k = 0; k < kKeyItems; ++k) {
+            if (act[k]) {
+                x[k] = q[k];
+                q[k] = nq[k];
+                act[k] = q[k] != x[k];
             }
-        A.pkey[ro + v] = x;
+            more |= act[k];
+        }
+        if (!more) break;
+    }
+#pragma unroll
+    for (int k = 0; k < kKeyItems; ++k) {
+        const int64_t v = v0 + (int64_t)k * kBlock;
+        if (v >= n) continue;
+        A.pkey[ro + v] = x[k];
         // the touched flag joins the row record, so the grouping gathers both in one 8-byte load
         // (row ends stay below 2^29)
-        if (t) reinterpret_cast<uint32_t*>(A.rows + ro + v)[1] |= 0x80000000u;
+        if (t[k]) reinterpret_cast<uint32_t*>(A.rows + ro + v)[1] |= 0x80000000u;
         if (write_vals) A.pval[ro + v] = (uint32_t)v;  // else the radix pass generates the indices
     }
 }
@@ -1216,7 +1247,8 @@ static int rps_prepare_impl(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int3
         const int super_bits = kb - fine_bits;
         LM_REQUIRE(super_bits <= lm::kMaxDigitBits, LM_ERR_ARG, "at most 2^%d particles",
                    (int)lm::kGroupFineBitsHuge + lm::kMaxDigitBits);
-        LM_LAUNCH(lm::K_RPS_KEYS, lm::rps_keys_kernel, dim3(nbp, S), dim3(lm::kBlock), 0, s, n, A, sp, super_bits == 0);
+        const unsigned nbk = lm::grid_for((n + lm::kKeyItems - 1) / lm::kKeyItems);
+        LM_LAUNCH(lm::K_RPS_KEYS, lm::rps_keys_kernel, dim3(nbk, S), dim3(lm::kBlock), 0, s, n, A, sp, super_bits == 0);
         LM_LAUNCH_CHECK();
         uint32_t* pk[2] = {A.pkey, A.ptk};
         uint32_t* pv[2] = {A.pval, A.ptv};
