@@ -1,28 +1,37 @@
 """Throughput of the fused hot path: particle-steps/s (advect + pairs + RPS).
 
-    python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--workload NAME] [--no-cpu] [--no-fp32] [--no-ensemble]
+                    [--no-adjoint] [--window-only]
 
 One step = one RK4 step of every particle, the fixed-radius pair search on
 the recorded float32 positions, and the rock-paper-scissors update over the
 sorted pairs, with positions and species recorded on the device.  The default
 workload is 490,000 microbes on a 700x700 lattice in a seeded 30-day
-synthetic field (720 hourly steps); the timed region runs K steps from the
-initial state with inputs already resident in HBM.  Prints one JSON line.
+synthetic field (720 hourly steps).  The timed region runs K steps with
+inputs already resident in HBM: all 720 from the initial state by default; a
+shorter window starts at mid-run (K = 20: steps 360-379), since the first
+steps of the run have almost no pairs.  Prints one JSON line.
 
-The line's ``adjoint`` object times the differentiable advection of 1,000,000
+``roofline`` is the kernel with the most time among those that move
+algorithmic bytes (advection, pair count, pair emission, RPS replay): its
+bytes over the window (from the chunk lengths actually launched and the
+window's pair counts) over its launch time measured with events on its own
+dispatches; ``roofline.path`` is the whole step against the same per-unit
+figures.  ``--window-only`` runs just the window once (the run a counter or
+trace profile of the window is taken from) and prints its launch counts.
+
+Further objects of the line:
+``fp32_path``: the FP32 path's workload (4M particles, 2,880 half-hour steps,
+1/12-degree field) in FP32 and FP64 from identical inputs, with both rates
+and the distance between their positions.  ``ensemble``: 64 independent runs
+(14.7M particles) batched with the autotuner's launch shape against the same
+runs one at a time.  ``adjoint``: the differentiable advection of 1,000,000
 particles over 240 hourly steps through 20 float32 field levels
 (lagrangian_microbes_amd.adjoint): the forward with checkpoints every 16
 steps, and loss.backward() of the mean squared great-circle distance to
-targets from a second seeded field (recompute plus reverse sweep);
-``adjoint_forward_ratio`` is backward time over forward time.
-
-Opt-in legs (their kernels share names with the 490k run's, so they stay out
-of the default run that the kernel-trace summary profiles): ``--fp32`` adds
-the FP32 path's workload (4M particles, 2,880 half-hour steps, 1/12-degree
-field) in FP32 and FP64 from identical inputs, with both rates and the
-distance between their positions; ``--ensemble`` adds 64 independent runs
-(14.7M particles) batched with the autotuner's launch shape against the same
-runs one at a time.
+targets from a second seeded field (recompute plus reverse sweep), then the
+RPS update on the final positions; ``adjoint_forward_ratio`` is backward
+time over forward time.
 """
 import argparse
 import concurrent.futures
@@ -57,8 +66,13 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint forward/backward timing")
     ap.add_argument("--adjoint-reps", type=int, default=3)
-    ap.add_argument("--fp32", action="store_true", help="also run the FP32 path at 4M particles (2,880 steps)")
-    ap.add_argument("--ensemble", action="store_true", help="also run the 64-run ensemble")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 path at 4M particles")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-run ensemble")
+    ap.add_argument("--ensemble-tune", choices=("short", "full"), default="short",
+                    help="autotuner grid of the ensemble leg (short: 8 launch shapes, full: 27)")
+    ap.add_argument("--window-only", action="store_true",
+                    help="run only the timed window once with per-launch events and print its launch counts "
+                         "(for counter and trace profiles of the window)")
     ap.add_argument("--fp32-steps", type=int, default=2880)
     ap.add_argument("--cpu-steps", type=int, default=48)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -80,71 +94,102 @@ def copy_calibration(nbytes):
                       "read_bytes_per_launch": nbytes, "write_bytes_per_launch": nbytes}), flush=True)
 
 
-def run_steps(runner, k, total):
+def window_start(total, steps):
+    """First step of the timed window.  A window as long as the workload
+    starts at step 0; a shorter one is placed at mid-run and ends inside the
+    run: at step 0 the lattice spacing exceeds the radius, so the first steps
+    have almost no pairs and no RPS work, unlike the rest of the run."""
+    if steps >= total:
+        return 0
+    return min(total // 2, total - steps)
+
+
+def position(runner, start):
+    """Back to the initial state, then (untimed, pipelined as the timed run)
+    to step `start`."""
+    import torch
+    runner.reset()
+    if start:
+        runner.run(start)
+    torch.cuda.synchronize()
+
+
+def run_steps(runner, k, total, on_chunk=None):
     """Run k steps from the runner's state (chunks pipelined), restarting the
-    workload at its end."""
-    done = 0
+    workload at its end.  Returns the chunk lengths launched (one batched
+    launch of the advection and of each preparation kernel per chunk)."""
+    done, sizes = 0, []
     while done < k:
         if runner.step >= total:
             runner.reset()
-        done += runner.run(min(k - done, total - runner.step))
-    return done
+        cb = None if on_chunk is None else on_chunk()
+        done += runner.run(min(k - done, total - runner.step), on_chunk=cb)
+        sizes += runner.last_schedule
+    return sizes
 
 
-def algorithmic_bytes(kernel, m):
-    """Algorithmic bytes one launch of `kernel` moves, from the per-unit figures
-    of the path: advection reads and writes the float64 position (32 B per
-    particle), writes the float32 record (8 B per particle and step) and reads
-    two field levels per step (F); the pair search reads each position once
-    (16 B per particle) and writes the pair list (8 B per pair); the RPS update
-    reads the pair list (8 B per pair) and reads and writes the species (2 B
-    per particle).  Kernels that only exist to organise the work (sorts, cell
-    tables, grouping, scans) have none: their HBM traffic is overhead, reported
-    from the byte counters instead.
-
-    Per step: n particles, P pairs.  Batched kernels cover `chunk` steps per
-    launch; rps_play runs once per step.
-    """
-    n, P, chunk, F = (m[k] for k in ("n", "P", "chunk", "F"))
-    per_launch = {
-        "advect_rk4_f64": 32 * n + (8 * n + F) * chunk,
-        "pair_count": 16 * n * chunk,
-        "pair_emit": 8 * P * chunk,
-        "rps_play": 8 * P + 2 * n,
-    }
-    return per_launch.get(kernel)
+def window_bytes(kernel, n, K, L, sum_pairs, F):
+    """Algorithmic bytes all launches of `kernel` move over a window of K steps
+    launched as L chunks, from the per-unit figures of the path: advection
+    reads and writes the float64 position once per launch (32 B per particle),
+    writes the float32 record (8 B per particle and step) and reads two field
+    levels per step (F bytes); the pair search reads each position once
+    (16 B per particle and step) and writes the pair list (8 B per pair); the
+    RPS update reads the pair list (8 B per pair) and reads and writes the
+    species (2 B per particle and step).  Kernels that only organise the work
+    (sorts, cell tables, grouping, scans) have none: their HBM traffic is
+    overhead, reported from the byte counters instead."""
+    return {
+        "advect_rk4_f64": 32 * n * L + (8 * n + F) * K,
+        "pair_count": 16 * n * K,
+        "pair_emit": 8 * sum_pairs,
+        "rps_play": 8 * sum_pairs + 2 * n * K,
+    }.get(kernel)
 
 
-def step_statistics(runner, steps, total, chunk):
-    """Mean per-step pair count and component sizes over `steps` steps, from a
-    pass run like the timed one (chunks pipelined)."""
-    runner.reset()
+def step_statistics(runner, start, steps, total):
+    """Per-step pair counts and component sizes over the window, from a pass
+    run like the timed one (chunks pipelined)."""
+    import torch
+    position(runner, start)
     rows = []
-    chunks = [0]
 
-    def collect(k, rx, ry, rs, cnt):
-        counts = cnt.cpu().numpy()
-        rps = runner.engine.rps_sets[chunks[0] % runner.PIPELINE_SETS]
-        for s in range(k):
-            st = rps.component_stats(s)
-            rows.append((counts[s], st.touched, st.wide_pairs, st.wide_particles))
-        chunks[0] += 1
+    def per_call():
+        chunks = [0]
 
-    runner.run(min(steps, total), on_chunk=collect)
-    P, U, Pw, Uw = np.array(rows, dtype=np.float64).mean(axis=0)
-    return {"P": P, "U": U, "Pw": Pw, "Uw": Uw}
+        def collect(k, rx, ry, rs, cnt):
+            counts = cnt.cpu().numpy()
+            rps = runner.engine.rps_sets[chunks[0] % runner.PIPELINE_SETS]
+            for s in range(k):
+                st = rps.component_stats(s)
+                rows.append((counts[s], st.touched, st.wide_pairs, st.wide_particles))
+            chunks[0] += 1
+        return collect
+
+    run_steps(runner, steps, total, on_chunk=per_call)
+    torch.cuda.synchronize()
+    a = np.array(rows, dtype=np.float64)
+    return {"P": a[:, 0].mean(), "sum_P": int(a[:, 0].sum()), "U": a[:, 1].mean(), "Pw": a[:, 2].mean(),
+            "Uw": a[:, 3].mean()}
 
 
-def measured_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary of
-    this workload (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy), if any."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+def traffic_file(workload, start, steps):
+    return os.path.join(ROOT, "profiles", f"traffic_{workload}_steps{start}-{start + steps}.json")
+
+
+def measured_traffic(workload, start, steps):
+    """HBM bytes per launch of each kernel over this window, from the
+    committed counter summary of the same window (FETCH_SIZE + WRITE_SIZE
+    scaled by a plain copy), if one exists."""
+    path = traffic_file(workload, start, steps)
     if not os.path.exists(path):
-        return None
+        return {}
     with open(path) as fh:
         t = json.load(fh)
-    k = t.get("kernels", {}).get(kernel)
-    return None if k is None else int(k["bytes_per_launch"])
+    w = t.get("window", {})
+    if (w.get("workload"), w.get("start"), w.get("steps")) != (workload, start, steps):
+        return {}
+    return {k: int(v["bytes_per_launch"]) for k, v in t.get("kernels", {}).items()}
 
 
 def cpu_baseline(w, runner, steps):
@@ -182,6 +227,7 @@ def cpu_baseline(w, runner, steps):
     rt = np.concatenate([q[3] for q in parts], axis=1)
     t_pairs = t_rps = 0.0
     npairs = 0
+    cpu_species, cpu_pairs = [], []
     for s in range(steps):
         a = time.perf_counter()
         pi, pj = sorted_kdtree_pairs(rl[s], rt[s], w.radius, w.norm)
@@ -191,12 +237,25 @@ def cpu_baseline(w, runner, steps):
         t_pairs += b - a
         t_rps += c - b
         npairs += len(pi)
+        cpu_species.append(sp)
+        cpu_pairs.append(len(pi))
     total = t_adv + t_pairs + t_rps
-    # GPU on the same sample, for a parity check of the sample
-    runner.advance(steps)
+    # the GPU over the same sample, through the timed (pipelined) path, for a
+    # parity check of the sample: every step's pair count and species, and the
+    # last step's recorded positions
+    got = {"species": [], "pairs": []}
+
+    def keep(k, rx, ry, rs, cnt):
+        got["species"] += [rs[s].cpu().numpy() for s in range(k)]
+        got["pairs"] += cnt.cpu().tolist()
+        got["last"] = (rx[k - 1].cpu().numpy(), ry[k - 1].cpu().numpy())
+
+    runner.run(steps, on_chunk=keep)
     torch.cuda.synchronize()
-    parity = bool(np.array_equal(runner.species.cpu().numpy(), sp)
-                  and np.array_equal(runner.rec_x[steps - 1].cpu().numpy(), rl[steps - 1]))
+    parity = bool(got["pairs"] == cpu_pairs
+                  and all(np.array_equal(g, c) for g, c in zip(got["species"], cpu_species))
+                  and np.array_equal(got["last"][0], rl[steps - 1])
+                  and np.array_equal(got["last"][1], rt[steps - 1]))
     return {
         "value": w.n * steps / total,
         "unit": "particle-steps/s",
@@ -253,7 +312,7 @@ def fp32_leg(steps):
             "f32": out["f32"], "f64": out["f64"], "f32_vs_f64_position_error": err}
 
 
-def ensemble_leg():
+def ensemble_leg(tune="short"):
     """The 64-run ensemble (14.7M particles, uneven runs, 240 hourly steps,
     FP32): the launch shape picked by the autotuner, the batched run timed
     from the initial state (best of two), then every run alone through the
@@ -275,8 +334,14 @@ def ensemble_leg():
                            chunk=chunk, precision="f32", runs=runs, cell_scale=cell_scale, play_blocks=play_blocks,
                            graphs=True)
 
+    cands = None if tune == "full" else autotune.candidates((1.0, 1.5), (8, 24), ((2048, 2048), (4096, 2048)))
     log(f"[bench] ensemble {e.n} particles in {len(e.runs)} runs: autotuning")
-    best, table = autotune.tune(make, steps=24)
+    def timed(make_runner, cand, steps):
+        t = autotune.time_runner(make_runner, cand, steps)
+        log(f"[bench] ensemble autotune {cand}: {1e3 * t:.3f} ms/step")
+        return t
+
+    best, table = autotune.tune(make, steps=24, cands=cands, timer=timed)
     log(f"[bench] ensemble autotuner picked {best}")
     r = make(**best)
     r.run(FusedRunner.PIPELINE_SETS * best["chunk"])  # warm-up: every chunk buffer set's replay graph captured
@@ -307,6 +372,8 @@ def ensemble_leg():
             torch.cuda.synchronize()
             t_run = min(t_run, time.perf_counter() - t0)
         per_run.append(t_run)
+        if len(per_run) % 16 == 0:
+            log(f"[bench] ensemble: {len(per_run)} of {len(e.runs)} runs timed alone")
         del rr
     torch.cuda.empty_cache()
     seq = float(sum(per_run))
@@ -365,10 +432,22 @@ def adjoint_leg(reps):
         one(True)
     kt = {k: {"launches": c, "total_ms": round(ms, 3)} for k, (c, ms) in timer.summary().items()}
     log(f"[bench] adjoint: forward {fwd:.2f} ms, backward {bwd:.2f} ms, ratio {bwd / fwd:.2f}")
+    # the RPS update, forward only, on the final positions of the last pass
+    with torch.no_grad():
+        X, Y = adjoint.advect(x0, y0, u, v, plan)
+    sp = torch.tensor(w.species(), device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    npairs = adjoint.interact_final(X, Y, sp, w.radius, w.params, w.key, w.steps, f.bounds)
+    torch.cuda.synchronize()
+    t_rps = time.perf_counter() - t0
+    counts = np.bincount(sp.cpu().numpy().astype(np.int64), minlength=4)[1:4]
     return {"workload": w.name, "particles": w.n, "steps": w.steps, "checkpoint_every": plan.ckpt,
             "levels": f.nt, "grid": [f.ny, f.nx], "loss_km2": runs[-1][2], "forward_ms": round(fwd, 3),
             "backward_ms": round(bwd, 3), "ratio": round(bwd / fwd, 3),
-            "particle_steps_per_s_forward": w.n * w.steps / (fwd * 1e-3), "kernels": kt}
+            "particle_steps_per_s_forward": w.n * w.steps / (fwd * 1e-3), "kernels": kt,
+            "final_interaction": {"pairs": npairs, "species_counts": [int(c) for c in counts],
+                                  "ms": round(1e3 * t_rps, 3)}}
 
 
 def main():
@@ -392,59 +471,75 @@ def main():
     log(f"[bench] workload {w.name}: {w.n} particles, {w.steps} steps available; building runner")
     runner = FusedRunner(lon, lat, w.species(), w.velocity_field(), w.radius, w.params, w.key, w.dt, w.norm,
                          chunk=args.chunk, cell_scale=args.cell_scale)
+    start = window_start(w.steps, args.steps)
 
     # warm-up (compiles nothing, but touches every kernel and sizes buffers)
     run_steps(runner, args.warmup, w.steps)
     torch.cuda.synchronize()
-    runner.reset()
-    log(f"[bench] warm-up {args.warmup} steps done; timing {args.steps} steps")
+    position(runner, start)
+    log(f"[bench] warm-up {args.warmup} steps done; timing {args.steps} steps from step {start}")
+
+    if args.window_only:
+        # the timed window once, with per-launch events, and nothing after it:
+        # the run a counter or trace profile of the window is taken from
+        with dev.KernelTimer() as timer:
+            sizes = run_steps(runner, args.steps, w.steps)
+            torch.cuda.synchronize()
+        kt = timer.summary()
+        print(json.dumps({"workload": w.name, "start": start, "steps": args.steps, "chunks": sizes,
+                          "launches": {k: c for k, (c, ms) in kt.items()},
+                          "avg_us": {k: round(1e3 * ms / c, 2) for k, (c, ms) in kt.items()}}), flush=True)
+        return
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    run_steps(runner, args.steps, w.steps)
+    sizes = run_steps(runner, args.steps, w.steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     runner.check_domain()
-    log(f"[bench] timed region {elapsed:.3f}s ({1e3 * elapsed / args.steps:.3f} ms/step)")
+    log(f"[bench] timed region {elapsed:.3f}s ({1e3 * elapsed / args.steps:.3f} ms/step), chunks {sizes}")
 
-    # same steps again with per-kernel events on each launch's own dispatch
-    runner.reset()
-    torch.cuda.synchronize()
+    # the same window again with events on each launch's own dispatch
+    position(runner, start)
     with dev.KernelTimer() as timer:
         run_steps(runner, args.steps, w.steps)
         torch.cuda.synchronize()
     kt = timer.summary()
     kernels = {k: {"launches": c, "avg_us": 1e3 * ms / c, "total_ms": ms} for k, (c, ms) in kt.items()}
-    dominant = max(kernels, key=lambda k: kernels[k]["total_ms"])
-    stats = step_statistics(runner, args.steps, w.steps, args.chunk)
-    P = stats["P"]
-    # two levels of u and v per step (float64)
-    m = dict(stats, n=w.n, chunk=args.chunk, F=2 * 8 * runner.dfield.level_len)
-    ab = algorithmic_bytes(dominant, m)
-    avg_s = kernels[dominant]["avg_us"] * 1e-6
-    roofline = None
-    if ab is not None:
-        achieved = ab / avg_s / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(w.name, dominant),
-                    "kernel": dominant, "avg_us": round(kernels[dominant]["avg_us"], 2),
-                    "algorithmic_bytes": int(ab)}
+    stats = step_statistics(runner, start, args.steps, w.steps)
+    P, K, L = stats["P"], args.steps, len(sizes)
+    F = 2 * 2 * runner.dfield.level_len * 8  # two levels of interleaved (u, v) float64 per step
+    traffic = measured_traffic(w.name, start, K)
     for k, v in kernels.items():
-        b, t = algorithmic_bytes(k, m), measured_traffic(w.name, k)
-        v["gbs"] = None if b is None else round(b / (v["avg_us"] * 1e-6) / 1e9, 1)
-        v["hbm_gbs"] = None if t is None else round(t / (v["avg_us"] * 1e-6) / 1e9, 1)
+        tb = window_bytes(k, w.n, K, L, stats["sum_P"], F)
+        v["bytes_per_launch"] = None if tb is None else tb / v["launches"]
+        v["gbs"] = None if tb is None else tb / (v["total_ms"] * 1e-3) / 1e9
+        t = traffic.get(k)
+        v["hbm_gbs"] = None if t is None else t / (v["avg_us"] * 1e-6) / 1e9
+    # the roofline kernel: the one with the most time among those that carry
+    # algorithmic bytes (the organising kernels have none)
+    with_bytes = [k for k, v in kernels.items() if v["gbs"] is not None]
+    rk = max(with_bytes, key=lambda k: kernels[k]["total_ms"])
+    rv = kernels[rk]
+    roofline = {"bound": "hbm", "achieved": round(rv["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(rv["gbs"] / HBM_PEAK_GBS, 5), "traffic": traffic.get(rk), "kernel": rk,
+                "avg_us": round(rv["avg_us"], 2), "algorithmic_bytes_per_launch": int(rv["bytes_per_launch"]),
+                "launches": rv["launches"]}
     # the whole step against the same per-unit figures (records included)
-    path_bytes = 50 * w.n + 16 * P + m["F"] + 9 * w.n
+    path_bytes = (59 * w.n + F) * K + 16 * stats["sum_P"]
+    path_gbs = path_bytes / elapsed / 1e9
+    roofline["path"] = {"algorithmic_bytes_per_step": int(path_bytes / K), "achieved": round(path_gbs, 1),
+                        "frac": round(path_gbs / HBM_PEAK_GBS, 5)}
 
-    value = w.n * args.steps / elapsed
+    value = w.n * K / elapsed
     result = {
         "metric": METRIC,
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": 1e3 * elapsed / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -452,30 +547,30 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "particles": w.n, "lattice": "700x700", "dt_s": w.dt,
                    "radius_deg": w.radius, "p": w.p, "field": w.describe()["field"],
+                   "window": {"first_step": start, "steps": K, "chunks": sizes},
                    "mean_pairs_per_step": round(P, 1), "mean_particles_with_pairs": round(stats["U"], 1),
                    "chunk": args.chunk, "cell_scale": args.cell_scale},
         "roofline": roofline,
-        "kernels": {k: {"launches": v["launches"], "avg_us": round(v["avg_us"], 2), "algorithmic_gbs": v["gbs"],
-                        "hbm_gbs": v["hbm_gbs"]} for k, v in kernels.items()},
-        "path": {"algorithmic_bytes_per_step": int(path_bytes),
-                 "achieved_gbs": round(path_bytes / (elapsed / args.steps) / 1e9, 1)},
+        "kernels": {k: {"launches": v["launches"], "avg_us": round(v["avg_us"], 2),
+                        "algorithmic_gbs": None if v["gbs"] is None else round(v["gbs"], 1),
+                        "hbm_gbs": None if v["hbm_gbs"] is None else round(v["hbm_gbs"], 1)}
+                    for k, v in kernels.items()},
         "adjoint_forward_ratio": None,
     }
+    log(f"[bench] roofline kernel {rk}: {rv['gbs']:.1f} GB/s algorithmic; path {path_gbs:.1f} GB/s")
 
     if not args.no_cpu:
-        runner.reset()
-        run_steps(runner, w.steps // 2, w.steps)
-        torch.cuda.synchronize()
+        position(runner, w.steps // 2)
         log(f"[bench] CPU baseline: {args.cpu_steps} steps from step {runner.step}")
         result["cpu_baseline"] = cpu_baseline(w, runner, args.cpu_steps)
     else:
         result["cpu_baseline"] = None
     del runner
     torch.cuda.empty_cache()
-    if args.fp32:
+    if not args.no_fp32:
         result["fp32_path"] = fp32_leg(args.fp32_steps)
-    if args.ensemble:
-        result["ensemble"] = ensemble_leg()
+    if not args.no_ensemble:
+        result["ensemble"] = ensemble_leg(args.ensemble_tune)
     if not args.no_adjoint:
         result["adjoint"] = adjoint_leg(args.adjoint_reps)
         result["adjoint_forward_ratio"] = result["adjoint"]["ratio"]

@@ -166,3 +166,24 @@ def great_circle_sq(lon, lat, lon_t, lat_t, radius=SPHERE_RADIUS_M):
 def trajectory_loss(lon, lat, lon_t, lat_t):
     """Mean squared great-circle distance to the targets, in km^2."""
     return great_circle_sq(lon, lat, lon_t, lat_t, radius=SPHERE_RADIUS_M / 1e3).mean()
+
+
+def interact_final(lon, lat, species, radius, params, key, step, bbox, norm=2.0):
+    """The rock-paper-scissors update on the positions after an advection,
+    forward only (the discrete update has no gradient): the pair search on
+    the float32-rounded positions, as the interaction pass reads the stored
+    float32 record (interaction_simulator.py:88-99), then the pairs in
+    sorted order through the rule of interactions.py:13-40 with Philox
+    uniforms keyed by (key, step).  lon/lat: float64 GPU tensors (detached
+    from any graph); species: int8 GPU tensor, updated in place.  Returns the
+    pair count."""
+    from .engine import InteractionEngine
+    n = lon.numel()
+    require(lat, torch.float64, numel=n, name="lat")
+    require(species, torch.int8, numel=n, name="species")
+    with torch.no_grad():
+        rx = lon.detach().to(torch.float32).reshape(1, n).contiguous()
+        ry = lat.detach().to(torch.float32).reshape(1, n).contiguous()
+        eng = InteractionEngine(n, bbox, radius, norm, lon.device, steps_max=1)
+        cnt = eng.run_chunk(rx, ry, species, params, key, step)
+        return int(cnt[0].item())

@@ -21,7 +21,8 @@ import torch
 from . import device as dev
 from .engine import InteractionEngine, new_key, walk_amplitude
 from .interactions import rock_paper_scissors_interaction
-from .netcdf_io import write_particle_dataset
+from .netcdf_io import ParticleDataset, copy_variables
+from .records import DEFAULT_RING_BYTES, RecordStream
 from .utils import pretty_time
 from .velocity_fields import VelocityField
 
@@ -60,14 +61,15 @@ class FusedRunner:
     run's own p and key apply and params/key are unused); cell_scale: the
     pair search's cell side over the radius; play_blocks: (wide, narrow)
     workgroups of the replay; graphs: replay each chunk's per-step launches
-    from a captured HIP graph.
+    from a captured HIP graph; pair_capacity: initial pair buffer length per
+    step (default 2 n; buffers grow when a step finds more pairs).
     """
 
     PIPELINE_SETS = 3  # chunk buffer sets of run(): replay, preparation, advection
 
     def __init__(self, lon0, lat0, species0, field, radius, params, key, dt_s, norm=2.0, chunk=100, Kh=0.0,
                  walk_key=(0, 0x6B68), device=None, bbox=None, precision="f64", runs=None, cell_scale=1.0,
-                 play_blocks=None, graphs=False):
+                 play_blocks=None, graphs=False, pair_capacity=None):
         self.device = torch.device(device or "cuda")
         self.n = int(np.asarray(lon0).size)
         self.lon0 = np.asarray(lon0, dtype=np.float64)
@@ -82,7 +84,7 @@ class FusedRunner:
         self.dfield = dev.DeviceField(field, self.device, precision)
         self.engine = InteractionEngine(self.n, bbox or field.bounds, radius, norm, self.device,
                                         steps_max=self.chunk, nsets=self.PIPELINE_SETS, runs=runs,
-                                        cell_scale=cell_scale,
+                                        cell_scale=cell_scale, initial_capacity=pair_capacity,
                                         play_blocks=play_blocks, graphs=graphs)
         self.rec_x = torch.empty((self.chunk, self.n), dtype=torch.float32, device=self.device)
         self.rec_y = torch.empty((self.chunk, self.n), dtype=torch.float32, device=self.device)
@@ -128,9 +130,11 @@ class FusedRunner:
         caller's stream.  The advection of a chunk waits for the replay that
         last used its set; the grouping waits for its advection.  After each
         chunk, ``on_chunk(k, rec_x, rec_y, rec_s, npairs)`` receives that
-        chunk's records (device views, valid until the callback returns).
-        The state and step counters end as after ``advance`` calls over the
-        same steps; returns the number of steps done.
+        chunk's records (device views, valid until the callback returns; if
+        it returns a torch.cuda.Event, e.g. after copies on another stream
+        that read them, until that event).  The state and step counters end
+        as after ``advance`` calls over the same steps; returns the number of
+        steps done.
         """
         total = max(0, int(nsteps))
         if total == 0:
@@ -143,11 +147,13 @@ class FusedRunner:
         sizes = chunk_schedule(total, self.chunk)
         if max(sizes) > self.chunk:  # the chunk buffers hold self.chunk steps
             raise RuntimeError(f"chunk schedule exceeds the buffers: {sizes}")
+        self.last_schedule = list(sizes)  # chunk lengths of this call, one batched launch each
         step0 = [self.step + sum(sizes[:c]) for c in range(len(sizes))]
         clock0 = [self.clock_steps + sum(sizes[:c]) for c in range(len(sizes))]
         advected = [None] * len(sizes)   # event after chunk c's advection
         prepared = [None] * len(sizes)   # event after chunk c's preparation
         played = [None] * nsets          # event after the replay that last used set b
+        released = [None] * nsets        # event the callback returned for set b's last chunk
         side.wait_stream(main)           # the state as the caller left it
         adv.wait_stream(main)
 
@@ -156,6 +162,8 @@ class FusedRunner:
             with torch.cuda.stream(adv):
                 if played[b] is not None:
                     adv.wait_event(played[b])
+                if released[b] is not None:
+                    adv.wait_event(released[b])
                 self.dfield.advect(self.lon, self.lat, clock0[c] * self.dt_s, self.dt_s, k, self._rec_sets[b][0],
                                    self._rec_sets[b][1], self.n, self.walk_amp, self.walk_key, step0[c], self.oob)
                 ev = torch.cuda.Event()
@@ -194,11 +202,13 @@ class FusedRunner:
                 main.wait_event(prepared[c])
             rs = self._rec_sets[b][2][:k]
             eng.play_set(b, k, self.species, rs)
+            ret = None
             if on_chunk is not None:
-                on_chunk(k, self._rec_sets[b][0][:k], self._rec_sets[b][1][:k], rs, eng.npairs_sets[b][:k])
+                ret = on_chunk(k, self._rec_sets[b][0][:k], self._rec_sets[b][1][:k], rs, eng.npairs_sets[b][:k])
             ev = torch.cuda.Event()  # after the callback's own work on this stream
             ev.record(main)
             played[b] = ev
+            released[b] = ret if isinstance(ret, torch.cuda.Event) else None
             self.step += k
             self.clock_steps += k
         main.wait_stream(side)
@@ -226,13 +236,17 @@ class MicrobeSimulator:
     Arguments follow ParticleAdvecter and InteractionSimulator; the
     interaction must be the rock-paper-scissors plugin tuple.  ``time_step``
     writes particle_data.nc and microbe_data.nc with the layout of the two
-    reference passes (set write_outputs=False to keep the records in memory
-    only) and leaves the final species in microbe_properties["species"].
+    reference passes (write_outputs=False skips the files and the host
+    copies of the records) and leaves the final species in
+    microbe_properties["species"].  Records leave the device chunk by chunk
+    through a ``RecordStream`` (pinned ring of ``ring_bytes``, copy stream,
+    writer thread) while the run goes on; ``timings`` holds the device
+    run's wall time and the output stages'.
     """
 
     def __init__(self, particle_lons, particle_lats, velocity_field, pair_interaction, interaction_radius,
                  interaction_norm=2, output_dir=".", output_chunk_iters=100, Kh=0, seed=None, philox_key=None,
-                 device=None):
+                 device=None, ring_bytes=DEFAULT_RING_BYTES):
         assert isinstance(velocity_field, VelocityField), "velocity_field must be a VelocityField"
         assert particle_lons.size == particle_lats.size
         fn, params, props = pair_interaction
@@ -254,7 +268,9 @@ class MicrobeSimulator:
         self.philox_key = tuple(philox_key) if philox_key is not None else new_key()
         self.walk_key = (new_key(seed)[0], 0x6B68)
         self.device = torch.device(device or "cuda")
+        self.ring_bytes = int(ring_bytes)
         self.iteration = 0
+        self.timings = {}
 
     def time_step(self, start_time, end_time, dt, write_outputs=True):
         dt_s = dt.total_seconds()
@@ -264,34 +280,64 @@ class MicrobeSimulator:
                              self.chunk, self.Kh, self.walk_key, self.device)
         runner.step = self.iteration  # uniforms keyed by the global step; the clock restarts at 0
         n = runner.n
-        lon = np.zeros((n, iters), dtype=np.float32)
-        lat = np.zeros((n, iters), dtype=np.float32)
-        spc = np.zeros((n, iters), dtype=np.int8)
+        times = [start_time + k * dt for k in range(iters)]
         self.pairs_per_step = np.zeros(iters, dtype=np.int64)
+        microbe_nc = os.path.join(self.output_dir, "microbe_data.nc")
+        particle_nc = os.path.join(self.output_dir, "particle_data.nc")
         tic = time()
-        done = 0
+        ds = None
+        fields = {"npairs": (torch.int64, ())}
+        if write_outputs:
+            ds = ParticleDataset(microbe_nc, n, times, {"longitude": np.float32, "latitude": np.float32,
+                                                        "species": np.int8})
+            fields.update({"longitude": (torch.float32, (n,)), "latitude": (torch.float32, (n,)),
+                           "species": (torch.int8, (n,))})
+
+        def sink(t0, k, arrays):
+            self.pairs_per_step[t0:t0 + k] = arrays["npairs"]
+            if ds is not None:
+                for name in ("longitude", "latitude", "species"):
+                    ds.write_steps(name, t0, arrays[name])
+
+        stream = RecordStream(fields, self.chunk, sink, self.device, steps=iters, ring_bytes=self.ring_bytes)
+        t_setup = time() - tic
+        done = [0]
 
         def keep(k, rx, ry, rs, cnt):
-            nonlocal done
-            lon[:, done:done + k] = rx.cpu().numpy().T
-            lat[:, done:done + k] = ry.cpu().numpy().T
-            spc[:, done:done + k] = rs.cpu().numpy().T
-            self.pairs_per_step[done:done + k] = cnt.cpu().numpy()
-            done += k
+            recs = {"npairs": cnt}
+            if ds is not None:
+                recs.update({"longitude": rx, "latitude": ry, "species": rs})
+            ev = stream.submit(done[0], k, recs)
+            done[0] += k
+            return ev
 
-        runner.run(iters, on_chunk=keep)
+        tic = time()
+        try:
+            runner.run(iters, on_chunk=keep)
+            torch.cuda.synchronize(self.device)
+            t_run = time() - tic
+        finally:
+            try:
+                stream.finish()  # the writer drains what was queued, also after an error
+            finally:
+                if ds is not None:
+                    ds.close()
+        t_drain = time() - tic - t_run
         runner.check_domain()
-        logger.info("Fused advection + interactions, {:d} steps: {:s}".format(iters, pretty_time(time() - tic)))
+        tic = time()
+        if write_outputs:
+            # particle_data.nc holds the same longitude and latitude columns
+            ParticleDataset(particle_nc, n, times, {"longitude": np.float32, "latitude": np.float32}).close()
+            copy_variables(microbe_nc, particle_nc, ["longitude", "latitude"])
+        t_copy = time() - tic
+        self.timings = {"setup_s": t_setup, "run_s": t_run, "writer_drain_s": t_drain, "particle_file_s": t_copy,
+                        "writer_s": stream.write_s, "submit_wait_s": stream.wait_s,
+                        "ring_bytes": stream.ring_bytes}
+        logger.info("Fused advection + interactions, {:d} steps: {:s} (outputs: {:s} more)".format(
+            iters, pretty_time(t_run), pretty_time(t_drain + t_copy)))
         self.microbe_properties["species"][:] = runner.species.cpu().numpy()
         self.final_lons = runner.lon.cpu().numpy()
         self.final_lats = runner.lat.cpu().numpy()
         # a later call continues from the float64 end state
         self.particle_lons, self.particle_lats = self.final_lons, self.final_lats
         self.iteration += iters
-        self.records = {"longitude": lon, "latitude": lat, "species": spc}
-        if write_outputs:
-            times = [start_time + k * dt for k in range(iters)]
-            write_particle_dataset(os.path.join(self.output_dir, "particle_data.nc"),
-                                   {"longitude": lon, "latitude": lat}, times, n)
-            write_particle_dataset(os.path.join(self.output_dir, "microbe_data.nc"),
-                                   {"longitude": lon, "latitude": lat, "species": spc}, times, n)

@@ -45,10 +45,21 @@ def pretty_time(t):
     return "{:.3g} mins".format(t / 60)
 
 
+# IEC binary prefixes, index k standing for 1024**k
+_BINARY_PREFIXES = ("", "Ki", "Mi", "Gi", "Ti", "Pi", "Ei", "Zi", "Yi")
+
+
 def pretty_filesize(num, suffix="B"):
-    """Bytes as a short binary-prefixed string (utils.py:70-84)."""
-    for unit in ("", "Ki", "Mi", "Gi", "Ti", "Pi", "Ei", "Zi"):
-        if abs(num) < 1024.0:
-            return "%3.1f %s%s" % (num, unit, suffix)
-        num /= 1024.0
-    return "%.1f %s%s" % (num, "Yi", suffix)
+    """Bytes as a short binary-prefixed string, one decimal: the prefix is
+    the largest power of 1024 not above |num| (capped at Yi), the value num
+    over that power.  The reference logs pickle sizes in this format
+    (utils.py:70-84, particle_advecter.py:249)."""
+    mag = abs(float(num))
+    top = len(_BINARY_PREFIXES) - 1
+    k = min(top, int(math.log(mag, 1024))) if mag >= 1024 else 0
+    # log() may round to either side of an exact power
+    if k < top and mag >= 1024.0 ** (k + 1):
+        k += 1
+    elif k > 0 and mag < 1024.0 ** k:
+        k -= 1
+    return "{:.1f} {}{}".format(num / 1024.0 ** k, _BINARY_PREFIXES[k], suffix)

@@ -1,9 +1,9 @@
 """Short text summaries of rocprofv3 CSV output for liblm kernels.
 
-    python -m lagrangian_microbes_amd.profile_summary trace DIR [--title T] [--command C]
+    python profile_summary.py trace DIR [--title T] [--command C] [--window WINDOW_JSON]
         [--passes KERNEL=N1,N2,...[:NAME1,NAME2,...]]
-    python -m lagrangian_microbes_amd.profile_summary traffic FETCH_DIR WRITE_DIR \
-        COPY_FETCH_DIR COPY_WRITE_DIR COPY_BYTES [--json OUT] [--command C]
+    python profile_summary.py traffic FETCH_DIR WRITE_DIR \
+        COPY_FETCH_DIR COPY_WRITE_DIR COPY_BYTES [--json OUT] [--command C] [--window WINDOW_JSON]
 
 ``trace`` reads the ``*_kernel_stats.csv`` of a ``rocprofv3 --kernel-trace
 --stats --output-format csv`` run and prints a markdown table of the liblm
@@ -12,7 +12,7 @@ kernel's dispatches, in launch order (``*_kernel_trace.csv``), into
 consecutive groups of N1, N2, ... launches and prints each group's average:
 the passes of a benchmark run over one workload.
 
-    python -m lagrangian_microbes_amd.profile_summary legs BENCH_JSON OUT_DIR [--command C]
+    python profile_summary.py legs BENCH_JSON OUT_DIR [--command C]
 
 ``legs`` renders the ``fp32_path`` and ``ensemble`` objects of a ``bench.py
 --fp32 --ensemble`` JSON line as ``bench_<workload>.md`` files in OUT_DIR.
@@ -23,6 +23,13 @@ COPY_BYTES bytes each way (``bench.py --copy-calibration COPY_BYTES``).  Each
 counter (KiB per dispatch) is scaled by COPY_BYTES over what it reads for the
 copy, which turns it into HBM bytes per dispatch.  ``--json`` writes the
 per-kernel bytes per launch that bench.py reports as ``roofline.traffic``.
+
+``--window WINDOW_JSON`` (both ``trace`` and ``traffic``) keeps only the
+timed window's dispatches: the JSON line of ``bench.py --window-only`` names
+the number of launches of each kernel in the window, and since nothing runs
+after the window in that mode, they are the last that many dispatches of the
+kernel.  Averages are then over the window, comparable with the live
+per-launch timing of the same window in ``bench.py``'s JSON line.
 """
 import argparse
 import collections
@@ -66,14 +73,47 @@ def one_file(d, pattern):
     return max(files, key=os.path.getmtime)
 
 
+def window_launches(path):
+    """{rocprof short kernel name: launches in the timed window} from the JSON
+    line of ``bench.py --window-only``."""
+    if not path:
+        return None
+    with open(path) as fh:
+        line = [ln for ln in fh if ln.lstrip().startswith("{")][-1]
+    w = json.loads(line)
+    by_timer = w["launches"]
+    out = {}
+    for k, tn in TIMER_NAMES.items():
+        if tn in by_timer:
+            out[k] = by_timer[tn]
+    return out
+
+
 def trace(args):
     acc = {}
-    for r in csv.DictReader(open(one_file(args.dir, "*_kernel_stats.csv"))):
-        k = short_name(r["Name"])
-        if k is None:
-            continue
-        tot, calls = acc.get(k, (0.0, 0))
-        acc[k] = (tot + float(r["TotalDurationNs"]), calls + int(r["Calls"]))
+    win = window_launches(args.window)
+    if win is None:
+        for r in csv.DictReader(open(one_file(args.dir, "*_kernel_stats.csv"))):
+            k = short_name(r["Name"])
+            if k is None:
+                continue
+            tot, calls = acc.get(k, (0.0, 0))
+            acc[k] = (tot + float(r["TotalDurationNs"]), calls + int(r["Calls"]))
+    else:
+        per = collections.defaultdict(list)
+        for r in csv.DictReader(open(one_file(args.dir, "*_kernel_trace.csv"))):
+            k = short_name(r["Kernel_Name"])
+            if k is not None:
+                per[k].append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]) - int(r["Start_Timestamp"])))
+        for k, rows in per.items():
+            # kernels sharing a timer name (copy16, adj_slab) split the window's launches
+            # between them; only kernels the window names are kept
+            n = win.get(k)
+            if not n:
+                continue
+            rows.sort()
+            last = rows[-n:]
+            acc[k] = (float(sum(d for _, d in last)), len(last))
     rows = [(tot, k, calls, tot / calls) for k, (tot, calls) in acc.items()]
     total = sum(r[0] for r in rows)
     out = [f"# {args.title}", ""]
@@ -103,20 +143,34 @@ def trace(args):
     print("\n".join(out))
 
 
-def counters(d, name):
-    acc = collections.defaultdict(list)
+def counters(d, name, win=None):
+    """Mean of counter `name` per dispatch, per kernel (over the last win[k]
+    dispatches of kernel k when a window is given)."""
+    acc = collections.defaultdict(dict)
     for r in csv.DictReader(open(one_file(d, "*_counter_collection.csv"))):
         if r["Counter_Name"] != name:
             continue
         k = short_name(r["Kernel_Name"])
         if k is not None:
-            acc[k].append(float(r["Counter_Value"]))
-    return {k: sum(v) / len(v) for k, v in acc.items()}, {k: len(v) for k, v in acc.items()}
+            did = int(r["Dispatch_Id"])
+            acc[k][did] = acc[k].get(did, 0.0) + float(r["Counter_Value"])
+    out, calls = {}, {}
+    for k, per in acc.items():
+        vals = [per[i] for i in sorted(per)]
+        if win is not None and k != "copy16_kernel":
+            n = win.get(k)
+            if not n:
+                continue
+            vals = vals[-n:]
+        out[k] = sum(vals) / len(vals)
+        calls[k] = len(vals)
+    return out, calls
 
 
 def traffic(args):
-    fetch, calls = counters(args.fetch_dir, "FETCH_SIZE")
-    write, _ = counters(args.write_dir, "WRITE_SIZE")
+    win = window_launches(args.window)
+    fetch, calls = counters(args.fetch_dir, "FETCH_SIZE", win)
+    write, _ = counters(args.write_dir, "WRITE_SIZE", win)
     cf, _ = counters(args.copy_fetch_dir, "FETCH_SIZE")
     cw, _ = counters(args.copy_write_dir, "WRITE_SIZE")
     nb = args.copy_bytes
@@ -143,7 +197,12 @@ def traffic(args):
     print("\n".join(out))
     if args.json:
         with open(args.json, "w") as fh:
-            json.dump({"copy_bytes": nb, "fetch_scale": sf, "write_scale": sw, "kernels": res}, fh, indent=1)
+            doc = {"copy_bytes": nb, "fetch_scale": sf, "write_scale": sw, "kernels": res}
+            if args.window:
+                with open(args.window) as wf:
+                    w = json.loads([ln for ln in wf if ln.lstrip().startswith("{")][-1])
+                doc["window"] = {"workload": w["workload"], "start": w["start"], "steps": w["steps"]}
+            json.dump(doc, fh, indent=1)
 
 
 def _fmt_rate(x):
@@ -205,12 +264,14 @@ def main():
     t.add_argument("--title", default="Kernel trace")
     t.add_argument("--command", default="")
     t.add_argument("--passes", default="")
+    t.add_argument("--window", default="")
     c = sub.add_parser("traffic")
     for a in ("fetch_dir", "write_dir", "copy_fetch_dir", "copy_write_dir"):
         c.add_argument(a)
     c.add_argument("copy_bytes", type=int)
     c.add_argument("--json")
     c.add_argument("--command", default="")
+    c.add_argument("--window", default="")
     g = sub.add_parser("legs")
     g.add_argument("bench_json")
     g.add_argument("out_dir")

@@ -14,7 +14,9 @@ lm_adjoint_backward behind lagrangian_microbes_amd.adjoint.Rk4Adjoint.
 - closed forms of a uniform zonal flow; checkpoint intervals 1..64 give the
   same gradients; the global-memory fallback for blocks whose LDS window
   does not fit; steps leaving the grid; a steady field; ragged and empty
-  sizes; and finite differences at the full 1M-particle, 240-step size.
+  sizes; finite differences on 32 nodes at the full 1M-particle, 240-step
+  size (eps = 1e-3 and 1e-4 m/s); and the forward-only RPS update on that
+  run's final positions against cKDTree pairs and the sequential oracle.
 """
 import numpy as np
 import pytest
@@ -294,9 +296,10 @@ def test_only_longitude_used(gpu, case10k):
     assert normwise(v.grad.cpu().numpy(), vr.grad.numpy()) <= 1e-5
 
 
-def test_full_size_finite_differences(gpu):
-    # 1,000,000 particles, 240 steps, 15 checkpoint segments: the gradient on 8 random sampled nodes
-    # against central differences of the same forward
+@pytest.fixture(scope="module")
+def full_size():
+    """The 1,000,000-particle, 240-step workload: its gradients against targets advected through the
+    second seeded field."""
     w = configs.adjoint_1m_10day()
     f = w.velocity_field()
     lon, lat = w.particles()
@@ -305,13 +308,49 @@ def test_full_size_finite_differences(gpu):
     tu, tv = f32_tensors(tf)
     with torch.no_grad():
         TX, TY = adjoint.advect(torch.tensor(lon, device="cuda"), torch.tensor(lat, device="cuda"), tu, tv, tplan)
-    g = gpu_grads(f, lon, lat, TX.cpu().numpy(), TY.cpu().numpy(), w.dt, w.steps)
+    tlon, tlat = TX.cpu().numpy(), TY.cpu().numpy()
+    g = gpu_grads(f, lon, lat, tlon, tlat, w.dt, w.steps)
     assert g["oob"] == 0
     rng = np.random.default_rng(5)
     nodes = []
     for comp, arr in ((0, g["gu"]), (1, g["gv"])):
         touched = np.argwhere(np.abs(arr) > 1e-2 * np.abs(arr).max())
-        nodes += [(comp, tuple(int(c) for c in touched[k])) for k in rng.choice(len(touched), 4, replace=False)]
-    # over 240 steps far more perturbed trajectories straddle a cell crossing than at 24 steps, and a
-    # node's float32 gradient sums ~1e5 samples: measured medians 1e-4 .. 2e-4
-    check_fd(f, lon, lat, TX.cpu().numpy(), TY.cpu().numpy(), w.dt, w.steps, g, nodes, median_tol=5e-4)
+        nodes += [(comp, tuple(int(c) for c in touched[k])) for k in rng.choice(len(touched), 16, replace=False)]
+    return w, f, lon, lat, tlon, tlat, g, nodes
+
+
+def test_full_size_finite_differences(gpu, full_size):
+    # 1,000,000 particles, 240 steps, 15 checkpoint segments: the gradient on 32 random sampled nodes
+    # (16 of u, 16 of v) against central differences of the same forward with eps = 1e-3 m/s.  Over
+    # 240 steps far more perturbed trajectories straddle a cell crossing than at 24 steps, and a node's
+    # float32 gradient sums ~1e5 samples: measured medians 1e-4 .. 2e-4
+    w, f, lon, lat, tlon, tlat, g, nodes = full_size
+    check_fd(f, lon, lat, tlon, tlat, w.dt, w.steps, g, nodes, median_tol=5e-4)
+
+
+def test_full_size_finite_differences_small_eps(gpu, full_size):
+    # the same 32 nodes with eps = 1e-4 m/s: ten times fewer perturbed trajectories straddle a cell
+    # crossing, so the differences come closer to the gradient; a wrong term in the reverse step
+    # (a dropped stage, a sign) would be off by O(1) at either eps
+    w, f, lon, lat, tlon, tlat, g, nodes = full_size
+    check_fd(f, lon, lat, tlon, tlat, w.dt, w.steps, g, nodes, eps=1e-4, median_tol=1e-4)
+
+
+def test_full_size_final_interaction(gpu, full_size):
+    # the RPS update runs forward only on the final state: pairs on the float32-rounded final positions
+    # against scipy's cKDTree, species against the sequential oracle with the same keyed uniforms
+    import oracle
+    from oracle.pairs import sorted_kdtree_pairs
+    w, f, lon, lat, _, _, g, _ = full_size
+    sp0 = w.species()
+    sp = torch.tensor(sp0, device="cuda")
+    X = torch.tensor(g["X"], device="cuda")
+    Y = torch.tensor(g["Y"], device="cuda")
+    npairs = adjoint.interact_final(X, Y, sp, w.radius, w.params, w.key, w.steps, f.bounds)
+    rx, ry = g["X"].astype(np.float32), g["Y"].astype(np.float32)
+    pi, pj = sorted_kdtree_pairs(rx, ry, w.radius, 2.0)
+    assert npairs == len(pi) > 1000
+    want = oracle.rps_sequential(sp0, pi, pj, w.p, w.p, w.p, w.key, w.steps)
+    got = sp.cpu().numpy()
+    assert not np.array_equal(want, sp0)
+    assert np.array_equal(got, want)

@@ -139,17 +139,30 @@ def test_generic_callback_path(gpu, c1_oracle, tmp_path):
     assert seen == expect
 
 
-def test_microbe_simulator_outputs(gpu, c1_oracle, tmp_path):
+@pytest.mark.parametrize("ring_bytes", [None, 1])
+def test_microbe_simulator_outputs(gpu, c1_oracle, tmp_path, ring_bytes):
+    # records stream to the files chunk by chunk; ring_bytes=1 leaves two pinned slots, so the run
+    # waits for the writer thread before most chunks
     w = configs.rps_10k_steady()
     lon, lat = w.particles()
     rps = rock_paper_scissors(w.n, w.p, w.p, w.p, seed=w.species_seed)
+    kw = {} if ring_bytes is None else {"ring_bytes": ring_bytes}
     sim = MicrobeSimulator(lon, lat, w.velocity_field(), rps, w.radius, output_dir=str(tmp_path),
-                           output_chunk_iters=7, philox_key=w.key)
+                           output_chunk_iters=7, philox_key=w.key, **kw)
     start, dt = datetime(2017, 1, 1), timedelta(hours=1)
     sim.time_step(start, start + 24 * dt, dt)
+    if ring_bytes is not None:
+        assert sim.timings["ring_bytes"] == 2 * 7 * (w.n * 9 + 8)
     out, times, N = read_particle_dataset(os.path.join(str(tmp_path), "microbe_data.nc"))
+    assert N == w.n and times == [start + k * dt for k in range(24)]
     assert np.array_equal(out["species"], c1_oracle["species"].T)
     assert np.array_equal(out["latitude"], c1_oracle["rec_lat"].T)
+    assert np.array_equal(out["longitude"], c1_oracle["rec_lon"].T)
+    part, _, _ = read_particle_dataset(os.path.join(str(tmp_path), "particle_data.nc"))
+    assert set(part) == {"longitude", "latitude"}
+    assert np.array_equal(part["longitude"], out["longitude"]) and np.array_equal(part["latitude"], out["latitude"])
+    assert list(sim.pairs_per_step) == [len(p[0]) for p in c1_oracle["pairs"]]
+    assert np.array_equal(rps[2]["species"], c1_oracle["species"][-1])
 
 
 def test_c2_full_size_properties(gpu):
