@@ -152,10 +152,12 @@ bin_table_kernel(int64_t n, const float2* __restrict__ xy, const uint16_t* __res
                  const uint32_t* __restrict__ keys_all,
                  const uint32_t* __restrict__ vals_all, const uint32_t* __restrict__ dstart, int64_t drow,
                  int64_t dstride, uint32_t fine, uint32_t* __restrict__ start_all, float4* __restrict__ recs_all) {
+    uint32_t bx, by;
+    xcd_block(&bx, &by);  // an XCD works through a contiguous range of steps: one step's arrays in its L2
     __shared__ uint32_t cnt[FINE_MAX];
     __shared__ uint32_t scan_lds[kBlock / 64];
-    const uint32_t d = blockIdx.x, nsuper = gridDim.x, t = threadIdx.x;
-    const int64_t s = blockIdx.y;
+    const uint32_t d = bx, nsuper = gridDim.x, t = threadIdx.x;
+    const int64_t s = by;
     const uint32_t* keys = keys_all + s * st.rec;
     const uint32_t* vals = vals_all + s * st.rec;
     uint32_t* start = start_all + s * st.tab;
@@ -340,9 +342,11 @@ pair_emit_kernel(int64_t n, const float* __restrict__ x_all, const float* __rest
                  const uint32_t* __restrict__ nbr_cnt_all, const uint32_t* __restrict__ nbr_all,
                  const uint32_t* __restrict__ row_off_all, int32_t* __restrict__ pair_i_all,
                  int32_t* __restrict__ pair_j_all) {
-    const int64_t i64 = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t bx, by;
+    xcd_block(&bx, &by);  // an XCD works through a contiguous range of steps: one step's arrays in its L2
+    const int64_t i64 = (int64_t)bx * kBlock + threadIdx.x;
     if (i64 >= n) return;
-    const int64_t s = blockIdx.y;
+    const int64_t s = by;
     const uint32_t* row_off = row_off_all + s * st.row;
     if ((int64_t)row_off[n] > st.pair) return;  // caller enlarges and calls again
     const uint32_t i = (uint32_t)i64;

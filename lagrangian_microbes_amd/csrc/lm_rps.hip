@@ -196,12 +196,14 @@ rps_init_kernel(int64_t n, RpsArrays A, RpsStrides st) {
 __global__ void __launch_bounds__(kBlock)
 rps_hook_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __restrict__ pj_all,
                 const int64_t* __restrict__ npairs, RpsArrays A, RpsStrides st) {
-    const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
+    uint32_t bx, by;
+    xcd_block(&bx, &by);  // an XCD works through a contiguous range of steps: one step's arrays in its L2
+    const int64_t s = by, ro = s * st.row, po = s * st.pair;
     const int64_t P = step_pairs(npairs, s, st.pair);
     const int32_t* pi = pi_all + po;
     const int32_t* pj = pj_all + po;
     uint32_t* parent = A.parent + ro;
-    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
+    for (int64_t k = (int64_t)bx * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
         const uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
         if (a < (uint64_t)n && b < (uint64_t)n && a != b) parent[max(a, b)] = min(a, b);
     }
@@ -223,14 +225,16 @@ __global__ void __launch_bounds__(kBlock)
 rps_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __restrict__ pj_all,
                  const int64_t* __restrict__ npairs, RpsArrays A, RpsStrides st, double pRS, double pPR,
                  double pSP, uint64_t k0, uint64_t k1, uint64_t step0, RunsView R) {
-    const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
+    uint32_t bx, by;
+    xcd_block(&bx, &by);  // an XCD works through a contiguous range of steps: one step's arrays in its L2
+    const int64_t s = by, ro = s * st.row, po = s * st.pair;
     const int64_t P = step_pairs(npairs, s, st.pair);
-    if (blockIdx.x == 0 && threadIdx.x == 0) A.plen[s] = P < 0 ? 0 : P;
+    if (bx == 0 && threadIdx.x == 0) A.plen[s] = P < 0 ? 0 : P;
     const int32_t* pi = pi_all + po;
     const int32_t* pj = pj_all + po;
     uint32_t* parent = A.parent + ro;
     uint8_t* bits = A.bits + po;
-    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
+    for (int64_t k = (int64_t)bx * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
         const uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
         uint64_t rank = (uint64_t)k, q0 = k0, q1 = k1;
         double a_rs = pRS, a_pr = pPR, a_sp = pSP;
@@ -290,8 +294,10 @@ constexpr int kKeyItems = 4;
 
 __global__ void __launch_bounds__(kBlock)
 rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
-    const int64_t s = blockIdx.y, ro = s * st.row;
-    const int64_t v0 = (int64_t)blockIdx.x * kBlock * kKeyItems + threadIdx.x;
+    uint32_t bx, by;
+    xcd_block(&bx, &by);  // an XCD works through a contiguous range of steps: one step's arrays in its L2
+    const int64_t s = by, ro = s * st.row;
+    const int64_t v0 = (int64_t)bx * kBlock * kKeyItems + threadIdx.x;
     uint32_t x[kKeyItems], q[kKeyItems];
     bool t[kKeyItems], act[kKeyItems];
 #pragma unroll
@@ -370,12 +376,14 @@ __global__ void __launch_bounds__(64 * kGroupWaves)
 rps_group_blk_kernel(int64_t n, RpsArrays A, RpsStrides st, const uint32_t* __restrict__ kin,
                      const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                      const uint32_t* __restrict__ dstart, int64_t drow, int64_t dstride, uint32_t fine) {
+    uint32_t bx, by;
+    xcd_block(&bx, &by);  // an XCD works through a contiguous range of steps: one step's arrays in its L2
     static_assert(64 * kGroupWaves == kBlock, "the block scan assumes kBlock threads");
     constexpr uint32_t NT = 64 * kGroupWaves;
     __shared__ uint32_t cnt[1u << FB];
     __shared__ uint32_t scan_lds[kGroupWaves];
-    const uint32_t d = blockIdx.x, nsuper = gridDim.x, t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-    const int64_t s = blockIdx.y, ro = s * st.row;
+    const uint32_t d = bx, nsuper = gridDim.x, t = threadIdx.x, lane = t & 63u, wave = t >> 6;
+    const int64_t s = by, ro = s * st.row;
     const uint32_t nn = (uint32_t)n;
     kin += ro;
     vin += ro;
@@ -493,8 +501,10 @@ rps_group_blk_kernel(int64_t n, RpsArrays A, RpsStrides st, const uint32_t* __re
 // reserves its list entries with one global add per list.
 __global__ void __launch_bounds__(kBlock)
 rps_comps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
+    uint32_t bx, by;
+    xcd_block(&bx, &by);  // an XCD works through a contiguous range of steps: one step's arrays in its L2
     __shared__ uint32_t cnt[2], base[2];
-    const int64_t s = blockIdx.y, ro = s * st.row;
+    const int64_t s = by, ro = s * st.row;
     const uint32_t* spk = A.spk + ro;
     const uint32_t* roff = A.roff + ro;
     // run bounds are clamped to the step's pairs, so that pairs not sorted by i
@@ -512,7 +522,7 @@ rps_comps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
     bool ok[kCompItems];
 #pragma unroll
     for (int r = 0; r < kCompItems; ++r) {
-        const int64_t e = ((int64_t)blockIdx.x * kCompItems + r) * kBlock + threadIdx.x;
+        const int64_t e = ((int64_t)bx * kCompItems + r) * kBlock + threadIdx.x;
         const int64_t ec = e < n ? e : (int64_t)last;
         L[r] = spk[ec];
         x[r] = A.ids[ro + ec];
@@ -523,7 +533,7 @@ rps_comps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
     uint32_t tch[kCompItems], pb[kCompItems], rbg[kCompItems], ro_e[kCompItems];
 #pragma unroll
     for (int r = 0; r < kCompItems; ++r) {
-        const int64_t e = ((int64_t)blockIdx.x * kCompItems + r) * kBlock + threadIdx.x;
+        const int64_t e = ((int64_t)bx * kCompItems + r) * kBlock + threadIdx.x;
         const int64_t ec = e < n ? e : (int64_t)last;
         const uint2 rw = A.rows[ro + x[r]];  // {row begin, row end | touched << 31}
         tch[r] = rw.y >> 31;
@@ -533,7 +543,7 @@ rps_comps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
     }
 #pragma unroll
     for (int r = 0; r < kCompItems; ++r) {
-        const int64_t e = ((int64_t)blockIdx.x * kCompItems + r) * kBlock + threadIdx.x;
+        const int64_t e = ((int64_t)bx * kCompItems + r) * kBlock + threadIdx.x;
         at[r] = kUnset;
         d[r] = make_uint4(0u, 0u, 0u, 0u);
         if (!ok[r] || !tch[r]) continue;  // past the end or untouched
@@ -562,12 +572,14 @@ rps_comps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
 __global__ void __launch_bounds__(kBlock)
 rps_fill_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __restrict__ pj_all, RpsArrays A,
                 RpsStrides st) {
-    const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
+    uint32_t bx, by;
+    xcd_block(&bx, &by);  // an XCD works through a contiguous range of steps: one step's arrays in its L2
+    const int64_t s = by, ro = s * st.row, po = s * st.pair;
     const int64_t P = A.plen[s];
     const int32_t* pi = pi_all + po;
     const int32_t* pj = pj_all + po;
     const uint2* info = A.info + ro;
-    for (int64_t k = (int64_t)blockIdx.x * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
+    for (int64_t k = (int64_t)bx * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
         const uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
         if (a >= (uint64_t)n) continue;
         const uint2 ia = info[a];
@@ -589,17 +601,19 @@ constexpr uint32_t kDepsTable = 2048;
 
 __global__ void __launch_bounds__(64)
 rps_deps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
+    uint32_t bx, by;
+    xcd_block(&bx, &by);  // an XCD works through a contiguous range of steps: one step's arrays in its L2
     __shared__ unsigned long long mask[kDepsTable];
-    const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
+    const int64_t s = by, ro = s * st.row, po = s * st.pair;
     const uint32_t lane = threadIdx.x;
     const uint32_t cnt = A.lists[4 * s + 1];
-    if (blockIdx.x >= cnt) return;  // uniform: no component for this block
+    if (bx >= cnt) return;  // uniform: no component for this block
     const uint2* rec = A.rec + po;
     unsigned long long* deps = A.deps + po;
     for (uint32_t t = lane; t < kDepsTable; t += 64) mask[t] = 0ull;
     __syncthreads();
     const unsigned long long bit = 1ull << lane;
-    for (uint32_t c = blockIdx.x; c < cnt; c += gridDim.x) {
+    for (uint32_t c = bx; c < cnt; c += gridDim.x) {
         const uint4 dsc = A.wide[ro + c];
         const uint32_t qb = dsc.x, np = dsc.y, psz = dsc.w;
         const bool table = psz <= kDepsTable;  // uniform over the wavefront
@@ -651,16 +665,18 @@ __device__ __forceinline__ uint32_t half_of(uint32_t word, uint32_t l) { return 
 
 __global__ void __launch_bounds__(64)
 rps_levels_kernel(int64_t n, RpsArrays A, RpsStrides st) {
+    uint32_t bx, by;
+    xcd_block(&bx, &by);  // an XCD works through a contiguous range of steps: one step's arrays in its L2
     static_assert(kLevelPairs < 65536u, "level counts must fit 16 bits");
     __shared__ uint32_t words[(kLevelSlots + 1) / 2];  // last[] during the level phase, then the packed counts
     __shared__ uint16_t lvl[kLevelPairs];
     uint16_t* last = reinterpret_cast<uint16_t*>(words);
-    const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
+    const int64_t s = by, ro = s * st.row, po = s * st.pair;
     const uint32_t lane = threadIdx.x;
     const uint32_t ncomp = A.lists[4 * s + 1];
     const uint2* rec = A.rec + po;
     const unsigned long long* deps = A.deps + po;
-    for (uint32_t c = blockIdx.x; c < ncomp; c += gridDim.x) {
+    for (uint32_t c = bx; c < ncomp; c += gridDim.x) {
         const uint4 dsc = A.wide[ro + c];
         const uint32_t qb = dsc.x, np = dsc.y, psz = dsc.w;
         if (np > kLevelPairs || psz > kLevelSlots || psz == 0) {

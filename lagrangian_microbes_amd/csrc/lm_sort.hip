@@ -51,16 +51,18 @@ __global__ void __launch_bounds__(kSortThreads)
 radix_hist_kernel(const uint32_t* __restrict__ keys, int64_t stride, const int64_t* __restrict__ seg_len,
                   int64_t uniform_len, int64_t max_len, int shift, uint32_t dmask, uint32_t* __restrict__ hist,
                   int64_t row, int64_t ntp) {
+    uint32_t bx, by;
+    xcd_block(&bx, &by);  // an XCD works through a contiguous range of steps: one step's arrays in its L2
     constexpr int R = 1 << BITS;
     static_assert(kSortGroup == 4, "one uint4 of counts per digit");
     __shared__ uint32_t h[kSortGroup][R];
-    const int g = blockIdx.y;
+    const int g = by;
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t len = seg_len_of(seg_len, uniform_len, max_len, g);
     for (int q = t; q < kSortGroup * R; q += kSortThreads) (&h[0][0])[q] = 0;
     __syncthreads();
     const uint32_t* k = keys + (int64_t)g * stride;
-    const int64_t first = (int64_t)blockIdx.x * kSortGroup * kSortTile + wave * kWaveChunk;
+    const int64_t first = (int64_t)bx * kSortGroup * kSortTile + wave * kWaveChunk;
     uint32_t kk[kSortGroup][kSortItems];
 #pragma unroll
     for (int j = 0; j < kSortGroup; ++j)
@@ -80,7 +82,7 @@ radix_hist_kernel(const uint32_t* __restrict__ keys, int64_t stride, const int64
         }
     __syncthreads();
     for (int q = t; q < R; q += kSortThreads)
-        *reinterpret_cast<uint4*>(hist + (int64_t)g * row + (int64_t)q * ntp + (int64_t)blockIdx.x * kSortGroup) =
+        *reinterpret_cast<uint4*>(hist + (int64_t)g * row + (int64_t)q * ntp + (int64_t)bx * kSortGroup) =
             make_uint4(h[0][q], h[1][q], h[2][q], h[3][q]);
 }
 
@@ -90,6 +92,8 @@ radix_scatter_kernel(const uint32_t* __restrict__ kin, const uint32_t* __restric
                      uint32_t* __restrict__ vout, int64_t stride, const int64_t* __restrict__ seg_len,
                      int64_t uniform_len, int64_t max_len, int shift, uint32_t dmask,
                      const uint32_t* __restrict__ offs, int64_t row, int64_t ntp) {
+    uint32_t bx, by;
+    xcd_block(&bx, &by);  // an XCD works through a contiguous range of steps: one step's arrays in its L2
     constexpr int R = 1 << BITS;
     constexpr int PER = R / kSortThreads;  // digits per thread in the digit scan
     static_assert(PER >= 1 && R % kSortThreads == 0, "digit count must be a multiple of the block");
@@ -98,10 +102,10 @@ radix_scatter_kernel(const uint32_t* __restrict__ kin, const uint32_t* __restric
     __shared__ uint32_t gdelta[R];            // global start - local start (mod 2^32)
     __shared__ uint32_t skey[kSortTile], sval[kSortTile];
     __shared__ uint32_t scan_lds[kSortWaves];
-    const int g = blockIdx.y;
+    const int g = by;
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int64_t len = seg_len_of(seg_len, uniform_len, max_len, g);
-    const int64_t group0 = (int64_t)blockIdx.x * kSortGroup * kSortTile;
+    const int64_t group0 = (int64_t)bx * kSortGroup * kSortTile;
     if (group0 >= len) return;  // uniform over the block
     const uint32_t* ki = kin + (int64_t)g * stride;
     // vin == NULL: each element's value is its index in the segment
@@ -112,7 +116,7 @@ radix_scatter_kernel(const uint32_t* __restrict__ kin, const uint32_t* __restric
 #pragma unroll
     for (int q = 0; q < PER; ++q)
         gb[q] = *reinterpret_cast<const uint4*>(offs + (int64_t)g * row + (int64_t)(t * PER + q) * ntp +
-                                                (int64_t)blockIdx.x * kSortGroup);
+                                                (int64_t)bx * kSortGroup);
     const unsigned long long below = (1ull << lane) - 1ull;
     uint32_t key[kSortItems], val[kSortItems];
 #pragma unroll
