@@ -317,6 +317,20 @@ int lm_adjoint_forward(int64_t n, double* lon, double* lat, const float* field, 
                        int32_t ckpt, double* checkpoints, void* scratch, int64_t scratch_bytes, uint32_t* oob_count,
                        void* stream);
 
+/* lm_adjoint_forward without the scratch argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No reference counterpart.  Calls no
+ * hipMalloc, hipFree or allocating library routine: the stage fields go to a
+ * static device array of 32 MiB per device, so the grid may have at most
+ * 699,050 nodes (2,097,152 when nlev == 1); larger grids are rejected with
+ * LM_ERR_ARG.  The array is shared by every call on a device and keeps the
+ * last call's contents: calls that overlap on one device race.  Results are
+ * those of lm_adjoint_forward with the same arguments. */
+int lm_adjoint_forward_noalloc(int64_t n, double* lon, double* lat, const float* field, int32_t nlev,
+                               const lm_grid* grid, const int32_t* lev0, const double* w, const int32_t* lev0_host,
+                               int32_t nsteps, double dt, int32_t ckpt, double* checkpoints, uint32_t* oob_count,
+                               void* stream);
+
 /* Bytes of device scratch lm_adjoint_backward needs for n particles,
  * checkpoint interval ckpt and the field's grid and level count (-1 on
  * invalid arguments). */

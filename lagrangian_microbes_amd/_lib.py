@@ -63,6 +63,8 @@ _SIGNATURES = {
     "lm_distance_histogram": (ctypes.c_int, [c_i64, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
     "lm_adjoint_forward": (ctypes.c_int, [c_i64, c_vp, c_vp, c_vp, c_i32, ctypes.POINTER(LmGrid), c_vp, c_vp, c_vp,
                                           c_i32, c_dbl, c_i32, c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "lm_adjoint_forward_noalloc": (ctypes.c_int, [c_i64, c_vp, c_vp, c_vp, c_i32, ctypes.POINTER(LmGrid), c_vp,
+                                                  c_vp, c_vp, c_i32, c_dbl, c_i32, c_vp, c_vp, c_vp]),
     "lm_adjoint_scratch_bytes": (c_i64, [c_i64, c_i32, ctypes.POINTER(LmGrid), c_i32]),
     "lm_adjoint_backward": (ctypes.c_int, [c_i64, c_vp, c_vp, c_i32, ctypes.POINTER(LmGrid), c_vp, c_vp, c_vp,
                                            c_i32, c_dbl, c_i32, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),

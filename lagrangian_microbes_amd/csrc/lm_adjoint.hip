@@ -47,6 +47,8 @@
 // with adds its moments to a global slab, turned into node values at the end.
 #include "lm_common.h"
 
+#include <mutex>
+
 namespace lm {
 
 constexpr int kAdjBlock = 256;
@@ -629,6 +631,41 @@ int adj_blend(const float* field, int32_t nlev, const lm_grid* grid, const int32
     return LM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Static scratch of lm_adjoint_forward_noalloc: the stage fields live in this
+// file-scope device array instead of caller memory.  Each device holds its
+// own copy at its own address from code-object load to process exit.  On the
+// host, the array's name is only a stand-in for the device symbol: host code
+// never passes it to a kernel or a copy.  Its device address is looked up
+// once per device (hipGetSymbolAddress, with that device current) and kept;
+// every call on a device shares that device's array, so calls that overlap
+// on one device race.  Nothing in it needs clearing: each chunk's stage
+// fields are written by the blend before the advection reads them.
+// ---------------------------------------------------------------------------
+constexpr int64_t kStaticScratchBytes = 32ll << 20;
+constexpr int kStaticScratchDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+std::mutex g_static_mu;
+void* g_static_ptr[kStaticScratchDevices];
+
+// Device address of `scratch` on the current device.
+int static_scratch(void** out) {
+    int dev = -1;
+    LM_HIP_CHECK(hipGetDevice(&dev));
+    LM_REQUIRE(dev >= 0 && dev < kStaticScratchDevices, LM_ERR_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    if (g_static_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        LM_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        LM_REQUIRE(p != nullptr && ((uintptr_t)p & 255) == 0, LM_ERR_HIP, "static scratch has no device address");
+        g_static_ptr[dev] = p;
+    }
+    *out = g_static_ptr[dev];
+    return LM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -670,6 +707,27 @@ int lm_adjoint_forward(int64_t n, double* lon, double* lat, const float* field, 
         LM_LAUNCH_CHECK();
     }
     return LM_OK;
+}
+
+int lm_adjoint_forward_noalloc(int64_t n, double* lon, double* lat, const float* field, int32_t nlev,
+                               const lm_grid* grid, const int32_t* lev0, const double* w, const int32_t* lev0_host,
+                               int32_t nsteps, double dt, int32_t ckpt, double* checkpoints, uint32_t* oob_count,
+                               void* stream) {
+    int rc = adj_check(n, nlev, grid, lev0_host, nsteps, ckpt);
+    if (rc) return rc;
+    if (n == 0 || nsteps == 0) return LM_OK;
+    // one step blends 3 stage fields (1 for a steady field) of 16 bytes per node
+    const int64_t nodes = level_len_of(grid);
+    const int64_t per_node = nlev == 1 ? 16 : 48;
+    LM_REQUIRE(nodes <= kStaticScratchBytes / per_node, LM_ERR_ARG,
+               "grid of %lld nodes: the static scratch of %lld bytes holds the stage fields of at most %lld nodes",
+               (long long)nodes, (long long)kStaticScratchBytes, (long long)(kStaticScratchBytes / per_node));
+    LM_REQUIRE(lon && lat && field && lev0 && w && checkpoints && oob_count, LM_ERR_ARG, "NULL pointer");
+    void* buf = nullptr;
+    rc = static_scratch(&buf);
+    if (rc) return rc;
+    return lm_adjoint_forward(n, lon, lat, field, nlev, grid, lev0, w, lev0_host, nsteps, dt, ckpt, checkpoints, buf,
+                              kStaticScratchBytes, oob_count, stream);
 }
 
 int lm_adjoint_backward(int64_t n, const double* checkpoints, const float* field, int32_t nlev, const lm_grid* grid,
