@@ -56,6 +56,10 @@ struct Metric {
     double r2;   // radius*radius (p = 2) or radius^p (general p)
     double p;
     float box;   // float32 bound above the radius: |dx| or |dy| beyond it rules a pair out
+    // p = 2: a float32 dx*dx + dy*dy at or below sure_in is within the radius,
+    // above sure_out beyond it, whatever the float64 test rounds (-inf / +inf:
+    // every candidate takes the float64 test)
+    float sure_in, sure_out;
 };
 
 // Per-step strides of a batch (elements).
@@ -144,6 +148,7 @@ bin_keys_kernel(int64_t n, const float* __restrict__ x, const float* __restrict_
 // order inside a bucket is free: the pair list is sorted later).  FINE_MAX
 // buckets per block: 4096 up to 2^22 buckets, 8192 up to 2^23.
 constexpr uint32_t kFineMax = 4096;
+constexpr int kPlace = 4;  // particles per thread in flight when placing records
 constexpr uint32_t kFineMaxWide = 8192;
 
 template <uint32_t FINE_MAX>
@@ -191,15 +196,26 @@ bin_table_kernel(int64_t n, const float2* __restrict__ xy, const uint16_t* __res
         if (base + f < (uint64_t)g.T) start[base + f] = a + cnt[f];
     if (d + 1 == nsuper && t == 0) start[g.T] = nn;
     __syncthreads();
-    for (uint32_t e = a + t; e < b; e += kBlock) {
-        const uint32_t i = vals[e];
-        const uint32_t pos = a + atomicAdd(&cnt[keys[e] & fmask], 1u);
-        if (i < nn && pos < b)
-        {
-            const float2 p = xy[s * st.rec + i];
-            recs_all[s * st.rec + pos] =
-                make_float4(p.x, p.y, __uint_as_float(i), __uint_as_float(run ? (uint32_t)run[i] : 0u));
+    // kPlace particles per thread at once, so their position gathers (random
+    // lines) are in flight together
+    for (uint32_t e0 = a + t; e0 < b; e0 += kBlock * kPlace) {
+        uint32_t ii[kPlace], pp[kPlace];
+#pragma unroll
+        for (int u = 0; u < kPlace; ++u) {
+            const uint32_t e = e0 + (uint32_t)u * kBlock;
+            const bool live = e < b;
+            ii[u] = live ? vals[e] : nn;
+            pp[u] = live ? a + atomicAdd(&cnt[keys[e] & fmask], 1u) : b;
         }
+        float2 pv[kPlace];
+#pragma unroll
+        for (int u = 0; u < kPlace; ++u)
+            pv[u] = ii[u] < nn && pp[u] < b ? xy[s * st.rec + ii[u]] : make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int u = 0; u < kPlace; ++u)
+            if (ii[u] < nn && pp[u] < b)
+                recs_all[s * st.rec + pp[u]] = make_float4(pv[u].x, pv[u].y, __uint_as_float(ii[u]),
+                                                           __uint_as_float(run ? (uint32_t)run[ii[u]] : 0u));
     }
 }
 
@@ -272,12 +288,15 @@ pair_count_kernel(int64_t n, CellGrid g, Metric m, Strides st, const uint32_t* _
     uint32_t rb[6], re[6];
     stencil_ranges(g, start, (uint32_t)n, cx, cy, my_run, rb, re);
     // the ranges as one sequence: candidate t lies in range r when
-    // pre[r] <= t < pre[r + 1]
-    uint32_t pre[7];
+    // pre[r] <= t < pre[r + 1], at slot t + dlt[r]
+    uint32_t pre[6], dlt[6];
     pre[0] = 0;
 #pragma unroll
-    for (int r = 0; r < 6; ++r) pre[r + 1] = pre[r] + (re[r] - rb[r]);
-    const uint32_t total = pre[6];
+    for (int r = 0; r < 6; ++r) {
+        dlt[r] = rb[r] - pre[r];  // mod 2^32
+        if (r < 5) pre[r + 1] = pre[r] + (re[r] - rb[r]);
+    }
+    const uint32_t total = pre[5] + (re[5] - rb[5]);
     uint32_t cnt = 0;
     uint32_t* row = nbr + s * st.nbr + (int64_t)i * kNbr;
     for (uint32_t t0 = 0; t0 < total; t0 += kCand) {
@@ -286,24 +305,36 @@ pair_count_kernel(int64_t n, CellGrid g, Metric m, Strides st, const uint32_t* _
         for (int u = 0; u < kCand; ++u) {
             // past the end: the last candidate again (a valid slot), discarded below
             const uint32_t t = min(t0 + u, total - 1u);
-            uint32_t q = rb[0] + t;
+            uint32_t d = dlt[0];
 #pragma unroll
-            for (int r = 1; r < 6; ++r) q = t >= pre[r] ? rb[r] + (t - pre[r]) : q;
-            o[u] = recs[q];
+            for (int r = 1; r < 6; ++r) d = t >= pre[r] ? dlt[r] : d;
+            o[u] = recs[t + d];
         }
 #pragma unroll
         for (int u = 0; u < kCand; ++u) {
             const uint32_t j = __float_as_uint(o[u].z);
-            // |dx| <= r and |dy| <= r hold for every p >= 1; a float32
-            // difference is within one rounding (relative 2^-24) of the exact
-            // one and m.box lies above r by more than that, so the prefilter
-            // only drops pairs the exact test drops.
-            // One predicate, evaluated without branches; one branch for the rare store.
-            const float ax = fabsf(o[u].x - me.x), ay = fabsf(o[u].y - me.y);
-            const bool cand = (t0 + u < total) & (j > i) & (__float_as_uint(o[u].w) == my_run) & (ax <= m.box) &
-                              (ay <= m.box);
-            const bool near = within<KIND>(m, me.x, me.y, o[u].x, o[u].y);
-            const bool hit = cand & near;
+            const bool cand = (t0 + u < total) & (j > i) & (__float_as_uint(o[u].w) == my_run);
+            bool hit;
+            if constexpr (KIND == kL2) {
+                // A float32 difference of two float32 positions is within one
+                // rounding (relative 2^-24) of the exact one, so the float32
+                // dx*dx + dy*dy is within 2.4e-7 (relative) of the exact sum of
+                // squares, and the float64 test within 2^-52: outside r^2 (1 -/+
+                // 1e-6) both decide alike.  Only candidates in that band take
+                // the float64 test (a branch almost never taken).
+                const float dx = o[u].x - me.x, dy = o[u].y - me.y;
+                const float d2 = fmaf(dx, dx, dy * dy);
+                hit = cand & (d2 <= m.sure_in);
+                if (cand & !(d2 <= m.sure_in) & !(d2 > m.sure_out))
+                    hit = within<KIND>(m, me.x, me.y, o[u].x, o[u].y);
+            } else {
+                // |dx| <= r and |dy| <= r hold for every p >= 1; a float32
+                // difference is within one rounding (relative 2^-24) of the exact
+                // one and m.box lies above r by more than that, so the prefilter
+                // only drops pairs the exact test drops.
+                const float ax = fabsf(o[u].x - me.x), ay = fabsf(o[u].y - me.y);
+                hit = cand & (ax <= m.box) & (ay <= m.box) & within<KIND>(m, me.x, me.y, o[u].x, o[u].y);
+            }
             if (hit & (cnt < kNbr)) row[cnt] = j;
             cnt += hit ? 1u : 0u;
         }
@@ -608,9 +639,17 @@ int lm_query_pairs_runs(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float*
     m.p = p;
     // above r by more than float32 rounding of a difference (2^-24 relative)
     m.box = std::nextafter((float)(radius * (1.0 + 1e-6)), INFINITY);
+    m.sure_in = -INFINITY;
+    m.sure_out = INFINITY;
     if (p == 2.0) {
         m.kind = lm::kL2;
         m.r2 = radius * radius;
+        // far from float32 underflow and overflow, the float32 bounds of the
+        // band around r^2 in which candidates take the float64 test
+        if (m.r2 >= 1e-30 && m.r2 <= 1e30) {
+            m.sure_in = std::nextafter((float)(m.r2 * (1.0 - 1e-6)), -INFINITY);
+            m.sure_out = std::nextafter((float)(m.r2 * (1.0 + 1e-6)), INFINITY);
+        }
     } else if (p == 1.0) {
         m.kind = lm::kL1;
         m.r2 = radius;

@@ -1227,8 +1227,10 @@ static int rps_prepare_impl(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int3
     LM_LAUNCH(lm::K_RPS_INIT, lm::rps_init_kernel, dim3(nbp, S), dim3(lm::kBlock), 0, s, n, A, sp);
     LM_LAUNCH_CHECK();
     if (pair_stride > 0) {
-        // one thread per pair slot of the capacity (threads past a step's count exit)
-        const unsigned npb = lm::grid_for(pair_stride);
+        // grid-stride over a step's pairs: at most one thread per particle (a
+        // step has fewer pairs than particles but in dense runs; the capacity,
+        // often twice the particles, would launch idle blocks)
+        const unsigned npb = lm::grid_for(pair_stride < n ? pair_stride : n);
         LM_LAUNCH(lm::K_RPS_HOOK, lm::rps_hook_kernel, dim3(npb, S), dim3(lm::kBlock), 0, s, n, pair_i, pair_j,
                   npairs, A, sp);
         LM_LAUNCH_CHECK();

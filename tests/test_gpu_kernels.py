@@ -105,6 +105,47 @@ def test_pairs_match_bruteforce(gpu, seed, n, spread, p):
         assert np.array_equal(gi, ki) and np.array_equal(gj, kj)
 
 
+def band_pairs(lo, rng, ncen=8):
+    """ncen^2 isolated pairs on the float32 grid of the binade [lo, 2 lo) (lo a power of two, or its
+    negative): centre and partner are exact float32 values u apart, so a pair's squared distance is
+    exactly (a^2 + b^2) u^2.  The radius is sqrt(N) u with N near (|lo| / 40 / u)^2 and the offsets
+    (a, b) put (a^2 + b^2) / N - 1 across -5e-6 .. 5e-6: inside the band around r^2 where the p = 2
+    test takes float64, and on both sides just outside it."""
+    u = float(np.spacing(np.float32(abs(lo))))
+    r = abs(lo) / 40
+    N = round((r / u) ** 2)
+    span = abs(lo)
+    cen = abs(lo) + span * (0.05 + 0.9 * (np.arange(ncen) + 0.5) / ncen)  # spacing > 3 r: pairs stay apart
+    assert span * 0.9 / ncen > 3 * r
+    cx, cy = [c.ravel() for c in np.meshgrid(cen, cen)]
+    sgn = 1.0 if lo > 0 else -1.0
+    cx = (sgn * cx / u).round() * u
+    cy = (sgn * cy / u).round() * u
+    m = cx.size
+    target = rng.uniform(-5e-6, 5e-6, m)
+    a = np.floor(rng.uniform(0.2, 0.7, m) * np.sqrt(N))
+    b = np.round(np.sqrt(N * (1 + target) - a * a))
+    rel = (a * a + b * b) / N - 1
+    x = np.concatenate([cx, cx + a * u]).astype(np.float32)
+    y = np.concatenate([cy, cy + b * u]).astype(np.float32)
+    assert np.array_equal(x[m:].astype(np.float64) - x[:m], a * u)  # exact float32 grid points
+    return x, y, np.sqrt(N) * u, rel
+
+
+@pytest.mark.parametrize("lo", [0.25, -0.125, 2.0 ** -10])
+def test_pairs_near_the_radius(gpu, lo):
+    rng = np.random.default_rng(7)
+    x, y, r, rel = band_pairs(lo, rng)
+    assert np.sum(np.abs(rel) < 1e-6) >= 5 and np.sum(rel > 1e-6) >= 5 and np.sum(rel < -1e-6) >= 5
+    gi, gj = gpu_pairs(x, y, r)
+    bi, bj = oracle.pairs_bruteforce(x, y, r)
+    ki, kj = sorted_kdtree_pairs(x, y, r)
+    assert np.array_equal(gi, bi) and np.array_equal(gj, bj)
+    assert np.array_equal(gi, ki) and np.array_equal(gj, kj)
+    m = rel.size
+    assert np.array_equal(np.sort(bi), np.flatnonzero(rel <= 0)) and np.all(bj == bi + m)
+
+
 def test_pairs_dense_cluster_selection_path(gpu):
     # 400 points inside one radius: every particle has far more than 8 neighbours
     rng = np.random.default_rng(3)
