@@ -379,16 +379,20 @@ pair_emit_kernel(int64_t n, const float* __restrict__ x_all, const float* __rest
     if (i64 >= n) return;
     const int64_t s = by;
     const uint32_t* row_off = row_off_all + s * st.row;
-    if ((int64_t)row_off[n] > st.pair) return;  // caller enlarges and calls again
     const uint32_t i = (uint32_t)i64;
+    // the step's total, this row's count and offset load together, before
+    // any branch on them
+    const uint32_t total = row_off[n];
     const uint32_t c = nbr_cnt_all[s * st.row + i];
-    if (c == 0) return;
     const uint32_t o = row_off[i];
+    if ((int64_t)total > st.pair) return;  // caller enlarges and calls again
+    if (c == 0) return;
     int32_t* pair_i = pair_i_all + s * st.pair;
     int32_t* pair_j = pair_j_all + s * st.pair;
     const uint32_t* row = nbr_all + s * st.nbr + (int64_t)i * kNbr;
     if (c <= 2) {
-        const uint32_t a = row[0], b = c == 2 ? row[1] : 0xffffffffu;
+        const uint2 ab = *reinterpret_cast<const uint2*>(row);  // rows are 32-byte aligned
+        const uint32_t a = ab.x, b = c == 2 ? ab.y : 0xffffffffu;
         pair_i[o] = (int32_t)i;
         pair_j[o] = (int32_t)min(a, b);
         if (c == 2) {

@@ -235,7 +235,14 @@ rps_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* _
     uint32_t* parent = A.parent + ro;
     uint8_t* bits = A.bits + po;
     for (int64_t k = (int64_t)bx * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
+        // the loads this pair needs are issued together (indices clamped, no
+        // branch between them), so they overlap each other and the draw below
         const uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
+        const uint32_t before = (uint32_t)pi[k > 0 ? k - 1 : 0], after = (uint32_t)pi[k + 1 < P ? k + 1 : k];
+        const uint32_t hi = max(a, b), lo = min(a, b);
+        // parent[b] == a was set by the hook pass, before this kernel: joins
+        // are permanent, so a pair seen joined at any time is joined
+        const uint32_t phi = parent[hi < (uint64_t)n ? hi : 0u];
         uint64_t rank = (uint64_t)k, q0 = k0, q1 = k1;
         double a_rs = pRS, a_pr = pPR, a_sp = pSP;
         if (RUNS) {
@@ -252,15 +259,15 @@ rps_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* _
         bits[k] = (uint8_t)((u < a_rs ? 1u : 0u) | (u < a_pr ? 2u : 0u) | (u < a_sp ? 4u : 0u));
         if (a < (uint64_t)n) {
             uint32_t* row = reinterpret_cast<uint32_t*>(A.rows + ro + a);  // {begin, end}
-            if (k == 0 || (uint32_t)pi[k - 1] != a) row[0] = (uint32_t)k;
-            if (k + 1 == P || (uint32_t)pi[k + 1] != a) row[1] = (uint32_t)(k + 1);
+            if (k == 0 || before != a) row[0] = (uint32_t)k;
+            if (k + 1 == P || after != a) row[1] = (uint32_t)(k + 1);
             // a has its row and the hook pointed b below a, which is how the
             // keys pass tells both have pairs; only a j below its i (never
             // in a list from the pair search) is marked here
             if (b < (uint64_t)n) {
                 if (b < a) A.touched[ro + b] = 1;
                 // parent[b] == a: b hangs below a (sets only merge), nothing to join
-                if (a != b && parent[max(a, b)] != min(a, b)) unite(parent, a, b);
+                if (a != b && phi != lo) unite(parent, a, b);
             }
         }
     }
@@ -580,12 +587,14 @@ rps_fill_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __
     const int32_t* pj = pj_all + po;
     const uint2* info = A.info + ro;
     for (int64_t k = (int64_t)bx * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
+        // two rounds of loads, each issued together (indices clamped, no branch)
         const uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
+        const uint32_t fb = A.bits[po + k];
+        const uint2 ia = info[a < (uint64_t)n ? a : 0u], ib = info[b < (uint64_t)n ? b : 0u];
         if (a >= (uint64_t)n) continue;
-        const uint2 ia = info[a];
-        const uint32_t sb = b < (uint64_t)n ? info[b].y : ia.y;
+        const uint32_t sb = b < (uint64_t)n ? ib.y : ia.y;
         const uint32_t pos = ia.x + (uint32_t)k;
-        if (pos < (uint64_t)P) A.rec[po + pos] = make_uint2((ia.y & kSlotMask) | ((uint32_t)A.bits[po + k] << 29), sb);
+        if (pos < (uint64_t)P) A.rec[po + pos] = make_uint2((ia.y & kSlotMask) | (fb << 29), sb);
     }
 }
 
@@ -822,6 +831,48 @@ __device__ __forceinline__ void play_rows(int8_t* slots, const uint32_t (&cnt)[k
     }
 }
 
+// A wavefront stages the species of particles ids[0 .. psz) (psz >= 1) into
+// LDS slots, or writes the slots back: kStage particles per lane at a time,
+// all their id loads in flight together, then all their species gathers (or
+// stores, each from its own register), so a component of P particles costs
+// ~2 P / (64 kStage) memory latencies rather than two per 64 particles.
+// Particles at or past n (none in a valid list) read as 0 and are not written.
+constexpr int kStage = 8;
+
+__device__ __forceinline__ void stage_in(int64_t n, const uint32_t* __restrict__ ids, uint32_t psz,
+                                         const int8_t* __restrict__ sp_in, int8_t* slots) {
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t t0 = 0; t0 < psz; t0 += 64u * kStage) {
+        uint32_t x[kStage];
+        int8_t v[kStage];
+#pragma unroll
+        for (int q = 0; q < kStage; ++q) x[q] = ids[min(t0 + 64u * q + lane, psz - 1u)];
+#pragma unroll
+        for (int q = 0; q < kStage; ++q) v[q] = sp_in[x[q] < (uint64_t)n ? x[q] : 0u];
+#pragma unroll
+        for (int q = 0; q < kStage; ++q) {
+            const uint32_t t = t0 + 64u * q + lane;
+            if (t < psz) slots[t] = x[q] < (uint64_t)n ? v[q] : (int8_t)0;
+        }
+    }
+}
+
+__device__ __forceinline__ void stage_out(int64_t n, const uint32_t* __restrict__ ids, uint32_t psz,
+                                          const int8_t* slots, int8_t* __restrict__ sp_out) {
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t t0 = 0; t0 < psz; t0 += 64u * kStage) {
+        uint32_t x[kStage];
+        int8_t v[kStage];
+#pragma unroll
+        for (int q = 0; q < kStage; ++q) x[q] = ids[min(t0 + 64u * q + lane, psz - 1u)];
+#pragma unroll
+        for (int q = 0; q < kStage; ++q) v[q] = slots[min(t0 + 64u * q + lane, psz - 1u)];
+#pragma unroll
+        for (int q = 0; q < kStage; ++q)
+            if (t0 + 64u * q + lane < psz && x[q] < (uint64_t)n) sp_out[x[q]] = v[q];
+    }
+}
+
 __device__ void play_levels(int64_t n, const uint32_t* __restrict__ ids, uint32_t psz,
                             const uint2* __restrict__ lrec, const uint32_t* __restrict__ lend, uint32_t np,
                             uint32_t depth, const int8_t* __restrict__ sp_in, int8_t* __restrict__ sp_out,
@@ -829,23 +880,11 @@ __device__ void play_levels(int64_t n, const uint32_t* __restrict__ ids, uint32_
     const uint32_t lane = threadIdx.x;
     const uint32_t top = psz - 1;  // 1 <= psz <= kLevelSlots
     uint32_t ends = lane < depth ? lend[lane] : np;  // ends of levels lb .. lb + 63
-    uint32_t x[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) x[q] = lane + 64 * q < psz ? ids[lane + 64 * q] : kUnset;
     RowCursor rc{0u, 0u, 0u, 0u};
     uint32_t ca[kRowBatch], cb[kRowBatch];
     uint2 ra[kRowBatch], rb[kRowBatch];
     next_rows(rc, ends, lrec, lend, np, depth, ca, ra);
-    int8_t v[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) v[q] = x[q] < (uint64_t)n ? sp_in[x[q]] : (int8_t)0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-        if (lane + 64 * q < psz) slots[lane + 64 * q] = v[q];
-    for (uint32_t t0 = 256 + lane; t0 < psz; t0 += 64) {  // more than 256 particles
-        const uint32_t xt = ids[t0];
-        slots[t0] = xt < (uint64_t)n ? sp_in[xt] : (int8_t)0;
-    }
+    stage_in(n, ids, psz, sp_in, slots);
     __builtin_amdgcn_wave_barrier();
     while (ca[0] > 0) {
         next_rows(rc, ends, lrec, lend, np, depth, cb, rb);
@@ -855,13 +894,7 @@ __device__ void play_levels(int64_t n, const uint32_t* __restrict__ ids, uint32_
         play_rows(slots, cb, rb, top);
     }
     __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-        if (x[q] < (uint64_t)n) sp_out[x[q]] = slots[lane + 64 * q];
-    for (uint32_t t0 = 256 + lane; t0 < psz; t0 += 64) {
-        const uint32_t xt = ids[t0];
-        if (xt < (uint64_t)n) sp_out[xt] = slots[t0];
-    }
+    stage_out(n, ids, psz, slots, sp_out);
 }
 
 // Component past the level limits: windows of 64 pairs, each replayed in
@@ -873,13 +906,13 @@ __device__ void play_windows(int64_t n, const uint32_t* __restrict__ ids, uint32
                              int8_t* slots) {
     const uint32_t lane = threadIdx.x;
     const bool in_lds = psz <= kWaveSlots;
-    for (uint32_t t = lane; t < psz; t += 64) {
-        const uint32_t x = ids[t];
-        const int8_t v = x < (uint64_t)n ? sp_in[x] : (int8_t)0;
-        if (in_lds)
-            slots[t] = v;
-        else if (x < (uint64_t)n)
-            sp_out[x] = v;
+    if (in_lds) {
+        stage_in(n, ids, psz, sp_in, slots);
+    } else {
+        for (uint32_t t = lane; t < psz; t += 64) {
+            const uint32_t x = ids[t];
+            if (x < (uint64_t)n) sp_out[x] = sp_in[x];
+        }
     }
     __syncthreads();  // (one wavefront) the copies above are visible to every lane
     uint2 r_next = lane < np ? rec[lane] : make_uint2(0u, 0u);
@@ -925,10 +958,7 @@ __device__ void play_windows(int64_t n, const uint32_t* __restrict__ ids, uint32
     }
     if (in_lds) {
         __builtin_amdgcn_wave_barrier();
-        for (uint32_t t = lane; t < psz; t += 64) {
-            const uint32_t x = ids[t];
-            if (x < (uint64_t)n) sp_out[x] = slots[t];
-        }
+        stage_out(n, ids, psz, slots, sp_out);
     }
 }
 
@@ -939,28 +969,46 @@ __device__ __forceinline__ void play_narrow(int64_t n, const uint32_t* __restric
                                             int8_t* __restrict__ sp_out, int8_t* my) {
     const uint32_t qb = dsc.x, np = min(dsc.y, kNarrowPairs);
     const uint32_t pb = dsc.z, psz = min(dsc.w, kNarrowSlots);
-    uint32_t id[kNarrowSlots];
-#pragma unroll
-    for (uint32_t t = 0; t < kNarrowSlots; ++t) id[t] = t < psz ? ids[pb + t] : kUnset;
+    if (psz == 0) return;
+    // every load is clamped inside the component rather than guarded, so
+    // none waits behind a branch: the species of kStage slots at a time are
+    // gathered together, and the records of the next 8 pairs load while the
+    // current 8 play
     uint2 r[8];
 #pragma unroll
-    for (uint32_t q = 0; q < 8; ++q) r[q] = q < np ? rec[qb + q] : make_uint2(0u, 0u);
+    for (uint32_t q = 0; q < 8; ++q) r[q] = np > 0 ? rec[qb + min(q, np - 1u)] : make_uint2(0u, 0u);
+    for (uint32_t t0 = 0; t0 < psz; t0 += kStage) {
+        uint32_t x[kStage];
+        int8_t v[kStage];
 #pragma unroll
-    for (uint32_t t = 0; t < kNarrowSlots; ++t)
-        if (t < psz) my[t] = id[t] < (uint64_t)n ? sp_in[id[t]] : (int8_t)0;
+        for (int q = 0; q < kStage; ++q) x[q] = ids[pb + min(t0 + q, psz - 1u)];
+#pragma unroll
+        for (int q = 0; q < kStage; ++q) v[q] = sp_in[x[q] < (uint64_t)n ? x[q] : 0u];
+#pragma unroll
+        for (int q = 0; q < kStage; ++q)
+            if (t0 + q < psz) my[t0 + q] = x[q] < (uint64_t)n ? v[q] : (int8_t)0;
+    }
     for (uint32_t q0 = 0; q0 < np; q0 += 8) {
         uint2 rn[8];
 #pragma unroll
-        for (uint32_t q = 0; q < 8; ++q) rn[q] = q0 + 8 + q < np ? rec[qb + q0 + 8 + q] : make_uint2(0u, 0u);
+        for (uint32_t q = 0; q < 8; ++q) rn[q] = rec[qb + min(q0 + 8 + q, np - 1u)];
 #pragma unroll
         for (uint32_t q = 0; q < 8; ++q)
             if (q0 + q < np) play_slots(my, r[q], kNarrowSlots - 1);
 #pragma unroll
         for (uint32_t q = 0; q < 8; ++q) r[q] = rn[q];
     }
+    for (uint32_t t0 = 0; t0 < psz; t0 += kStage) {
+        uint32_t x[kStage];
+        int8_t v[kStage];
 #pragma unroll
-    for (uint32_t t = 0; t < kNarrowSlots; ++t)
-        if (t < psz && id[t] < (uint64_t)n) sp_out[id[t]] = my[t];
+        for (int q = 0; q < kStage; ++q) x[q] = ids[pb + min(t0 + q, psz - 1u)];
+#pragma unroll
+        for (int q = 0; q < kStage; ++q) v[q] = my[min(t0 + q, psz - 1u)];
+#pragma unroll
+        for (int q = 0; q < kStage; ++q)
+            if (t0 + q < psz && x[q] < (uint64_t)n) sp_out[x[q]] = v[q];
+    }
 }
 
 __global__ void __launch_bounds__(64)
