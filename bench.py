@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--window-only", action="store_true",
                     help="run only the timed window once with per-launch events and print its launch counts "
                          "(for counter and trace profiles of the window)")
+    ap.add_argument("--outputs", action="store_true",
+                    help="also time MicrobeSimulator at 490k with and without the output files (writes ~6 GB)")
     ap.add_argument("--fp32-steps", type=int, default=2880)
     ap.add_argument("--cpu-steps", type=int, default=48)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -390,6 +392,49 @@ def ensemble_leg(tune="short"):
             "speedup_vs_runs_alone": round(seq / batched, 3)}
 
 
+def outputs_leg(steps):
+    """MicrobeSimulator on the 490k workload with the output files written
+    (records streamed through the pinned ring to particle_data.nc and
+    microbe_data.nc in a temporary directory) and without: the device run's
+    time both ways, then the time the writer needed after the run and the
+    file sizes.  The files are deleted afterwards."""
+    import shutil
+    import tempfile
+    from datetime import datetime, timedelta
+    from lagrangian_microbes_amd import configs
+    from lagrangian_microbes_amd.interactions import rock_paper_scissors
+    from lagrangian_microbes_amd.simulator import MicrobeSimulator
+
+    w = configs.rps_490k_30day()
+    lon, lat = w.particles()
+    f = w.velocity_field()
+    start, dt = datetime(2017, 1, 1), timedelta(seconds=w.dt)
+    out = {}
+    tmp = tempfile.mkdtemp(prefix="lm_outputs_")
+    try:
+        free = shutil.disk_usage(tmp).free
+        need = 2 * w.n * steps * 9
+        if free < 2 * need:
+            return {"skipped": f"{free / 2**30:.1f} GiB free, {need / 2**30:.1f} GiB of files"}
+        for mode in ("warm-up", "off", "on"):
+            rps = rock_paper_scissors(w.n, w.p, w.p, w.p, seed=w.species_seed)
+            sim = MicrobeSimulator(lon, lat, f, rps, w.radius, output_dir=tmp, output_chunk_iters=100,
+                                   philox_key=w.key)
+            t0 = time.perf_counter()
+            sim.time_step(start, start + steps * dt, dt, write_outputs=(mode == "on"))
+            wall = time.perf_counter() - t0
+            if mode != "warm-up":
+                out[mode] = {**{k: (round(v, 4) if isinstance(v, float) else v) for k, v in sim.timings.items()},
+                             "call_s": round(wall, 3)}
+                log(f"[bench] outputs {mode}: run {sim.timings['run_s']:.3f}s, call {wall:.3f}s")
+        sizes = {name: os.path.getsize(os.path.join(tmp, name)) for name in ("microbe_data.nc", "particle_data.nc")}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return {"workload": w.name, "particles": w.n, "steps": steps, "file_bytes": sizes,
+            "run_s_without_outputs": out["off"]["run_s"], "run_s_with_outputs": out["on"]["run_s"],
+            "run_slowdown": round(out["on"]["run_s"] / out["off"]["run_s"] - 1.0, 4), "timings": out}
+
+
 def adjoint_leg(reps):
     """The adjoint workload (1M particles, 240 steps, 20 float32 levels): times the
     differentiable forward and loss.backward() with events on the stream they run on,
@@ -571,6 +616,8 @@ def main():
         result["fp32_path"] = fp32_leg(args.fp32_steps)
     if not args.no_ensemble:
         result["ensemble"] = ensemble_leg(args.ensemble_tune)
+    if args.outputs:
+        result["outputs"] = outputs_leg(w.steps)
     if not args.no_adjoint:
         result["adjoint"] = adjoint_leg(args.adjoint_reps)
         result["adjoint_forward_ratio"] = result["adjoint"]["ratio"]

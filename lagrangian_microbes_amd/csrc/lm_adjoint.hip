@@ -224,21 +224,51 @@ __device__ __forceinline__ void level_corners(const LevelView& V, const AdjStage
                             fma(st.w, (double)b[c].y - (double)a[c].y, (double)a[c].y));
 }
 
-// adj_sample_fast at stage st plus the linearisation; false outside.
-__device__ __forceinline__ bool adj_linearise(const LevelView& V, const AdjStage& st, double x, double y, double* ud,
-                                              double* vd, Lin* L) {
-    int i, j;
-    double xi, eta;
-    if (!adj_locate(V.g, x, y, &i, &j, &xi, &eta)) return false;
-    double2 q[4];
-    level_corners(V, st, i, j, q);
+// cos and sin of a step's start latitude y0 (degrees), from which the
+// stage latitudes' are taken by angle addition: for y = y0 + e degrees,
+// d = e pi / 180, cos y = cos y0 cos d - sin y0 sin d and sin y = sin y0 cos d
+// + cos y0 sin d, with cos d = 1 - d^2/2 + d^4/24 and sin d = d (1 - d^2/6 +
+// d^4/120), whose truncation (below d^6/720 and d^7/5040) is under 1e-19 for
+// |d| <= 2^-9 rad (0.11 degrees, farther than a stage moves in one step at
+// ocean speeds); a stage farther away takes the full polynomials.  The
+// results agree with cos_deg / sin_deg to rounding: the reverse sweep's
+// trajectory, like the recompute's, only positions the gradient.
+struct Trig0 {
+    double y0, c0, s0;
+};
+
+__device__ __forceinline__ Trig0 trig_at(double y0) { return Trig0{y0, cos_deg(y0), sin_deg(y0)}; }
+
+__device__ __forceinline__ void trig_near(const Trig0& T, double y, double* c, double* sn) {
+    const double d = (y - T.y0) * LM_DEG2RAD;
+    if (fabs(d) <= 0x1p-9) {
+        const double d2 = d * d;
+        const double cd = fma(fma(d2, 1.0 / 24.0, -0.5), d2, 1.0);
+        const double sd = fma(fma(d2, 1.0 / 120.0, -1.0 / 6.0), d2, 1.0) * d;
+        *c = fma(T.c0, cd, -(T.s0 * sd));
+        *sn = fma(T.s0, cd, T.c0 * sd);
+    } else {
+        *c = cos_deg(y);
+        *sn = sin_deg(y);
+    }
+}
+
+// adj_sample_fast at stage st plus the linearisation, from the cell's
+// blended corners q (level_corners at stage st of the cell (i, j) that
+// adj_locate gave for (x, y), with fractional position (xi, eta)).  T: the
+// trigonometry of the step's start latitude.
+__device__ __forceinline__ void adj_linearise_at(const LevelView& V, const Trig0& T, const double2 (&q)[4], int i,
+                                                 int j, double xi, double eta, double y, double* ud, double* vd,
+                                                 Lin* L) {
     const double ub = fma(xi, q[1].x - q[0].x, q[0].x);
     const double ut = fma(xi, q[3].x - q[2].x, q[2].x);
     const double u = fma(eta, ut - ub, ub);
     const double vb = fma(xi, q[1].y - q[0].y, q[0].y);
     const double vt = fma(xi, q[3].y - q[2].y, q[2].y);
     const double v = fma(eta, vt - vb, vb);
-    const double rc = recip(cos_deg(y));
+    double cy, sy;
+    trig_near(T, y, &cy, &sy);
+    const double rc = recip(cy);
     const double kc = LM_INV_M_PER_DEG * rc;
     *ud = u * kc;
     *vd = v * LM_INV_M_PER_DEG;
@@ -248,13 +278,24 @@ __device__ __forceinline__ bool adj_linearise(const LevelView& V, const AdjStage
     const double dv_dy = (vt - vb) * V.g.inv_dy;
     // d(1/cos(lat))/dlat = (pi/180) sin(lat) / cos(lat)^2
     L->dux = kc * du_dx;
-    L->duy = kc * fma(u * LM_DEG2RAD * rc, sin_deg(y), du_dy);
+    L->duy = kc * fma(u * LM_DEG2RAD * rc, sy, du_dy);
     L->dvx = LM_INV_M_PER_DEG * dv_dx;
     L->dvy = LM_INV_M_PER_DEG * dv_dy;
     L->xi = (float)xi;
     L->eta = (float)eta;
     L->kc = (float)kc;
     L->cell = (j << 16) | i;
+}
+
+// The same from the position; false outside.
+__device__ __forceinline__ bool adj_linearise(const LevelView& V, const AdjStage& st, const Trig0& T, double x,
+                                              double y, double* ud, double* vd, Lin* L) {
+    int i, j;
+    double xi, eta;
+    if (!adj_locate(V.g, x, y, &i, &j, &xi, &eta)) return false;
+    double2 q[4];
+    level_corners(V, st, i, j, q);
+    adj_linearise_at(V, T, q, i, j, xi, eta, y, ud, vd, L);
     return true;
 }
 
@@ -479,10 +520,11 @@ adj_backward_kernel(int64_t n, LevelView V, int nlev, int s0, int m, double hh, 
                 sc{2 * s + 2, V.lev0[2 * s + 2], V.w[2 * s + 2]};
             Lin L1, L2, L3, L4;
             double ud, vd;
-            if (!adj_linearise(V, sa, x0, y0, &ud, &vd, &L1)) continue;
-            if (!adj_linearise(V, sb, fma(ud, hh, x0), fma(vd, hh, y0), &ud, &vd, &L2)) continue;
-            if (!adj_linearise(V, sb, fma(ud, hh, x0), fma(vd, hh, y0), &ud, &vd, &L3)) continue;
-            if (!adj_linearise(V, sc, fma(ud, h, x0), fma(vd, h, y0), &ud, &vd, &L4)) continue;
+            const Trig0 T = trig_at(y0);
+            if (!adj_linearise(V, sa, T, x0, y0, &ud, &vd, &L1)) continue;
+            if (!adj_linearise(V, sb, T, fma(ud, hh, x0), fma(vd, hh, y0), &ud, &vd, &L2)) continue;
+            if (!adj_linearise(V, sb, T, fma(ud, hh, x0), fma(vd, hh, y0), &ud, &vd, &L3)) continue;
+            if (!adj_linearise(V, sc, T, fma(ud, h, x0), fma(vd, h, y0), &ud, &vd, &L4)) continue;
             const double a4x = h6 * lx, a4y = h6 * ly;
             double a3x = 2.0 * a4x, a3y = 2.0 * a4y;
             double a2x = a3x, a2y = a3y;

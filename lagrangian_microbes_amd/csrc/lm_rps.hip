@@ -307,6 +307,17 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
     const int64_t v0 = (int64_t)bx * kBlock * kKeyItems + threadIdx.x;
     uint32_t x[kKeyItems], q[kKeyItems];
     bool t[kKeyItems], act[kKeyItems];
+    // every item's loads are issued together: indices clamped into the step
+    // (n >= 1), no branch between them
+    uint32_t pv[kKeyItems], ry[kKeyItems], tv[kKeyItems];
+#pragma unroll
+    for (int k = 0; k < kKeyItems; ++k) {
+        const int64_t v = v0 + (int64_t)k * kBlock;
+        const int64_t vc = v < n ? v : n - 1;
+        pv[k] = A.parent[ro + vc];
+        ry[k] = A.rows[ro + vc].y;
+        tv[k] = A.touched[ro + vc];
+    }
 #pragma unroll
     for (int k = 0; k < kKeyItems; ++k) {
         const int64_t v = v0 + (int64_t)k * kBlock;
@@ -314,8 +325,8 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
         // a particle has pairs if it has a row (as some pair's i) or a parent
         // below it (as some pair's j: hooked, and parents only move down);
         // one without pairs is keyed by itself: a run of one, never listed
-        const uint32_t p = live ? A.parent[ro + v] : 0u;
-        t[k] = live && ((p != (uint32_t)v) | (A.rows[ro + v].y != 0u) | (A.touched[ro + v] != 0));
+        const uint32_t p = live ? pv[k] : 0u;
+        t[k] = live && ((p != (uint32_t)v) | (ry[k] != 0u) | (tv[k] != 0u));
         x[k] = (uint32_t)v;
         q[k] = p;
         act[k] = t[k] && p != (uint32_t)v;
@@ -329,9 +340,11 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
     // end), all of a thread's at once: the trees are shallow after the hook
     // pass and the union-find, and path halving here would add scattered stores
     for (;;) {
+        // q[k] is always a particle index (a parent, or 0 past the end), so
+        // the walks' loads go out unconditionally, all at once
         uint32_t nq[kKeyItems];
 #pragma unroll
-        for (int k = 0; k < kKeyItems; ++k) nq[k] = act[k] ? A.parent[ro + q[k]] : 0u;
+        for (int k = 0; k < kKeyItems; ++k) nq[k] = A.parent[ro + q[k]];
         bool more = false;
 #pragma unroll
         for (int k = 0; k < kKeyItems; ++k) {
@@ -626,9 +639,14 @@ rps_deps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
         const uint4 dsc = A.wide[ro + c];
         const uint32_t qb = dsc.x, np = dsc.y, psz = dsc.w;
         const bool table = psz <= kDepsTable;  // uniform over the wavefront
+        if (np == 0) continue;
+        // the next window's records load while this one is processed
+        uint2 r_next = rec[qb + min(lane, np - 1u)];
         for (uint32_t w = 0; w < np; w += 64) {
             const bool live = w + lane < np;
-            const uint2 r = live ? rec[qb + w + lane] : make_uint2(kUnset, kUnset - 1u);
+            const uint2 r_cur = r_next;
+            r_next = rec[qb + min(w + 64u + lane, np - 1u)];
+            const uint2 r = live ? r_cur : make_uint2(kUnset, kUnset - 1u);
             const uint32_t a = live ? (r.x & kSlotMask) : kUnset, b = live ? r.y : kUnset - 1u;
             unsigned long long d = 0ull;
             if (table) {
@@ -695,10 +713,18 @@ rps_levels_kernel(int64_t n, RpsArrays A, RpsStrides st) {
         for (uint32_t t = lane; t < psz; t += 64) last[t] = 0;
         __syncthreads();
         uint32_t dmax = 0;
+        // the next window's records and masks load while this one resolves
+        // (np >= 1: a listed component has pairs)
+        uint2 r_next = rec[qb + min(lane, np - 1u)];
+        unsigned long long d_next = deps[qb + min(lane, np - 1u)];
         for (uint32_t w = 0; w < np; w += 64) {
             const bool live = w + lane < np;
-            const uint2 r = live ? rec[qb + w + lane] : make_uint2(0u, 0u);
-            const unsigned long long dep = live ? deps[qb + w + lane] : 0ull;
+            const uint2 r_cur = r_next;
+            const unsigned long long d_cur = d_next;
+            r_next = rec[qb + min(w + 64u + lane, np - 1u)];
+            d_next = deps[qb + min(w + 64u + lane, np - 1u)];
+            const uint2 r = live ? r_cur : make_uint2(0u, 0u);
+            const unsigned long long dep = live ? d_cur : 0ull;
             const uint32_t a = min(r.x & kSlotMask, psz - 1), b = min(r.y, psz - 1);
             const unsigned long long all = __ballot(live);
             unsigned long long done = 0ull;
@@ -743,10 +769,24 @@ rps_levels_kernel(int64_t n, RpsArrays A, RpsStrides st) {
             carry += __shfl(inc, 63, 64);
         }
         __syncthreads();
-        for (uint32_t q = lane; q < np; q += 64) {
-            const uint32_t l = lvl[q] - 1u;
-            const uint32_t pos = half_of(atomicAdd(&words[l >> 1], 1u << (16u * (l & 1u))), l);
-            if (pos < np) A.lrec[po + qb + pos] = rec[qb + q];
+        // records into level order, four per lane at once (their loads in flight together)
+        for (uint32_t q0 = lane; q0 < np; q0 += 256) {
+            uint2 rv[4];
+            uint32_t pos[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) rv[u] = rec[qb + min(q0 + 64u * u, np - 1u)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t q = q0 + 64u * u;
+                pos[u] = kUnset;
+                if (q < np) {
+                    const uint32_t l = lvl[q] - 1u;
+                    pos[u] = half_of(atomicAdd(&words[l >> 1], 1u << (16u * (l & 1u))), l);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (pos[u] < np) A.lrec[po + qb + pos[u]] = rv[u];
         }
         if (lane == 0) A.depth[ro + c] = dmax;
         __syncthreads();
@@ -1246,6 +1286,29 @@ void lm_rps_ws_destroy(lm_rps_ws* ws) { rps_ws_free(ws); }
 
 int64_t lm_rps_ws_bytes(const lm_rps_ws* ws) { return ws ? ws->bytes : 0; }
 
+// The tail of a prepare, once the particles are grouped by component: row
+// offsets in grouped order, the component lists, the pair records at their
+// run positions, the wide components' dependency masks and levels.
+static int rps_prepare_tail(lm_rps_ws* ws, int64_t n, unsigned S, const int32_t* pair_i, const int32_t* pair_j,
+                            int64_t pair_stride, hipStream_t s) {
+    const lm::RpsStrides sp{ws->row, pair_stride};
+    lm::RpsArrays& A = ws->A;
+    const unsigned nbp = lm::grid_for(n);
+    const unsigned npb = lm::grid_for(pair_stride < n ? pair_stride : n);
+    int rc = lm::exclusive_scan_u32_batched(A.rcnt, n, ws->row, A.roff, ws->row, (int32_t)S, ws->scan_tmp, nullptr, s);
+    if (rc) return rc;
+    LM_LAUNCH(lm::K_RPS_COMPS, lm::rps_comps_kernel, dim3((nbp + lm::kCompItems - 1) / lm::kCompItems, S),
+              dim3(lm::kBlock), 0, s, n, A, sp);
+    LM_LAUNCH_CHECK();
+    LM_LAUNCH(lm::K_RPS_FILL, lm::rps_fill_kernel, dim3(npb, S), dim3(lm::kBlock), 0, s, n, pair_i, pair_j, A, sp);
+    LM_LAUNCH_CHECK();
+    LM_LAUNCH(lm::K_RPS_DEPS, lm::rps_deps_kernel, dim3(1024, S), dim3(64), 0, s, n, A, sp);
+    LM_LAUNCH_CHECK();
+    LM_LAUNCH(lm::K_RPS_LEVELS, lm::rps_levels_kernel, dim3(1024, S), dim3(64), 0, s, n, A, sp);
+    LM_LAUNCH_CHECK();
+    return LM_OK;
+}
+
 static int rps_prepare_impl(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int32_t* pair_i, const int32_t* pair_j,
                             int64_t pair_stride, const int64_t* npairs, double pRS, double pPR, double pSP,
                             uint64_t key0, uint64_t key1, uint64_t step0, const uint16_t* run, int32_t nruns,
@@ -1343,23 +1406,13 @@ static int rps_prepare_impl(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int3
         LM_LAUNCH_CHECK();
         A.spk = pk[1 - in];
         A.ids = pv[1 - in];
-        if ((rc = lm::exclusive_scan_u32_batched(A.rcnt, n, ws->row, A.roff, ws->row, nsteps, ws->scan_tmp, nullptr,
-                                                 s)))
-            return rc;
-        LM_LAUNCH(lm::K_RPS_COMPS, lm::rps_comps_kernel, dim3((nbp + lm::kCompItems - 1) / lm::kCompItems, S),
-                  dim3(lm::kBlock), 0, s, n, A, sp);
-        LM_LAUNCH_CHECK();
-        LM_LAUNCH(lm::K_RPS_FILL, lm::rps_fill_kernel, dim3(npb, S), dim3(lm::kBlock), 0, s, n, pair_i,
-                  pair_j, A, sp);
-        LM_LAUNCH_CHECK();
-        LM_LAUNCH(lm::K_RPS_DEPS, lm::rps_deps_kernel, dim3(1024, S), dim3(64), 0, s, n, A, sp);
-        LM_LAUNCH_CHECK();
-        LM_LAUNCH(lm::K_RPS_LEVELS, lm::rps_levels_kernel, dim3(1024, S), dim3(64), 0, s, n, A, sp);
-        LM_LAUNCH_CHECK();
+        if ((rc = rps_prepare_tail(ws, n, S, pair_i, pair_j, pair_stride, s))) return rc;
         ws->pair_stride_used = pair_stride;
-    } else {
-        ws->pair_stride_used = 0;
+        ws->last_n = n;
+        ws->last_steps = nsteps;
+        return LM_OK;
     }
+    ws->pair_stride_used = 0;
     ws->last_n = n;
     ws->last_steps = nsteps;
     return LM_OK;

@@ -198,7 +198,7 @@ class ParticleDataset:
     2 or 5, or None for CDF-2 when every variable is under 2 GiB, else CDF-5.
     """
 
-    def __init__(self, path, n_particles, times, names, version=None, threads=4):
+    def __init__(self, path, n_particles, times, names, version=None, threads=8):
         n, T = int(n_particles), len(times)
         per_step = {k: np.dtype(v) for k, v in names.items()}
         biggest = max([n * T * dt.itemsize for dt in per_step.values()] + [8 * max(n, T)])
@@ -236,27 +236,34 @@ class ParticleDataset:
 
     def write_steps(self, name, t0, block):
         """block: [k, n] (step-major, native byte order) -> columns t0..t0+k-1
-        of variable `name`.  Particle tiles go to worker threads (numpy's
-        strided copies release the GIL)."""
-        block = np.asarray(block)
-        k = block.shape[0]
-        if block.ndim != 2 or block.shape[1] != self.n:
-            raise ValueError(f"block must be [k, {self.n}] (got {block.shape})")
-        dst = self.vars[name]
-        if t0 < 0 or t0 + k > dst.shape[1]:
-            raise ValueError(f"steps {t0}..{t0 + k} outside the {dst.shape[1]}-step time axis")
-        tile = 8192
+        of variable `name`."""
+        self.write_block(t0, {name: block})
 
-        def part(i0):
-            i1 = min(self.n, i0 + tile)
+    def write_block(self, t0, blocks):
+        """{name: [k, n] block} -> columns t0..t0+k-1 of each variable.  The
+        (variable, particle tile) pieces go to the worker threads together
+        (numpy's strided copies release the GIL)."""
+        jobs = []
+        tile = 8192
+        for name, block in blocks.items():
+            block = np.asarray(block)
+            k = block.shape[0]
+            if block.ndim != 2 or block.shape[1] != self.n:
+                raise ValueError(f"block must be [k, {self.n}] (got {block.shape})")
+            dst = self.vars[name]
+            if t0 < 0 or t0 + k > dst.shape[1]:
+                raise ValueError(f"steps {t0}..{t0 + k} outside the {dst.shape[1]}-step time axis")
+            jobs += [(dst, block, i0, min(self.n, i0 + tile), k) for i0 in range(0, self.n, tile)]
+
+        def part(job):
+            dst, block, i0, i1, k = job
             dst[i0:i1, t0:t0 + k] = block[:, i0:i1].T
 
-        starts = range(0, self.n, tile)
         if self._pool is None:
-            for i0 in starts:
-                part(i0)
+            for job in jobs:
+                part(job)
         else:
-            list(self._pool.map(part, starts))
+            list(self._pool.map(part, jobs))
 
     def close(self):
         if self._pool is not None:

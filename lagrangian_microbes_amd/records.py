@@ -14,10 +14,11 @@ without stopping that run:
 - a writer thread waits for each copy and hands the slot to a sink (the
   NetCDF writer), then frees the slot.
 
-The ring is bounded (``ring_bytes``, at least two slots): the run only waits
-when every slot still waits for the writer, so a run whose records fit the
-ring never waits, and a larger one proceeds at the writer's pace instead of
-holding [N, steps] arrays in host memory.
+The ring is bounded (``ring_bytes``, at least two slots, one per chunk of
+the run at most): the run only waits when every slot still waits for the
+writer, so a run whose records fit the ring never waits, and a larger one
+proceeds at the writer's pace instead of holding [N, steps] arrays in host
+memory.
 """
 import queue
 import threading
@@ -25,7 +26,7 @@ import time
 
 import torch
 
-DEFAULT_RING_BYTES = 4 << 30
+DEFAULT_RING_BYTES = 8 << 30
 
 
 class RecordStream:
@@ -33,9 +34,9 @@ class RecordStream:
     e.g. {"longitude": (torch.float32, (n,)), "npairs": (torch.int64, ())};
     chunk: the most steps one submit carries; sink(t0, k, arrays) runs on the
     writer thread with host numpy views [k, ...] valid during the call;
-    steps: the run's length (no more slots than its chunks are allocated)."""
+    chunks: how many submits the run makes (no more slots are allocated)."""
 
-    def __init__(self, fields, chunk, sink, device=None, steps=None, ring_bytes=DEFAULT_RING_BYTES):
+    def __init__(self, fields, chunk, sink, device=None, chunks=None, ring_bytes=DEFAULT_RING_BYTES):
         self.device = torch.device(device or "cuda")
         self.fields = dict(fields)
         self.chunk = int(chunk)
@@ -43,8 +44,8 @@ class RecordStream:
         per_step = sum(torch.empty((), dtype=dt).element_size() * _numel(shape) for dt, shape in self.fields.values())
         slot_bytes = max(1, per_step * self.chunk)
         nslots = max(2, int(ring_bytes) // slot_bytes)
-        if steps is not None:
-            nslots = max(1, min(nslots, -(-int(steps) // self.chunk)))
+        if chunks is not None:
+            nslots = max(1, min(nslots, int(chunks)))
         self.slots = [{name: torch.empty((self.chunk,) + tuple(shape), dtype=dt, pin_memory=True)
                        for name, (dt, shape) in self.fields.items()} for _ in range(nslots)]
         self.copy = torch.cuda.Stream(self.device)

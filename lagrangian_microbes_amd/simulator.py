@@ -296,10 +296,10 @@ class MicrobeSimulator:
         def sink(t0, k, arrays):
             self.pairs_per_step[t0:t0 + k] = arrays["npairs"]
             if ds is not None:
-                for name in ("longitude", "latitude", "species"):
-                    ds.write_steps(name, t0, arrays[name])
+                ds.write_block(t0, {name: arrays[name] for name in ("longitude", "latitude", "species")})
 
-        stream = RecordStream(fields, self.chunk, sink, self.device, steps=iters, ring_bytes=self.ring_bytes)
+        stream = RecordStream(fields, self.chunk, sink, self.device, chunks=len(chunk_schedule(iters, self.chunk)),
+                              ring_bytes=self.ring_bytes)
         t_setup = time() - tic
         done = [0]
 
