@@ -64,8 +64,10 @@ class DeviceField:
         if self._stage_buf is None or self._stage_buf.numel() < need:
             self._stage_buf = torch.empty(need, dtype=self.dtype, device=self.device)
         out = self._stage_buf[:need]
-        lev0_d = torch.from_numpy(lev0.astype(np.int32)).to(self.device)
-        w_d = torch.from_numpy(w.astype(np.float64)).to(self.device)
+        # pinned, non-blocking uploads: a synchronous one would hold the host
+        # until this stream drains, and with it a caller's launch loop
+        lev0_d = torch.from_numpy(lev0.astype(np.int32)).pin_memory().to(self.device, non_blocking=True)
+        w_d = torch.from_numpy(w.astype(np.float64)).pin_memory().to(self.device, non_blocking=True)
         blend = field_blend_f64 if self.precision == "f64" else field_blend_f32
         blend(self.levels, self.level_len, lev0_d, w_d, out, lev0_host=lev0)
         return out, self.level_len

@@ -180,9 +180,13 @@ class FusedRunner:
                 ev.record(side)
                 prepared[c] = ev
 
-        for c in range(min(2, len(sizes))):
-            advect(c)
+        # chunk 0's preparation goes out before chunk 1's advection: it waits
+        # for nothing else
+        advect(0)
         group(0)
+        if len(sizes) > 1:
+            advect(1)
+        backup, worst_host = self._backup, self._worst_host
         for c, k in enumerate(sizes):
             b = c % nsets
             if c + 1 < len(sizes):
@@ -190,18 +194,31 @@ class FusedRunner:
             if c + 2 < len(sizes):
                 advect(c + 2)
             main.wait_event(prepared[c])
-            worst = eng.worst_count(b, k)
-            if worst > eng.capacity:
-                # larger pair buffers for every set, then the prepared chunks
-                # are grouped again from their records (positions are kept)
+            rs = self._rec_sets[b][2][:k]
+            # The replay goes out before the host has seen the chunk's pair
+            # counts, so it starts as soon as the preparation ends; the counts
+            # come back (a pinned copy) while it runs.  A step that overflowed
+            # the pair buffers voids the replay: the chunk is then grouped
+            # again with larger buffers (positions are kept; so is the next
+            # prepared chunk, whose buffers are replaced) and replayed from its
+            # starting species.
+            backup.copy_(self.species)
+            worst_host.copy_(eng.npairs_sets[b][:k].max(), non_blocking=True)
+            counted = torch.cuda.Event()
+            counted.record(main)
+            eng.play_set(b, k, self.species, rs)
+            counted.synchronize()
+            worst = int(worst_host.item())
+            while worst > eng.capacity:
                 torch.cuda.synchronize(self.device)
                 eng.fits(worst)
                 group(c)
                 if c + 1 < len(sizes):
                     group(c + 1)
                 main.wait_event(prepared[c])
-            rs = self._rec_sets[b][2][:k]
-            eng.play_set(b, k, self.species, rs)
+                self.species.copy_(backup)
+                eng.play_set(b, k, self.species, rs)
+                worst = eng.worst_count(b, k)
             ret = None
             if on_chunk is not None:
                 ret = on_chunk(k, self._rec_sets[b][0][:k], self._rec_sets[b][1][:k], rs, eng.npairs_sets[b][:k])
@@ -222,6 +239,8 @@ class FusedRunner:
             self._rec_sets = [(self.rec_x, self.rec_y, self.rec_s)] + extra
             self._side = torch.cuda.Stream(self.device)
             self._adv = torch.cuda.Stream(self.device)
+            self._backup = torch.empty_like(self.species)  # a chunk's starting species
+            self._worst_host = torch.zeros((), dtype=torch.int64, pin_memory=True)
 
     def check_domain(self):
         n_oob = int(self.oob.item())
