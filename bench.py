@@ -393,11 +393,12 @@ def ensemble_leg(tune="short"):
 
 
 def outputs_leg(steps):
-    """MicrobeSimulator on the 490k workload with the output files written
-    (records streamed through the pinned ring to particle_data.nc and
-    microbe_data.nc in a temporary directory) and without: the device run's
-    time both ways, then the time the writer needed after the run and the
-    file sizes.  The files are deleted afterwards."""
+    """MicrobeSimulator on the 490k workload without output files, with them
+    (particle_data.nc and microbe_data.nc in a temporary directory; the
+    records stay in HBM during the run and are written after it) and with
+    them streamed through the pinned host ring during the run instead: the
+    device run's time each way, the output stages' times and the file
+    sizes.  The files are deleted afterwards."""
     import shutil
     import tempfile
     from datetime import datetime, timedelta
@@ -416,23 +417,28 @@ def outputs_leg(steps):
         need = 2 * w.n * steps * 9
         if free < 2 * need:
             return {"skipped": f"{free / 2**30:.1f} GiB free, {need / 2**30:.1f} GiB of files"}
-        for mode in ("warm-up", "off", "on"):
+        for mode in ("warm-up", "off", "on", "on-streamed"):
             rps = rock_paper_scissors(w.n, w.p, w.p, w.p, seed=w.species_seed)
+            kw = {"device_record_bytes": 0} if mode == "on-streamed" else {}
             sim = MicrobeSimulator(lon, lat, f, rps, w.radius, output_dir=tmp, output_chunk_iters=100,
-                                   philox_key=w.key)
+                                   philox_key=w.key, **kw)
             t0 = time.perf_counter()
-            sim.time_step(start, start + steps * dt, dt, write_outputs=(mode == "on"))
+            sim.time_step(start, start + steps * dt, dt, write_outputs=mode.startswith("on"))
             wall = time.perf_counter() - t0
             if mode != "warm-up":
                 out[mode] = {**{k: (round(v, 4) if isinstance(v, float) else v) for k, v in sim.timings.items()},
                              "call_s": round(wall, 3)}
                 log(f"[bench] outputs {mode}: run {sim.timings['run_s']:.3f}s, call {wall:.3f}s")
-        sizes = {name: os.path.getsize(os.path.join(tmp, name)) for name in ("microbe_data.nc", "particle_data.nc")}
+            if mode == "on":
+                sizes = {name: os.path.getsize(os.path.join(tmp, name))
+                         for name in ("microbe_data.nc", "particle_data.nc")}
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     return {"workload": w.name, "particles": w.n, "steps": steps, "file_bytes": sizes,
             "run_s_without_outputs": out["off"]["run_s"], "run_s_with_outputs": out["on"]["run_s"],
-            "run_slowdown": round(out["on"]["run_s"] / out["off"]["run_s"] - 1.0, 4), "timings": out}
+            "run_slowdown": round(out["on"]["run_s"] / out["off"]["run_s"] - 1.0, 4),
+            "run_slowdown_streamed": round(out["on-streamed"]["run_s"] / out["off"]["run_s"] - 1.0, 4),
+            "timings": out}
 
 
 def adjoint_leg(reps):

@@ -18,10 +18,12 @@ time coordinate as "<unit> since <first time>" counts in the coarsest exact
 unit, int32 in CDF-2 (NetCDF-3 has no 64-bit integers) and int64 in CDF-5,
 like the particle numbers.
 
-Files are written in place through memory maps: ``ParticleDataset`` lays
-out the header and the fixed-size variables, preallocates the file, and
-``write_steps`` stores a block of steps ([k, N], one row per step, as the
-device records them) into the variable's columns.  The format grammar
+Files are written in place: ``ParticleDataset`` lays out the header and the
+fixed-size variables and preallocates the file; ``write_steps`` stores a
+block of steps ([k, N], one row per step, as the device records them) into
+the variable's columns through a memory map, and ``write_rows`` a block of
+whole particles (rows of the variable, one contiguous span) with pwrite.
+The format grammar
 followed is the NetCDF classic/64-bit-offset/64-bit-data format
 specification (magic "CDF" + version byte, numrecs, tagged dimension,
 attribute and variable lists, big-endian values padded to 4 bytes).
@@ -265,13 +267,36 @@ class ParticleDataset:
         else:
             list(self._pool.map(part, jobs))
 
+    def write_rows(self, name, p0, rows):
+        """rows: [m, T] values (particle-major; converted to the file's
+        big-endian type) or their raw bytes already in the file's byte order
+        (a uint8 array) -> rows p0..p0+m-1 of variable `name`: one contiguous
+        span of the file, written with pwrite (no memory map; the GIL is
+        released while it runs)."""
+        dst = self.vars[name]
+        T, es = dst.shape[1], dst.dtype.itemsize
+        arr = np.asarray(rows)
+        if arr.dtype != np.uint8:
+            arr = np.ascontiguousarray(arr, dtype=dst.dtype)
+        data = memoryview(np.ascontiguousarray(arr)).cast("B")
+        if T == 0 or len(data) % (T * es) or p0 < 0 or p0 + len(data) // (T * es) > self.n:
+            raise ValueError(f"{len(data)} bytes are not whole rows of {name} from particle {p0}")
+        begin, _ = self.layout[name]
+        off, done = begin + p0 * T * es, 0
+        while done < len(data):
+            got = os.pwrite(self._fh.fileno(), data[done:done + (1 << 30)], off + done)
+            if got <= 0:
+                raise OSError(f"pwrite stopped with {len(data) - done} bytes left")
+            done += got
+
     def close(self):
         if self._pool is not None:
             self._pool.shutdown()
             self._pool = None
         self.vars = {}
         if self._mm is not None:
-            self._mm.flush()
+            # no msync: the written pages are the file's page cache, which
+            # later reads see and the kernel writes back (as after write())
             try:
                 self._mm.close()
             except BufferError:  # a caller still holds a view: the map closes with it

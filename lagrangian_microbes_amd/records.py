@@ -1,5 +1,15 @@
 """Step records from the device to the host, off the compute path.
 
+Two ways, chosen by the run's size:
+
+- ``DeviceRecords``: a run whose records fit a device-memory budget keeps
+  them in HBM ([T, n] arrays the chunks write in place), so nothing leaves
+  the device while it runs; ``write_device_records`` then transposes each
+  variable on the device into the file's particle-major rows and streams
+  them out in large sequential writes;
+- ``RecordStream``: larger runs move each chunk's records to the host while
+  the run goes on, as below.
+
 The reference collects every step's positions and species on the host
 (particle_advecter.py:226-235,246-249; interaction_simulator.py:108-110) and
 writes them once at the end (particle_advecter.py:262-303,
@@ -20,6 +30,7 @@ writer, so a run whose records fit the ring never waits, and a larger one
 proceeds at the writer's pace instead of holding [N, steps] arrays in host
 memory.
 """
+import concurrent.futures
 import queue
 import threading
 import time
@@ -27,6 +38,99 @@ import time
 import torch
 
 DEFAULT_RING_BYTES = 8 << 30
+DEFAULT_DEVICE_RECORD_BYTES = 64 << 30  # at most this much HBM (and a quarter of the device) holds a run's records
+BLOCK_BYTES = 256 << 20                 # pinned host blocks of write_device_records
+
+
+def device_record_budget(device, limit=None):
+    """Bytes of records a run may keep on `device`: `limit` (default
+    DEFAULT_DEVICE_RECORD_BYTES), at most a quarter of the device's memory
+    and half of what is free now."""
+    free, total = torch.cuda.mem_get_info(torch.device(device))
+    cap = DEFAULT_DEVICE_RECORD_BYTES if limit is None else int(limit)
+    return max(0, min(cap, total // 4, free // 2))
+
+
+class DeviceRecords:
+    """A run's step records resident on the device: longitude, latitude
+    (float32) and species (int8) as [T, n] arrays (row t = step t, as the
+    chunks write them) and the pair count of every step."""
+
+    BYTES_PER_PARTICLE_STEP = 9
+
+    def __init__(self, n, T, device):
+        self.device = torch.device(device)
+        self.n, self.T = int(n), int(T)
+        self.longitude = torch.empty((self.T, self.n), dtype=torch.float32, device=self.device)
+        self.latitude = torch.empty((self.T, self.n), dtype=torch.float32, device=self.device)
+        self.species = torch.empty((self.T, self.n), dtype=torch.int8, device=self.device)
+        self.npairs = torch.zeros(self.T, dtype=torch.int64, device=self.device)
+
+    @classmethod
+    def nbytes(cls, n, T):
+        return cls.BYTES_PER_PARTICLE_STEP * int(n) * int(T) + 8 * int(T)
+
+    @property
+    def arrays(self):
+        """(rec_x, rec_y, rec_s) for FusedRunner.run(records=...)."""
+        return self.longitude, self.latitude, self.species
+
+
+def write_device_records(arrays, files, block_bytes=None, threads=4):
+    """Writes device arrays {name: [T, n] tensor (row t = step t)} into the
+    particle-major variables of open ParticleDatasets: files = [(dataset,
+    names it holds)].  Each array is cut into blocks of particles; a block
+    is transposed on the device to [m, T] (the file's row order) and its
+    values' bytes put in the file's big-endian order, then copied into
+    one of two pinned host buffers and written with pwrite to every file
+    that holds the variable, on worker threads, so the copy of the next
+    block overlaps the writes of this one.  Returns the seconds spent
+    waiting for copies and for writes.  block_bytes: size of each pinned
+    buffer (default BLOCK_BYTES)."""
+    block_bytes = BLOCK_BYTES if block_bytes is None else int(block_bytes)
+    if not arrays:
+        return 0.0, 0.0
+    device = next(iter(arrays.values())).device
+    bufs = [torch.empty(int(block_bytes), dtype=torch.uint8, pin_memory=True) for _ in range(2)]
+    pending = [[], []]
+    copy = torch.cuda.Stream(device)
+    copy.wait_stream(torch.cuda.current_stream(device))  # the arrays as the caller left them
+    wait_copy = wait_write = 0.0
+    j = 0
+    with concurrent.futures.ThreadPoolExecutor(threads) as pool:
+        for name, X in arrays.items():
+            T, n = X.shape
+            es = X.element_size()
+            targets = [ds for ds, names in files if name in names]
+            if T == 0 or n == 0 or not targets:
+                continue
+            m = max(1, int(block_bytes) // (T * es))
+            for p0 in range(0, n, m):
+                p1 = min(n, p0 + m)
+                slot = j % 2
+                a = time.perf_counter()
+                for f in pending[slot]:  # the buffer's previous block is written
+                    f.result()
+                wait_write += time.perf_counter() - a
+                with torch.cuda.stream(copy):
+                    blk = X[:, p0:p1].t().contiguous().view(-1).view(torch.uint8)
+                    if es > 1:  # the file's values are big-endian: bytes reversed on the device
+                        blk = blk.view(-1, es).flip(1).reshape(-1)
+                    host = bufs[slot][:blk.numel()]
+                    host.copy_(blk, non_blocking=True)
+                    ev = torch.cuda.Event()
+                    ev.record(copy)
+                a = time.perf_counter()
+                ev.synchronize()
+                wait_copy += time.perf_counter() - a
+                data = host.numpy()
+                pending[slot] = [pool.submit(ds.write_rows, name, p0, data) for ds in targets]
+                j += 1
+        a = time.perf_counter()
+        for f in pending[0] + pending[1]:
+            f.result()
+        wait_write += time.perf_counter() - a
+    return wait_copy, wait_write
 
 
 class RecordStream:

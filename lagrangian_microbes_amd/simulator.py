@@ -22,7 +22,7 @@ from . import device as dev
 from .engine import InteractionEngine, new_key, walk_amplitude
 from .interactions import rock_paper_scissors_interaction
 from .netcdf_io import ParticleDataset, copy_variables
-from .records import DEFAULT_RING_BYTES, RecordStream
+from .records import DEFAULT_RING_BYTES, DeviceRecords, RecordStream, device_record_budget, write_device_records
 from .utils import pretty_time
 from .velocity_fields import VelocityField
 
@@ -120,7 +120,7 @@ class FusedRunner:
         self.clock_steps += k
         return k
 
-    def run(self, nsteps, on_chunk=None):
+    def run(self, nsteps, on_chunk=None, records=None):
         """Run nsteps steps with consecutive chunks overlapped on the device.
 
         Three streams, three sets of chunk buffers (set c % 3 for chunk c):
@@ -132,9 +132,12 @@ class FusedRunner:
         chunk, ``on_chunk(k, rec_x, rec_y, rec_s, npairs)`` receives that
         chunk's records (device views, valid until the callback returns; if
         it returns a torch.cuda.Event, e.g. after copies on another stream
-        that read them, until that event).  The state and step counters end
-        as after ``advance`` calls over the same steps; returns the number of
-        steps done.
+        that read them, until that event).  With ``records`` (rec_x, rec_y,
+        rec_s: device float32, float32, int8 arrays [>= nsteps, n]) step t of
+        the call writes its records to row t of those instead of the chunk
+        buffers, so a run whose records stay on the device copies nothing.
+        The state and step counters end as after ``advance`` calls over the
+        same steps; returns the number of steps done.
         """
         total = max(0, int(nsteps))
         if total == 0:
@@ -148,8 +151,22 @@ class FusedRunner:
         if max(sizes) > self.chunk:  # the chunk buffers hold self.chunk steps
             raise RuntimeError(f"chunk schedule exceeds the buffers: {sizes}")
         self.last_schedule = list(sizes)  # chunk lengths of this call, one batched launch each
-        step0 = [self.step + sum(sizes[:c]) for c in range(len(sizes))]
-        clock0 = [self.clock_steps + sum(sizes[:c]) for c in range(len(sizes))]
+        offs = [sum(sizes[:c]) for c in range(len(sizes))]
+        if records is not None:
+            for t, dt in zip(records, (torch.float32, torch.float32, torch.int8)):
+                on_device = t.device.type == self.device.type and self.device.index in (None, t.device.index)
+                if (t.dtype != dt or t.dim() != 2 or t.shape[0] < total or t.shape[1] != self.n
+                        or not t.is_contiguous() or not on_device):
+                    raise ValueError(f"records must be contiguous {dt} [>= {total}, {self.n}] on {self.device}")
+
+        def recs(c):  # chunk c's records: rows of the caller's arrays, or its buffer set
+            k = sizes[c]
+            if records is not None:
+                return tuple(t[offs[c]:offs[c] + k] for t in records)
+            return tuple(t[:k] for t in self._rec_sets[c % nsets])
+
+        step0 = [self.step + o for o in offs]
+        clock0 = [self.clock_steps + o for o in offs]
         advected = [None] * len(sizes)   # event after chunk c's advection
         prepared = [None] * len(sizes)   # event after chunk c's preparation
         played = [None] * nsets          # event after the replay that last used set b
@@ -164,18 +181,19 @@ class FusedRunner:
                     adv.wait_event(played[b])
                 if released[b] is not None:
                     adv.wait_event(released[b])
-                self.dfield.advect(self.lon, self.lat, clock0[c] * self.dt_s, self.dt_s, k, self._rec_sets[b][0],
-                                   self._rec_sets[b][1], self.n, self.walk_amp, self.walk_key, step0[c], self.oob)
+                rx, ry, _ = recs(c)
+                self.dfield.advect(self.lon, self.lat, clock0[c] * self.dt_s, self.dt_s, k, rx, ry, self.n,
+                                   self.walk_amp, self.walk_key, step0[c], self.oob)
                 ev = torch.cuda.Event()
                 ev.record(adv)
                 advected[c] = ev
 
         def group(c):
-            b, k = c % nsets, sizes[c]
+            b = c % nsets
+            rx, ry, _ = recs(c)
             with torch.cuda.stream(side):
                 side.wait_event(advected[c])
-                eng.prepare_set(b, self._rec_sets[b][0][:k], self._rec_sets[b][1][:k], self.params, self.key,
-                                step0[c])
+                eng.prepare_set(b, rx, ry, self.params, self.key, step0[c])
                 ev = torch.cuda.Event()
                 ev.record(side)
                 prepared[c] = ev
@@ -194,7 +212,7 @@ class FusedRunner:
             if c + 2 < len(sizes):
                 advect(c + 2)
             main.wait_event(prepared[c])
-            rs = self._rec_sets[b][2][:k]
+            rx, ry, rs = recs(c)
             # The replay goes out before the host has seen the chunk's pair
             # counts, so it starts as soon as the preparation ends; the counts
             # come back (a pinned copy) while it runs.  A step that overflowed
@@ -221,7 +239,7 @@ class FusedRunner:
                 worst = eng.worst_count(b, k)
             ret = None
             if on_chunk is not None:
-                ret = on_chunk(k, self._rec_sets[b][0][:k], self._rec_sets[b][1][:k], rs, eng.npairs_sets[b][:k])
+                ret = on_chunk(k, rx, ry, rs, eng.npairs_sets[b][:k])
             ev = torch.cuda.Event()  # after the callback's own work on this stream
             ev.record(main)
             played[b] = ev
@@ -257,15 +275,18 @@ class MicrobeSimulator:
     writes particle_data.nc and microbe_data.nc with the layout of the two
     reference passes (write_outputs=False skips the files and the host
     copies of the records) and leaves the final species in
-    microbe_properties["species"].  Records leave the device chunk by chunk
-    through a ``RecordStream`` (pinned ring of ``ring_bytes``, copy stream,
-    writer thread) while the run goes on; ``timings`` holds the device
-    run's wall time and the output stages'.
+    microbe_properties["species"].  A run whose records fit the device
+    budget (``device_record_bytes``, see records.device_record_budget) keeps
+    them in HBM and writes the files after the run (``DeviceRecords``,
+    ``write_device_records``); a larger one moves them to the host chunk by
+    chunk through a ``RecordStream`` (pinned ring of ``ring_bytes``, copy
+    stream, writer thread) while the run goes on.  ``timings`` holds the
+    device run's wall time and the output stages'.
     """
 
     def __init__(self, particle_lons, particle_lats, velocity_field, pair_interaction, interaction_radius,
                  interaction_norm=2, output_dir=".", output_chunk_iters=100, Kh=0, seed=None, philox_key=None,
-                 device=None, ring_bytes=DEFAULT_RING_BYTES):
+                 device=None, ring_bytes=DEFAULT_RING_BYTES, device_record_bytes=None):
         assert isinstance(velocity_field, VelocityField), "velocity_field must be a VelocityField"
         assert particle_lons.size == particle_lats.size
         fn, params, props = pair_interaction
@@ -288,6 +309,7 @@ class MicrobeSimulator:
         self.walk_key = (new_key(seed)[0], 0x6B68)
         self.device = torch.device(device or "cuda")
         self.ring_bytes = int(ring_bytes)
+        self.device_record_bytes = device_record_bytes
         self.iteration = 0
         self.timings = {}
 
@@ -298,11 +320,62 @@ class MicrobeSimulator:
                              self.velocity_field, self.radius, self.params, self.philox_key, dt_s, self.norm,
                              self.chunk, self.Kh, self.walk_key, self.device)
         runner.step = self.iteration  # uniforms keyed by the global step; the clock restarts at 0
-        n = runner.n
         times = [start_time + k * dt for k in range(iters)]
+        paths = (os.path.join(self.output_dir, "microbe_data.nc"), os.path.join(self.output_dir, "particle_data.nc"))
+        resident = (write_outputs and iters > 0 and DeviceRecords.nbytes(runner.n, iters)
+                    <= device_record_budget(self.device, self.device_record_bytes))
+        if resident:
+            self.timings = self._run_resident(runner, iters, times, paths)
+        else:
+            self.timings = self._run_streamed(runner, iters, times, paths, write_outputs)
+        t = self.timings
+        logger.info("Fused advection + interactions, {:d} steps: {:s} (outputs: {:s} more)".format(
+            iters, pretty_time(t["run_s"]), pretty_time(t["output_s"])))
+        self.microbe_properties["species"][:] = runner.species.cpu().numpy()
+        self.final_lons = runner.lon.cpu().numpy()
+        self.final_lats = runner.lat.cpu().numpy()
+        # a later call continues from the float64 end state
+        self.particle_lons, self.particle_lats = self.final_lons, self.final_lats
+        self.iteration += iters
+
+    def _run_resident(self, runner, iters, times, paths):
+        """The run with its records in HBM, then the files written from there."""
+        n = runner.n
+        tic = time()
+        recs = DeviceRecords(n, iters, self.device)
+        done = [0]
+
+        def keep(k, rx, ry, rs, cnt):  # the records are already in place; the pair counts join them
+            recs.npairs[done[0]:done[0] + k].copy_(cnt)
+            done[0] += k
+
+        t_setup = time() - tic
+        tic = time()
+        runner.run(iters, on_chunk=keep, records=recs.arrays)
+        torch.cuda.synchronize(self.device)
+        t_run = time() - tic
+        tic = time()
+        self.pairs_per_step = recs.npairs.cpu().numpy()
+        with ParticleDataset(paths[0], n, times, {"longitude": np.float32, "latitude": np.float32,
+                                                  "species": np.int8}, threads=1) as microbe, \
+                ParticleDataset(paths[1], n, times, {"longitude": np.float32, "latitude": np.float32},
+                                threads=1) as particle:
+            wc, ww = write_device_records({"longitude": recs.longitude, "latitude": recs.latitude,
+                                           "species": recs.species},
+                                          [(microbe, ("longitude", "latitude", "species")),
+                                           (particle, ("longitude", "latitude"))])
+        t_out = time() - tic
+        del recs
+        runner.check_domain()
+        return {"records": "device", "setup_s": t_setup, "run_s": t_run, "output_s": t_out, "copy_wait_s": wc,
+                "write_wait_s": ww, "device_record_bytes": DeviceRecords.nbytes(n, iters)}
+
+    def _run_streamed(self, runner, iters, times, paths, write_outputs):
+        """The run with each chunk's records moved to the host as it goes
+        (only the pair counts without outputs)."""
+        n = runner.n
+        microbe_nc, particle_nc = paths
         self.pairs_per_step = np.zeros(iters, dtype=np.int64)
-        microbe_nc = os.path.join(self.output_dir, "microbe_data.nc")
-        particle_nc = os.path.join(self.output_dir, "particle_data.nc")
         tic = time()
         ds = None
         fields = {"npairs": (torch.int64, ())}
@@ -349,14 +422,6 @@ class MicrobeSimulator:
             ParticleDataset(particle_nc, n, times, {"longitude": np.float32, "latitude": np.float32}).close()
             copy_variables(microbe_nc, particle_nc, ["longitude", "latitude"])
         t_copy = time() - tic
-        self.timings = {"setup_s": t_setup, "run_s": t_run, "writer_drain_s": t_drain, "particle_file_s": t_copy,
-                        "writer_s": stream.write_s, "submit_wait_s": stream.wait_s,
-                        "ring_bytes": stream.ring_bytes}
-        logger.info("Fused advection + interactions, {:d} steps: {:s} (outputs: {:s} more)".format(
-            iters, pretty_time(t_run), pretty_time(t_drain + t_copy)))
-        self.microbe_properties["species"][:] = runner.species.cpu().numpy()
-        self.final_lons = runner.lon.cpu().numpy()
-        self.final_lats = runner.lat.cpu().numpy()
-        # a later call continues from the float64 end state
-        self.particle_lons, self.particle_lats = self.final_lons, self.final_lats
-        self.iteration += iters
+        return {"records": "stream" if write_outputs else "none", "setup_s": t_setup, "run_s": t_run,
+                "output_s": t_drain + t_copy, "writer_drain_s": t_drain, "particle_file_s": t_copy,
+                "writer_s": stream.write_s, "submit_wait_s": stream.wait_s, "ring_bytes": stream.ring_bytes}

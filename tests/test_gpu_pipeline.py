@@ -139,19 +139,28 @@ def test_generic_callback_path(gpu, c1_oracle, tmp_path):
     assert seen == expect
 
 
-@pytest.mark.parametrize("ring_bytes", [None, 1])
-def test_microbe_simulator_outputs(gpu, c1_oracle, tmp_path, ring_bytes):
-    # records stream to the files chunk by chunk; ring_bytes=1 leaves two pinned slots, so the run
+@pytest.mark.parametrize("mode,kw", [("device", {}), ("stream", {"device_record_bytes": 0}),
+                                     ("stream", {"device_record_bytes": 0, "ring_bytes": 1})])
+def test_microbe_simulator_outputs(gpu, c1_oracle, tmp_path, mode, kw):
+    # device: the records stay in HBM during the run and are written afterwards (in blocks of 3,000
+    # particles here, so the transposed blocks alternate between the two pinned buffers); stream:
+    # they go to the files chunk by chunk, and ring_bytes=1 leaves two pinned slots, so the run
     # waits for the writer thread before most chunks
+    from lagrangian_microbes_amd import records
     w = configs.rps_10k_steady()
     lon, lat = w.particles()
     rps = rock_paper_scissors(w.n, w.p, w.p, w.p, seed=w.species_seed)
-    kw = {} if ring_bytes is None else {"ring_bytes": ring_bytes}
     sim = MicrobeSimulator(lon, lat, w.velocity_field(), rps, w.radius, output_dir=str(tmp_path),
                            output_chunk_iters=7, philox_key=w.key, **kw)
     start, dt = datetime(2017, 1, 1), timedelta(hours=1)
-    sim.time_step(start, start + 24 * dt, dt)
-    if ring_bytes is not None:
+    block = records.BLOCK_BYTES
+    records.BLOCK_BYTES = 3000 * 24 * 4
+    try:
+        sim.time_step(start, start + 24 * dt, dt)
+    finally:
+        records.BLOCK_BYTES = block
+    assert sim.timings["records"] == mode
+    if "ring_bytes" in kw:
         assert sim.timings["ring_bytes"] == 2 * 7 * (w.n * 9 + 8)
     out, times, N = read_particle_dataset(os.path.join(str(tmp_path), "microbe_data.nc"))
     assert N == w.n and times == [start + k * dt for k in range(24)]

@@ -9,7 +9,9 @@
   lie more than 2^32 bytes past its start, and the next variable starts
   past 2^32;
 - block writes of steps into columns, and the file-to-file column copy that
-  makes particle_data.nc from microbe_data.nc.
+  makes particle_data.nc from microbe_data.nc;
+- row writes of whole particles (the device-resident records' path), from
+  arrays and from raw bytes, and their argument checks.
 """
 import struct
 from datetime import datetime, timedelta
@@ -138,3 +140,24 @@ def test_block_writes_and_column_copy(tmp_path, version):
     if version == 2:
         with netcdf_file(p, "r", mmap=False) as f:
             assert np.array_equal(f.variables["latitude"][:], -lon.T)
+
+
+@pytest.mark.parametrize("version", [2, 5])
+def test_row_writes(tmp_path, version):
+    rng = np.random.default_rng(4)
+    n, T = 10_007, 13
+    lon = rng.random((n, T)).astype(np.float32)   # particle-major, as the file holds it
+    sp = rng.integers(1, 4, (n, T)).astype(np.int8)
+    path = str(tmp_path / "rows.nc")
+    with ParticleDataset(path, n, hourly(T), {"longitude": np.float32, "species": np.int8}, version=version,
+                         threads=1) as ds:
+        for p0 in range(0, n, 3000):
+            ds.write_rows("longitude", p0, lon[p0:p0 + 3000])
+            ds.write_rows("species", p0, sp[p0:p0 + 3000].view(np.uint8).reshape(-1))  # raw bytes
+        with pytest.raises(ValueError):
+            ds.write_rows("longitude", n - 1, lon[:2])        # past the last particle
+        with pytest.raises(ValueError):
+            ds.write_rows("longitude", 0, lon[0, :5])         # not whole rows
+    d, times, nn = read_particle_dataset(path)
+    assert nn == n and times == hourly(T)
+    assert np.array_equal(d["longitude"], lon) and np.array_equal(d["species"], sp)
