@@ -5,7 +5,8 @@
 // T, so the three cells of a stencil row are three consecutive buckets; when
 // the box has fewer than T cells the table is dense and nothing aliases):
 //   1. bin_keys     : each particle's bucket
-//   2. sort         : stable radix sort of (bucket, index)
+//   2. partition    : one radix pass on the bucket's high bits (super-
+//                     buckets; no order inside one is needed)
 //   3. bin_table    : bucket starts (every bucket between two neighbouring
 //                     sorted keys starts at the later one) and records
 //                     {x, y, index} in bucket order
@@ -687,8 +688,10 @@ int lm_query_pairs_runs(lm_pairs_ws* ws, int32_t nsteps, int64_t n, const float*
     int rc = LM_OK;
     int res = 0;
     if (super_bits > 0) {
+        // unstable: the binning places a bucket's records in any order (the
+        // pair list is sorted later), so no ranking of equal digits is needed
         rc = lm::radix_partition_u32(ws->keys, ws->vals, st.rec, nsteps, n, fine_bits, super_bits, &ws->sort, s,
-                                     &dstart, &dstride, true);
+                                     &dstart, &dstride, true, false);
         if (rc) return rc;
         res = 1;
     }

@@ -43,15 +43,15 @@ int radix_sort_u32(uint32_t* keys[2], uint32_t* vals[2], int64_t stride, int32_t
                    int64_t uniform_len, int64_t max_len, int key_bits, SortScratch* scratch, hipStream_t stream,
                    int* result);
 
-// One stable pass over the digit (key >> shift) & (2^bits - 1) of nseg
-// segments of len elements each (bits <= kMaxDigitBits), from keys/vals[0]
-// into keys/vals[1]: the elements grouped by digit.  Digit d of segment g
-// starts at digit_start[g * scratch->row + d * digit_stride] (the offsets the
-// pass scattered with, valid until the scratch is used again).  With
-// index_vals, vals[0] is not read: each element's value is its index in the
-// segment.
+// One pass over the digit (key >> shift) & (2^bits - 1) of nseg segments of
+// len elements each (bits <= kMaxDigitBits), from keys/vals[0] into
+// keys/vals[1]: the elements grouped by digit, in their input order inside a
+// digit when stable, else in any order.  Digit d of segment g starts at
+// digit_start[g * scratch->row + d * digit_stride] (the offsets the pass
+// scattered with, valid until the scratch is used again).  With index_vals,
+// vals[0] is not read: each element's value is its index in the segment.
 int radix_partition_u32(uint32_t* keys[2], uint32_t* vals[2], int64_t stride, int32_t nseg, int64_t len,
                         int shift, int bits, SortScratch* scratch, hipStream_t stream, const uint32_t** digit_start,
-                        int64_t* digit_stride, bool index_vals = false);
+                        int64_t* digit_stride, bool index_vals = false, bool stable = true);
 
 }  // namespace lm

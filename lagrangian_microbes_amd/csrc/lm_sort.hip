@@ -2,9 +2,10 @@
 //
 // Per pass over a digit of up to 10 bits (R = 2^bits buckets), with blocks of
 // kSortGroup consecutive tiles of 2048 elements:
-//   hist    : per (segment, tile) the digit counts, stored digit-major,
-//             hist[g][d * ntp + tile] (ntp = tiles rounded up to the group),
-//             one 16-byte store per digit and block
+//   hist    : per (segment, tile) the digit counts (LDS atomics: counting
+//             needs no order, and the keys here give few conflicts), stored
+//             digit-major, hist[g][d * ntp + tile] (ntp = tiles rounded up to
+//             the group), one 16-byte store per digit and block
 //   scan    : exclusive scan of each segment's histogram -> the output offset
 //             of (digit, tile); digit-major order puts every tile's run of a
 //             digit after the earlier tiles' runs of that digit
@@ -18,7 +19,10 @@
 //             tile sorted by digit; the tile is staged there in LDS and written
 //             out in that order, so consecutive lanes store to consecutive
 //             addresses of a run.  The next tile's keys load meanwhile.
-// Equal digits keep their input order, which makes every pass stable.
+// Equal digits keep their input order, which makes every pass stable.  A
+// caller that needs no order inside a digit (a partition whose buckets are
+// placed in any order later) asks for an unstable pass: the rank is then an
+// LDS atomic's return value, with no ballots.
 #include "lm_sort.h"
 #include "lm_scan.h"
 
@@ -76,9 +80,7 @@ radix_hist_kernel(const uint32_t* __restrict__ keys, int64_t stride, const int64
 #pragma unroll
         for (int r = 0; r < kSortItems; ++r) {
             const bool valid = first + (int64_t)j * kSortTile + r * 64 + lane < len;
-            const uint32_t d = (kk[j][r] >> shift) & dmask;
-            const unsigned long long m = match_digit<BITS>(valid, d);
-            if (valid && __popcll(m & ((1ull << lane) - 1ull)) == 0) atomicAdd(&h[j][d], (uint32_t)__popcll(m));
+            if (valid) atomicAdd(&h[j][(kk[j][r] >> shift) & dmask], 1u);
         }
     __syncthreads();
     for (int q = t; q < R; q += kSortThreads)
@@ -86,7 +88,7 @@ radix_hist_kernel(const uint32_t* __restrict__ keys, int64_t stride, const int64
             make_uint4(h[0][q], h[1][q], h[2][q], h[3][q]);
 }
 
-template <int BITS>
+template <int BITS, bool STABLE>
 __global__ void __launch_bounds__(kSortThreads)
 radix_scatter_kernel(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
                      uint32_t* __restrict__ vout, int64_t stride, const int64_t* __restrict__ seg_len,
@@ -140,11 +142,15 @@ radix_scatter_kernel(const uint32_t* __restrict__ kin, const uint32_t* __restric
         for (int r = 0; r < kSortItems; ++r) {
             const bool valid = base + r * 64 + lane < len;
             const uint32_t d = (key[r] >> shift) & dmask;
-            const unsigned long long m = match_digit<BITS>(valid, d);
-            const uint32_t rank = (uint32_t)__popcll(m & below);
-            const uint32_t run = valid ? wcnt[wave][d] : 0u;
-            mine[r] = run + rank;
-            if (valid && rank == 0) wcnt[wave][d] = run + (uint32_t)__popcll(m);
+            if constexpr (STABLE) {
+                const unsigned long long m = match_digit<BITS>(valid, d);
+                const uint32_t rank = (uint32_t)__popcll(m & below);
+                const uint32_t run = valid ? wcnt[wave][d] : 0u;
+                mine[r] = run + rank;
+                if (valid && rank == 0) wcnt[wave][d] = run + (uint32_t)__popcll(m);
+            } else {
+                mine[r] = valid ? atomicAdd(&wcnt[wave][d], 1u) : 0u;
+            }
         }
         // next tile's elements, in flight while this one is placed
         uint32_t nkey[kSortItems], nval[kSortItems];
@@ -255,11 +261,25 @@ void sort_scratch_free(SortScratch* s) {
     s->bytes = 0;
 }
 
-// One stable pass over the digit (key >> shift) & dmask (at most
-// kMaxDigitBits wide) from keys/vals[cur] into keys/vals[1 - cur].
+template <int BITS>
+static void launch_scatter(bool stable, dim3 grid, hipStream_t stream, const uint32_t* kin, const uint32_t* vin,
+                           uint32_t* kout, uint32_t* vout, int64_t stride, const int64_t* seg_len,
+                           int64_t uniform_len, int64_t max_len, int shift, uint32_t dmask, const uint32_t* offs,
+                           int64_t row, int64_t ntp) {
+    if (stable)
+        LM_LAUNCH(K_SORT_SCATTER, (radix_scatter_kernel<BITS, true>), grid, dim3(kSortThreads), 0, stream, kin, vin,
+                  kout, vout, stride, seg_len, uniform_len, max_len, shift, dmask, offs, row, ntp);
+    else
+        LM_LAUNCH(K_SORT_SCATTER, (radix_scatter_kernel<BITS, false>), grid, dim3(kSortThreads), 0, stream, kin, vin,
+                  kout, vout, stride, seg_len, uniform_len, max_len, shift, dmask, offs, row, ntp);
+}
+
+// One pass over the digit (key >> shift) & dmask (at most kMaxDigitBits
+// wide) from keys/vals[cur] into keys/vals[1 - cur]; stable unless asked
+// otherwise.
 static int radix_pass(uint32_t* keys[2], uint32_t* vals[2], int cur, int64_t stride, int32_t nseg,
                       const int64_t* seg_len, int64_t uniform_len, int64_t max_len, int shift, int bits,
-                      SortScratch* sc, hipStream_t stream, bool index_vals = false) {
+                      SortScratch* sc, hipStream_t stream, bool index_vals = false, bool stable = true) {
     const uint32_t* vin = index_vals ? nullptr : (const uint32_t*)vals[cur];
     const int64_t ntp = sort_padded_tiles(max_len);
     const dim3 grid((unsigned)(ntp / kSortGroup), (unsigned)nseg);
@@ -282,15 +302,14 @@ static int radix_pass(uint32_t* keys[2], uint32_t* vals[2], int cur, int64_t str
     if (rc) return rc;
     const uint32_t* offs = (const uint32_t*)sc->offs;
     if (tbits == 8)
-        LM_LAUNCH(K_SORT_SCATTER, radix_scatter_kernel<8>, grid, dim3(kSortThreads), 0, stream, kin, vin,
-                  keys[1 - cur], vals[1 - cur], stride, seg_len, uniform_len, max_len, shift, dmask, offs, sc->row, ntp);
+        launch_scatter<8>(stable, grid, stream, kin, vin, keys[1 - cur], vals[1 - cur], stride, seg_len, uniform_len,
+                          max_len, shift, dmask, offs, sc->row, ntp);
     else if (tbits == 9)
-        LM_LAUNCH(K_SORT_SCATTER, radix_scatter_kernel<9>, grid, dim3(kSortThreads), 0, stream, kin, vin,
-                  keys[1 - cur], vals[1 - cur], stride, seg_len, uniform_len, max_len, shift, dmask, offs, sc->row, ntp);
+        launch_scatter<9>(stable, grid, stream, kin, vin, keys[1 - cur], vals[1 - cur], stride, seg_len, uniform_len,
+                          max_len, shift, dmask, offs, sc->row, ntp);
     else
-        LM_LAUNCH(K_SORT_SCATTER, radix_scatter_kernel<kMaxDigitBits>, grid, dim3(kSortThreads), 0, stream, kin,
-                  vin, keys[1 - cur], vals[1 - cur], stride, seg_len, uniform_len, max_len, shift, dmask, offs,
-                  sc->row, ntp);
+        launch_scatter<kMaxDigitBits>(stable, grid, stream, kin, vin, keys[1 - cur], vals[1 - cur], stride, seg_len,
+                                      uniform_len, max_len, shift, dmask, offs, sc->row, ntp);
     LM_LAUNCH_CHECK();
     return LM_OK;
 }
@@ -321,13 +340,13 @@ int radix_sort_u32(uint32_t* keys[2], uint32_t* vals[2], int64_t stride, int32_t
 
 int radix_partition_u32(uint32_t* keys[2], uint32_t* vals[2], int64_t stride, int32_t nseg, int64_t len,
                         int shift, int bits, SortScratch* sc, hipStream_t stream, const uint32_t** digit_start,
-                        int64_t* digit_stride, bool index_vals) {
+                        int64_t* digit_stride, bool index_vals, bool stable) {
     LM_REQUIRE(nseg >= 1 && nseg <= sc->nseg_max && len <= sc->max_len && stride >= len, LM_ERR_ARG,
                "partition request exceeds its scratch");
     LM_REQUIRE(bits >= 1 && bits <= kMaxDigitBits && shift >= 0 && shift + bits <= 32, LM_ERR_ARG,
                "partition digit out of range");
     LM_REQUIRE(len >= 1, LM_ERR_ARG, "empty partition");
-    int rc = radix_pass(keys, vals, 0, stride, nseg, nullptr, len, len, shift, bits, sc, stream, index_vals);
+    int rc = radix_pass(keys, vals, 0, stride, nseg, nullptr, len, len, shift, bits, sc, stream, index_vals, stable);
     if (rc) return rc;
     *digit_start = sc->offs;
     *digit_stride = sort_padded_tiles(len);
