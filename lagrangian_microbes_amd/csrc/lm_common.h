@@ -182,6 +182,19 @@ constexpr uint64_t kStreamWalk = 1;
 #define LM_DEG2RAD 0.017453292519943295
 // 1 / (1852 * 60): degrees per metre of arc.
 #define LM_INV_M_PER_DEG 0x1.2df73a213fa8bp-17
+// Keeps the loads that produced these 32-bit values ahead of this point: the
+// compiler may otherwise sink a load into a later branch that alone uses it,
+// where it waits on its own, so a group of loads issued together would take
+// several memory latencies instead of one.  An empty statement that reads
+// and writes each value in a VGPR: it only forces the values to be present
+// here (one wait for the group).
+__device__ __forceinline__ void lm_pin1(uint32_t& v) { asm volatile("" : "+v"(v)); }
+
+template <typename... T>
+__device__ __forceinline__ void lm_pin(T&... v) {
+    (lm_pin1(v), ...);
+}
+
 // XCD-aware block order.  MI355X has 8 XCDs, each with its own L2, and hands
 // workgroups to them round-robin by linear workgroup id.  This maps the
 // dispatch order so that XCD x runs the x-th contiguous eighth of the

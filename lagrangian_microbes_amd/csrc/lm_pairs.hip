@@ -177,7 +177,18 @@ bin_table_kernel(int64_t n, const float2* __restrict__ xy, const uint16_t* __res
     const uint32_t fmask = fine - 1u;
     for (uint32_t f = t; f < fine; f += kBlock) cnt[f] = 0;
     __syncthreads();
-    for (uint32_t e = a + t; e < b; e += kBlock) atomicAdd(&cnt[keys[e] & fmask], 1u);
+    // kPlace keys per thread in flight at once (indices clamped into the
+    // super-bucket, loads pinned ahead of the atomics)
+    for (uint32_t e0 = a + t; e0 < b; e0 += kBlock * kPlace) {
+        uint32_t kk[kPlace];
+#pragma unroll
+        for (int u = 0; u < kPlace; ++u) kk[u] = keys[min(e0 + (uint32_t)u * kBlock, b - 1u)];
+#pragma unroll
+        for (int u = 0; u < kPlace; ++u) lm_pin(kk[u]);
+#pragma unroll
+        for (int u = 0; u < kPlace; ++u)
+            if (e0 + (uint32_t)u * kBlock < b) atomicAdd(&cnt[kk[u] & fmask], 1u);
+    }
     __syncthreads();
     // exclusive offsets: thread t owns `per` consecutive buckets
     const uint32_t per = (fine + kBlock - 1) / kBlock;
@@ -199,24 +210,35 @@ bin_table_kernel(int64_t n, const float2* __restrict__ xy, const uint16_t* __res
     __syncthreads();
     // kPlace particles per thread at once, so their position gathers (random
     // lines) are in flight together
+    // two rounds of loads per kPlace particles, each issued at once (indices
+    // clamped, loads pinned ahead of their uses): indices and keys, then the
+    // positions the indices point at
     for (uint32_t e0 = a + t; e0 < b; e0 += kBlock * kPlace) {
-        uint32_t ii[kPlace], pp[kPlace];
+        uint32_t ii[kPlace], kk[kPlace], pp[kPlace];
 #pragma unroll
         for (int u = 0; u < kPlace; ++u) {
-            const uint32_t e = e0 + (uint32_t)u * kBlock;
-            const bool live = e < b;
-            ii[u] = live ? vals[e] : nn;
-            pp[u] = live ? a + atomicAdd(&cnt[keys[e] & fmask], 1u) : b;
+            const uint32_t e = min(e0 + (uint32_t)u * kBlock, b - 1u);
+            ii[u] = vals[e];
+            kk[u] = keys[e];
+        }
+#pragma unroll
+        for (int u = 0; u < kPlace; ++u) lm_pin(ii[u], kk[u]);
+#pragma unroll
+        for (int u = 0; u < kPlace; ++u) {
+            const bool live = e0 + (uint32_t)u * kBlock < b && ii[u] < nn;
+            pp[u] = live ? a + atomicAdd(&cnt[kk[u] & fmask], 1u) : b;
         }
         float2 pv[kPlace];
 #pragma unroll
-        for (int u = 0; u < kPlace; ++u)
-            pv[u] = ii[u] < nn && pp[u] < b ? xy[s * st.rec + ii[u]] : make_float2(0.0f, 0.0f);
+        for (int u = 0; u < kPlace; ++u) pv[u] = xy[s * st.rec + (ii[u] < nn ? ii[u] : 0u)];
+        uint32_t rr[kPlace];
+#pragma unroll
+        for (int u = 0; u < kPlace; ++u) rr[u] = run ? (uint32_t)run[ii[u] < nn ? ii[u] : 0u] : 0u;
 #pragma unroll
         for (int u = 0; u < kPlace; ++u)
-            if (ii[u] < nn && pp[u] < b)
+            if (pp[u] < b)
                 recs_all[s * st.rec + pp[u]] = make_float4(pv[u].x, pv[u].y, __uint_as_float(ii[u]),
-                                                           __uint_as_float(run ? (uint32_t)run[ii[u]] : 0u));
+                                                           __uint_as_float(rr[u]));
     }
 }
 
@@ -281,9 +303,9 @@ pair_count_kernel(int64_t n, CellGrid g, Metric m, Strides st, const uint32_t* _
     const uint32_t* start = start_all + s * st.tab;
     const float4* recs = recs_all + s * st.rec;
     const float4 me = recs[s0];
-    const uint32_t i = __float_as_uint(me.z);
+    uint32_t i = __float_as_uint(me.z), my_run = __float_as_uint(me.w);
+    lm_pin(i, my_run);  // the whole record in one load, not split around the branch
     if (i >= (uint64_t)n) return;
-    const uint32_t my_run = __float_as_uint(me.w);
     int cx, cy;
     cell_of(g, me.x, me.y, &cx, &cy);
     uint32_t rb[6], re[6];
@@ -311,6 +333,10 @@ pair_count_kernel(int64_t n, CellGrid g, Metric m, Strides st, const uint32_t* _
             for (int r = 1; r < 6; ++r) d = t >= pre[r] ? dlt[r] : d;
             o[u] = recs[t + d];
         }
+        // every load of the batch is issued before any use of the first: the
+        // scheduler would otherwise start testing candidate 0 early and wait
+        // for it before issuing the rest
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int u = 0; u < kCand; ++u) {
             const uint32_t j = __float_as_uint(o[u].z);
@@ -381,19 +407,21 @@ pair_emit_kernel(int64_t n, const float* __restrict__ x_all, const float* __rest
     const int64_t s = by;
     const uint32_t* row_off = row_off_all + s * st.row;
     const uint32_t i = (uint32_t)i64;
-    // the step's total, this row's count and offset load together, before
-    // any branch on them
+    // the step's total, this row's count and offset and its first two
+    // partners (read whatever the count, then used as it says) load
+    // together, before any branch on them
     const uint32_t total = row_off[n];
-    const uint32_t c = nbr_cnt_all[s * st.row + i];
-    const uint32_t o = row_off[i];
+    const uint32_t* row = nbr_all + s * st.nbr + (int64_t)i * kNbr;
+    uint32_t c = nbr_cnt_all[s * st.row + i], o = row_off[i];
+    const uint2 ab = *reinterpret_cast<const uint2*>(row);  // rows are 32-byte aligned
+    uint32_t r0 = ab.x, r1 = ab.y;
+    lm_pin(c, o, r0, r1);
     if ((int64_t)total > st.pair) return;  // caller enlarges and calls again
     if (c == 0) return;
     int32_t* pair_i = pair_i_all + s * st.pair;
     int32_t* pair_j = pair_j_all + s * st.pair;
-    const uint32_t* row = nbr_all + s * st.nbr + (int64_t)i * kNbr;
     if (c <= 2) {
-        const uint2 ab = *reinterpret_cast<const uint2*>(row);  // rows are 32-byte aligned
-        const uint32_t a = ab.x, b = c == 2 ? ab.y : 0xffffffffu;
+        const uint32_t a = r0, b = c == 2 ? r1 : 0xffffffffu;
         pair_i[o] = (int32_t)i;
         pair_j[o] = (int32_t)min(a, b);
         if (c == 2) {

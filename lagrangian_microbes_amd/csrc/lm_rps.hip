@@ -236,13 +236,10 @@ rps_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* _
     uint8_t* bits = A.bits + po;
     for (int64_t k = (int64_t)bx * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
         // the loads this pair needs are issued together (indices clamped, no
-        // branch between them), so they overlap each other and the draw below
-        const uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
-        const uint32_t before = (uint32_t)pi[k > 0 ? k - 1 : 0], after = (uint32_t)pi[k + 1 < P ? k + 1 : k];
-        const uint32_t hi = max(a, b), lo = min(a, b);
-        // parent[b] == a was set by the hook pass, before this kernel: joins
-        // are permanent, so a pair seen joined at any time is joined
-        const uint32_t phi = parent[hi < (uint64_t)n ? hi : 0u];
+        // branch between them) and overlap the draw below; they are pinned
+        // after it, so that none sinks into the branches further down
+        uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
+        uint32_t before = (uint32_t)pi[k > 0 ? k - 1 : 0], after = (uint32_t)pi[k + 1 < P ? k + 1 : k];
         uint64_t rank = (uint64_t)k, q0 = k0, q1 = k1;
         double a_rs = pRS, a_pr = pPR, a_sp = pSP;
         if (RUNS) {
@@ -256,7 +253,13 @@ rps_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* _
             a_sp = q.pSP;
         }
         const double u = u01_from_u64(philox4x64_10(rank, step0 + (uint64_t)s, kStreamPairs, 0, q0, q1).v[0]);
-        bits[k] = (uint8_t)((u < a_rs ? 1u : 0u) | (u < a_pr ? 2u : 0u) | (u < a_sp ? 4u : 0u));
+        const uint8_t fbits = (uint8_t)((u < a_rs ? 1u : 0u) | (u < a_pr ? 2u : 0u) | (u < a_sp ? 4u : 0u));
+        lm_pin(a, b, before, after);
+        const uint32_t hi = max(a, b), lo = min(a, b);
+        // parent[b] == a was set by the hook pass, before this kernel: joins
+        // are permanent, so a pair seen joined at any time is joined
+        const uint32_t phi = parent[hi < (uint64_t)n ? hi : 0u];
+        bits[k] = fbits;
         if (a < (uint64_t)n) {
             uint32_t* row = reinterpret_cast<uint32_t*>(A.rows + ro + a);  // {begin, end}
             if (k == 0 || before != a) row[0] = (uint32_t)k;
@@ -319,6 +322,8 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
         tv[k] = A.touched[ro + vc];
     }
 #pragma unroll
+    for (int k = 0; k < kKeyItems; ++k) lm_pin(pv[k], ry[k], tv[k]);
+#pragma unroll
     for (int k = 0; k < kKeyItems; ++k) {
         const int64_t v = v0 + (int64_t)k * kBlock;
         const bool live = v < n;
@@ -345,6 +350,8 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
         uint32_t nq[kKeyItems];
 #pragma unroll
         for (int k = 0; k < kKeyItems; ++k) nq[k] = A.parent[ro + q[k]];
+#pragma unroll
+        for (int k = 0; k < kKeyItems; ++k) lm_pin(nq[k]);
         bool more = false;
 #pragma unroll
         for (int k = 0; k < kKeyItems; ++k) {
@@ -600,10 +607,13 @@ rps_fill_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __
     const int32_t* pj = pj_all + po;
     const uint2* info = A.info + ro;
     for (int64_t k = (int64_t)bx * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
-        // two rounds of loads, each issued together (indices clamped, no branch)
-        const uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
-        const uint32_t fb = A.bits[po + k];
-        const uint2 ia = info[a < (uint64_t)n ? a : 0u], ib = info[b < (uint64_t)n ? b : 0u];
+        // two rounds of loads, each issued together (indices clamped, no
+        // branch) and pinned so that none sinks into the branches below
+        uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
+        uint32_t fb = A.bits[po + k];
+        lm_pin(a, b, fb);
+        uint2 ia = info[a < (uint64_t)n ? a : 0u], ib = info[b < (uint64_t)n ? b : 0u];
+        lm_pin(ia.x, ia.y, ib.y);
         if (a >= (uint64_t)n) continue;
         const uint32_t sb = b < (uint64_t)n ? ib.y : ia.y;
         const uint32_t pos = ia.x + (uint32_t)k;
