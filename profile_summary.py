@@ -13,6 +13,11 @@ consecutive groups of N1, N2, ... launches and prints each group's average:
 the passes of a benchmark run over one workload.
 
     python profile_summary.py legs BENCH_JSON OUT_DIR [--command C]
+    python profile_summary.py units PMC_DIR [PMC_DIR ...] [--title T] [--command C] [--window WINDOW_JSON]
+
+``units`` reads ``--pmc`` runs of SQ_WAVES, SQ_INSTS_VALU, TA_TA_BUSY_sum,
+TA_ADDR_STALLED_BY_TC_CYCLES_sum and GRBM_GUI_ACTIVE (any passes) and prints
+per kernel the VALU issue share and how busy the address unit was.
 
 ``legs`` renders the ``fp32_path`` and ``ensemble`` objects of a ``bench.py
 --fp32 --ensemble`` JSON line as ``bench_<workload>.md`` files in OUT_DIR.
@@ -205,6 +210,68 @@ def traffic(args):
             json.dump(doc, fh, indent=1)
 
 
+# MI355X: 256 CUs (one address unit each) in 8 XCDs, 4 SIMDs per CU; a wave64
+# VALU instruction issues once per 4 cycles per SIMD; clock 2.4 GHz
+CUS, XCDS, SIMDS, CLOCK_HZ = 256, 8, 1024, 2.4e9
+
+
+def units(args):
+    """Per kernel over the window: VALU instructions per wavefront, the share
+    of the VALU issue rate used, and the share of time the vector memory
+    path's address unit (TA) was busy or stalled by the cache, from one or
+    more ``--pmc`` passes (durations from each pass's own dispatches)."""
+    win = window_launches(args.window)
+    per = collections.defaultdict(lambda: collections.defaultdict(float))
+    dur = collections.defaultdict(dict)
+    for d in args.dirs:
+        by_kernel = collections.defaultdict(lambda: collections.defaultdict(dict))
+        for r in csv.DictReader(open(one_file(d, "*_counter_collection.csv"))):
+            k = short_name(r["Kernel_Name"])
+            if k is None:
+                continue
+            did = int(r["Dispatch_Id"])
+            by_kernel[k][did][r["Counter_Name"]] = (by_kernel[k][did].get(r["Counter_Name"], 0.0)
+                                                     + float(r["Counter_Value"]))
+            dur[(d, k)][did] = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9
+        for k, disp in by_kernel.items():
+            ids = sorted(disp)
+            if win is not None:
+                n = win.get(k)
+                if not n:
+                    continue
+                ids = ids[-n:]
+            for did in ids:
+                for name, v in disp[did].items():
+                    per[k][name] += v / len(ids)
+            per[k]["_dispatches"] = len(ids)
+            per[k][f"_dur:{d}"] = sum(dur[(d, k)][i] for i in ids) / len(ids)
+    out = [f"# {args.title}", ""]
+    if args.command:
+        out += ["Commands:", "", "```", args.command, "```", ""]
+    out += [f"Per dispatch of the window. VALU share = SQ_INSTS_VALU x 4 cycles / (duration x {SIMDS} SIMDs x "
+            f"{CLOCK_HZ / 1e9:.1f} GHz); TA busy = (TA_TA_BUSY_sum / {CUS} address units) / (GRBM_GUI_ACTIVE / "
+            f"{XCDS} XCDs), TA stalled likewise from TA_ADDR_STALLED_BY_TC_CYCLES_sum.", "",
+            "| kernel | timer name | dispatches | avg us | VALU / wave | VALU share | TA busy | TA stalled by cache |",
+            "|---|---|---:|---:|---:|---:|---:|---:|"]
+    rows = []
+    for k, c in per.items():
+        durs = [v for name, v in c.items() if name.startswith("_dur:")]
+        t = sum(durs) / len(durs) if durs else 0.0
+        valu = c.get("SQ_INSTS_VALU")
+        waves = c.get("SQ_WAVES")
+        grbm = c.get("GRBM_GUI_ACTIVE")
+        ta = c.get("TA_TA_BUSY_sum")
+        st = c.get("TA_ADDR_STALLED_BY_TC_CYCLES_sum")
+        f = lambda x: "" if x is None else f"{x:.2f}"
+        rows.append((t, f"| {k} | {TIMER_NAMES.get(k, '')} | {int(c['_dispatches'])} | {t * 1e6:.1f} | "
+                        f"{'' if not (valu and waves) else f'{valu / waves:.0f}'} | "
+                        f"{f(None if valu is None or not t else valu * 4 / (t * SIMDS * CLOCK_HZ))} | "
+                        f"{f(None if not (ta and grbm) else (ta / CUS) / (grbm / XCDS))} | "
+                        f"{f(None if not (st and grbm) else (st / CUS) / (grbm / XCDS))} |"))
+    out += [r for _, r in sorted(rows, reverse=True)]
+    print("\n".join(out))
+
+
 def _fmt_rate(x):
     return f"{x:.2e}".replace("e+0", "e+")
 
@@ -272,12 +339,17 @@ def main():
     c.add_argument("--json")
     c.add_argument("--command", default="")
     c.add_argument("--window", default="")
+    u = sub.add_parser("units")
+    u.add_argument("dirs", nargs="+")
+    u.add_argument("--title", default="Compute-unit pipes per kernel")
+    u.add_argument("--command", default="")
+    u.add_argument("--window", default="")
     g = sub.add_parser("legs")
     g.add_argument("bench_json")
     g.add_argument("out_dir")
     g.add_argument("--command", default="")
     args = ap.parse_args()
-    {"trace": trace, "traffic": traffic, "legs": legs}[args.cmd](args)
+    {"trace": trace, "traffic": traffic, "units": units, "legs": legs}[args.cmd](args)
 
 
 if __name__ == "__main__":
