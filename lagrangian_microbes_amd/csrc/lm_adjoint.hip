@@ -21,9 +21,11 @@
 //              step into scratch, and per block the bounding box of the cells
 //              its stage samples touch
 //   backward   steps in reverse.  Each step re-samples its four stages from the
-//              stored start position (corners blended from the float32
-//              levels, which stay in L2) keeping each stage's velocity Jacobian
-//              and interpolation weights in registers, then sweeps back with
+//              stored start position (from per-lane bilinear coefficients of
+//              the current cell and level pair, gathered from the float32
+//              levels, which stay in L2, when the cell or pair changes)
+//              keeping each stage's velocity Jacobian and interpolation
+//              weights in registers, then sweeps back with
 //              lambda = dL/dx at the step's end:
 //                a4 = h/6 l, a3 = a2 = h/3 l, a1 = h/6 l, dL/dx0 = l
 //                stage 4 (x4 = x0 + h k3):   g = J4^T a4; dx0 += g; a3 += h g
@@ -42,9 +44,10 @@
 // Field-gradient accumulation: per lane in registers while successive stage
 // samples stay in one cell and level pair (a particle crosses a 0.25 degree
 // cell in several steps), then per block in an LDS window (the segment's
-// level pairs over the block's bounding box), then one global atomic per
-// touched node.  A block whose window does not fit the LDS it was launched
-// with adds its moments to a global slab, turned into node values at the end.
+// level pairs over the block's bounding box; LDS atomics), then one global
+// atomic per touched node.  A block whose window does not fit the LDS it was
+// launched with adds its moments to a global slab, turned into node values
+// at the end.
 #include "lm_common.h"
 
 #include <mutex>
@@ -211,17 +214,72 @@ struct LevelView {
     lm_grid g;
 };
 
-// Corners of cell (j, i) at stage st, blended as adj_blend_kernel does.
-__device__ __forceinline__ void level_corners(const LevelView& V, const AdjStage& st, int i, int j, double2 q[4]) {
-    const float2* r0 = V.glev + (int64_t)st.l * V.level_len + (int64_t)j * V.g.nx + i;
+// One cell's bilinear coefficients at one level pair, per lane.  At time
+// weight w, component c (u, v) of the field at fractional position (xi, eta)
+// is C0 + C1 xi + C2 eta + C3 xi eta with Ck = A[4c + k] + w D[4c + k]: A
+// from level l's corners (C0 = q00, C1 = q01 - q00, C2 = q10 - q00,
+// C3 = q11 - q10 - q01 + q00), D from level l + 1's minus level l's (zero
+// for a steady field).  That is the corner blend fma(w, f1 - f0, f0)
+// followed by the bilinear sample, reassociated: equal to rounding.  A
+// particle stays in one cell and level pair for many stage samples (it
+// crosses a 0.25 degree cell in tens of steps), so most samples reuse the
+// lane's coefficients: no gather, no blend of corners.
+struct CellCoef {
+    int l, cell;  // l < 0: empty; cell = (j << 16) | i
+    double A[8], D[8];
+};
+
+// The coefficients of cell (j, i) at level pair l from the float32 levels.
+__device__ __forceinline__ void coef_gather(const LevelView& V, int l, int i, int j, double (&A)[8], double (&D)[8]) {
+    const float2* r0 = V.glev + (int64_t)l * V.level_len + (int64_t)j * V.g.nx + i;
     const int64_t ls = V.gstride;
     const int pitch = V.g.nx;
     const float2 a[4] = {r0[0], r0[1], r0[pitch], r0[pitch + 1]};
     const float2 b[4] = {r0[ls], r0[ls + 1], r0[ls + pitch], r0[ls + pitch + 1]};
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-        q[c] = make_double2(fma(st.w, (double)b[c].x - (double)a[c].x, (double)a[c].x),
-                            fma(st.w, (double)b[c].y - (double)a[c].y, (double)a[c].y));
+    for (int c = 0; c < 2; ++c) {
+        double fa[4], fb[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            fa[k] = c ? (double)a[k].y : (double)a[k].x;
+            fb[k] = c ? (double)b[k].y : (double)b[k].x;
+        }
+        const double a1 = fa[1] - fa[0], a3 = (fa[3] - fa[2]) - a1;
+        const double b1 = fb[1] - fb[0], b3 = (fb[3] - fb[2]) - b1;
+        A[4 * c] = fa[0];
+        A[4 * c + 1] = a1;
+        A[4 * c + 2] = fa[2] - fa[0];
+        A[4 * c + 3] = a3;
+        D[4 * c] = fb[0] - fa[0];
+        D[4 * c + 1] = b1 - a1;
+        D[4 * c + 2] = (fb[2] - fb[0]) - (fa[2] - fa[0]);
+        D[4 * c + 3] = b3 - a3;
+    }
+}
+
+// The reverse sweep's gradient window: block-shared LDS (dynamic,
+// lds_bytes) over the level pairs [lo, lo + npair) and the cells of the
+// block's box (ch x cw cells from (j0, i0)), 16 floats of moment slots per
+// (level pair, cell), indexed (p * ch + cj) * cw + ci; used when it fits.
+// A flush outside it (every flush of a block whose box does not fit, or one
+// that rounding puts just past the recompute's box, which covers every
+// sample of the recompute's own trajectory) goes to the global slab.
+struct BlockBox {
+    int lo, npair, j0, i0, ch, cw;
+    bool win;
+};
+
+// Index of (level pair l, cell (j, i)) in the block's box, or -1.
+__device__ __forceinline__ int box_index(const BlockBox& B, int l, int i, int j) {
+    const int p = l - B.lo, cj = j - B.j0, ci = i - B.i0;
+    const bool in = (unsigned)p < (unsigned)B.npair && (unsigned)cj < (unsigned)B.ch && (unsigned)ci < (unsigned)B.cw;
+    return in ? (p * B.ch + cj) * B.cw + ci : -1;
+}
+
+__device__ __forceinline__ void coef_fill(const LevelView& V, int l, int i, int j, CellCoef& cc) {
+    coef_gather(V, l, i, j, cc.A, cc.D);
+    cc.l = l;
+    cc.cell = (j << 16) | i;
 }
 
 // cos and sin of a step's start latitude y0 (degrees), from which the
@@ -229,10 +287,14 @@ __device__ __forceinline__ void level_corners(const LevelView& V, const AdjStage
 // d = e pi / 180, cos y = cos y0 cos d - sin y0 sin d and sin y = sin y0 cos d
 // + cos y0 sin d, with cos d = 1 - d^2/2 + d^4/24 and sin d = d (1 - d^2/6 +
 // d^4/120), whose truncation (below d^6/720 and d^7/5040) is under 1e-19 for
-// |d| <= 2^-9 rad (0.11 degrees, farther than a stage moves in one step at
-// ocean speeds); a stage farther away takes the full polynomials.  The
-// results agree with cos_deg / sin_deg to rounding: the reverse sweep's
-// trajectory, like the recompute's, only positions the gradient.
+// |d| <= 2^-9 rad (0.11 degrees, farther than a particle moves in one step
+// at ocean speeds); farther away takes several such rotations.  A step's own
+// start latitude is taken the same way from the step after it (the reverse
+// sweep's previous one): the chain restarts from the polynomials at each
+// segment, so it spans at most one segment's steps and its rounding stays
+// within a few units in the last place.  The results agree with cos_deg /
+// sin_deg to rounding: the reverse sweep's trajectory, like the
+// recompute's, only positions the gradient.
 struct Trig0 {
     double y0, c0, s0;
 };
@@ -240,63 +302,69 @@ struct Trig0 {
 __device__ __forceinline__ Trig0 trig_at(double y0) { return Trig0{y0, cos_deg(y0), sin_deg(y0)}; }
 
 __device__ __forceinline__ void trig_near(const Trig0& T, double y, double* c, double* sn) {
-    const double d = (y - T.y0) * LM_DEG2RAD;
-    if (fabs(d) <= 0x1p-9) {
-        const double d2 = d * d;
-        const double cd = fma(fma(d2, 1.0 / 24.0, -0.5), d2, 1.0);
-        const double sd = fma(fma(d2, 1.0 / 120.0, -1.0 / 6.0), d2, 1.0) * d;
-        *c = fma(T.c0, cd, -(T.s0 * sd));
-        *sn = fma(T.s0, cd, T.c0 * sd);
-    } else {
-        *c = cos_deg(y);
-        *sn = sin_deg(y);
+    double d = (y - T.y0) * LM_DEG2RAD;
+    // Farther than 2^-9 rad (never at ocean speeds; up to pi inside a grid):
+    // k equal rotations of at most 2^-9 rad each.  Beyond pi (or NaN) the
+    // position is outside every grid and the result unused: one rotation.
+    int k = 1;
+    if (!(fabs(d) <= 0x1p-9) && fabs(d) <= 3.2) {
+        k = (int)ceil(fabs(d) * 512.0);
+        d /= (double)k;
     }
+    const double d2 = d * d;
+    const double cd = fma(fma(d2, 1.0 / 24.0, -0.5), d2, 1.0);
+    const double sd = fma(fma(d2, 1.0 / 120.0, -1.0 / 6.0), d2, 1.0) * d;
+    double cy = T.c0, sy = T.s0;
+    for (int t = 0; t < k; ++t) {
+        const double c1 = fma(cy, cd, -(sy * sd));
+        sy = fma(sy, cd, cy * sd);
+        cy = c1;
+    }
+    *c = cy;
+    *sn = sy;
 }
 
-// adj_sample_fast at stage st plus the linearisation, from the cell's
-// blended corners q (level_corners at stage st of the cell (i, j) that
-// adj_locate gave for (x, y), with fractional position (xi, eta)).  T: the
-// trigonometry of the step's start latitude.
-__device__ __forceinline__ void adj_linearise_at(const LevelView& V, const Trig0& T, const double2 (&q)[4], int i,
-                                                 int j, double xi, double eta, double y, double* ud, double* vd,
-                                                 Lin* L) {
-    const double ub = fma(xi, q[1].x - q[0].x, q[0].x);
-    const double ut = fma(xi, q[3].x - q[2].x, q[2].x);
-    const double u = fma(eta, ut - ub, ub);
-    const double vb = fma(xi, q[1].y - q[0].y, q[0].y);
-    const double vt = fma(xi, q[3].y - q[2].y, q[2].y);
-    const double v = fma(eta, vt - vb, vb);
-    double cy, sy;
-    trig_near(T, y, &cy, &sy);
+// adj_sample_fast at stage st plus the linearisation: the sample at (x, y)
+// from the lane's cell coefficients (refilled when the cell or the level
+// pair differs), with cos and sin of the latitude y given (cy, sy).  False
+// outside the grid.
+__device__ __forceinline__ bool adj_linearise(const LevelView& V, const AdjStage& st, CellCoef& cc, double x,
+                                              double y, double cy, double sy, double* ud, double* vd, Lin* L) {
+    int i, j;
+    double xi, eta;
+    if (!adj_locate(V.g, x, y, &i, &j, &xi, &eta)) return false;
+    const int cell = (j << 16) | i;
+    if (cc.l != st.l || cc.cell != cell) coef_fill(V, st.l, i, j, cc);
+    double c[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[k] = fma(st.w, cc.D[k], cc.A[k]);
+    const double u_xi = fma(eta, c[3], c[1]), v_xi = fma(eta, c[7], c[5]);
+    const double u_eta = fma(xi, c[3], c[2]), v_eta = fma(xi, c[7], c[6]);
+    const double u = fma(xi, u_xi, fma(eta, c[2], c[0]));
+    const double v = fma(xi, v_xi, fma(eta, c[6], c[4]));
     const double rc = recip(cy);
     const double kc = LM_INV_M_PER_DEG * rc;
     *ud = u * kc;
     *vd = v * LM_INV_M_PER_DEG;
-    const double du_dx = fma(eta, (q[3].x - q[2].x) - (q[1].x - q[0].x), q[1].x - q[0].x) * V.g.inv_dx;
-    const double dv_dx = fma(eta, (q[3].y - q[2].y) - (q[1].y - q[0].y), q[1].y - q[0].y) * V.g.inv_dx;
-    const double du_dy = (ut - ub) * V.g.inv_dy;
-    const double dv_dy = (vt - vb) * V.g.inv_dy;
     // d(1/cos(lat))/dlat = (pi/180) sin(lat) / cos(lat)^2
-    L->dux = kc * du_dx;
-    L->duy = kc * fma(u * LM_DEG2RAD * rc, sy, du_dy);
-    L->dvx = LM_INV_M_PER_DEG * dv_dx;
-    L->dvy = LM_INV_M_PER_DEG * dv_dy;
+    L->dux = kc * (u_xi * V.g.inv_dx);
+    L->duy = kc * fma(u * LM_DEG2RAD * rc, sy, u_eta * V.g.inv_dy);
+    L->dvx = LM_INV_M_PER_DEG * (v_xi * V.g.inv_dx);
+    L->dvy = LM_INV_M_PER_DEG * (v_eta * V.g.inv_dy);
     L->xi = (float)xi;
     L->eta = (float)eta;
     L->kc = (float)kc;
-    L->cell = (j << 16) | i;
+    L->cell = cell;
+    return true;
 }
 
-// The same from the position; false outside.
-__device__ __forceinline__ bool adj_linearise(const LevelView& V, const AdjStage& st, const Trig0& T, double x,
-                                              double y, double* ud, double* vd, Lin* L) {
-    int i, j;
-    double xi, eta;
-    if (!adj_locate(V.g, x, y, &i, &j, &xi, &eta)) return false;
-    double2 q[4];
-    level_corners(V, st, i, j, q);
-    adj_linearise_at(V, T, q, i, j, xi, eta, y, ud, vd, L);
-    return true;
+// The same at a stage position whose latitude's cos and sin come from T.
+__device__ __forceinline__ bool adj_linearise_near(const LevelView& V, const AdjStage& st, CellCoef& cc,
+                                                   const Trig0& T, double x, double y, double* ud, double* vd,
+                                                   Lin* L) {
+    double cy, sy;
+    trig_near(T, y, &cy, &sy);
+    return adj_linearise(V, st, cc, x, y, cy, sy, ud, vd, L);
 }
 
 // Field-gradient accumulation in moment form.  Samples in one cell (j, i)
@@ -308,17 +376,13 @@ __device__ __forceinline__ bool adj_linearise(const LevelView& V, const AdjStage
 // m0 - m1 - m2 + m3, (0,1) is m1 - m3, (1,0) is m2 - m3, (1,1) is m3, with
 // m = S - W for level l and m = W for level l + 1.
 //
-// Slots: 16 floats, [comp][S, W][4 moments], per level pair and cell.  A
-// block's LDS window holds the slots of level pairs [lo, lo + npair) over the
-// cells of its box (ch x cw cells from (j0, i0)); a flush outside the window,
-// or from a block whose window does not fit, goes to the global slab of all
+// Slots: 16 floats, [comp][S, W][4 moments], per level pair and cell: the
+// block's LDS window (BlockBox) or, outside it, the global slab of all
 // (nlev - 1 or 1) pairs x (ny - 1) x (nx - 1) cells.  Window and slab become
 // node gradients at the end: the block's write-out and adj_slab_kernel.
 struct GradSink {
-    float* lds;   // NULL: the block's window did not fit
     float* slab;  // [pairs][ny - 1][nx - 1][16]
     int cx, cy;   // cells along x, y
-    int lo, npair, j0, i0, ch, cw;
 };
 
 struct GradCache {
@@ -334,15 +398,19 @@ __device__ __forceinline__ float corner_of(const float* m, int dj, int di) {
     return m3;
 }
 
-__device__ __forceinline__ void cache_flush(GradCache& gc, const GradSink& sk) {
+// win: the block's window in LDS (read only when B.win).
+__device__ __forceinline__ void cache_flush(GradCache& gc, const BlockBox& B, float* win, const GradSink& sk) {
     if (gc.l < 0) return;
     const int cj = gc.cell >> 16, ci = gc.cell & 0xffff;
-    const int wl = gc.l - sk.lo, wj = cj - sk.j0, wi = ci - sk.i0;
-    // the recompute's box covers every sample, so the slab takes only blocks whose window did not fit
-    if (sk.lds && (unsigned)wl < (unsigned)sk.npair && (unsigned)wj < (unsigned)sk.ch && (unsigned)wi < (unsigned)sk.cw) {
-        float* slot = sk.lds + ((wl * sk.ch + wj) * sk.cw + wi) * 16;
+    const int t = B.win ? box_index(B, gc.l, ci, cj) : -1;
+    if (t >= 0) {
+        // workgroup scope, unlike the slab's device-scope atomics: the two
+        // loops are not merged into one through a generic pointer, so these
+        // stay LDS atomics
+        float* slot = win + t * 16;
 #pragma unroll
-        for (int k = 0; k < 16; ++k) atomicAdd(slot + k, gc.m[k]);
+        for (int k = 0; k < 16; ++k)
+            __hip_atomic_fetch_add(slot + k, gc.m[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     } else {
         float* slot = sk.slab + (((int64_t)gc.l * sk.cy + cj) * sk.cx + ci) * 16;
 #pragma unroll
@@ -395,10 +463,10 @@ __device__ __forceinline__ void cache_accumulate(GradCache& gc, float w, float x
 }
 
 // Field gradient of one stage sample with adjoint (au, av) of its velocity.
-__device__ __forceinline__ void cache_add(GradCache& gc, const GradSink& sk, const AdjStage& st, const Lin& L,
-                                          double au, double av) {
+__device__ __forceinline__ void cache_add(GradCache& gc, const BlockBox& B, float* win, const GradSink& sk,
+                                          const AdjStage& st, const Lin& L, double au, double av) {
     if (gc.l != st.l || gc.cell != L.cell) {
-        cache_flush(gc, sk);
+        cache_flush(gc, B, win, sk);
         gc.l = st.l;
         gc.cell = L.cell;
 #pragma unroll
@@ -481,50 +549,60 @@ adj_recompute_kernel(int64_t n, const double2* __restrict__ ck, AdjField F, int 
 // [lo, lo + npair) are the ones the segment's stages read; lds_bytes is the
 // dynamic LDS the launch carries for the block's gradient window.
 __global__ void __launch_bounds__(kAdjBlock)
-adj_backward_kernel(int64_t n, LevelView V, int nlev, int s0, int m, double hh, double h, double h6,
-                    const double2* __restrict__ pos, const int4* __restrict__ box, double* __restrict__ glon,
-                    double* __restrict__ glat, float* __restrict__ gfield, float* __restrict__ slab, int lo, int npair,
-                    int lds_bytes) {
-    extern __shared__ float win[];
+adj_backward_kernel(int64_t n, LevelView V, const int32_t* __restrict__ lev0, const double* __restrict__ wt, int nlev,
+                    int s0, int m, double hh, double h, double h6, const double2* __restrict__ pos,
+                    const int4* __restrict__ box, double* __restrict__ glon, double* __restrict__ glat,
+                    float* __restrict__ gfield, float* __restrict__ slab, int lo, int npair, int lds_bytes) {
+    extern __shared__ float lds[];
     uint32_t bx, by;
     xcd_block(&bx, &by);  // as the recompute, whose box of logical block bx this block reads
     const int4 b = box[bx];
     if (b.y < 0) return;  // no lane of the block completed a step inside the grid
+    BlockBox B{lo, npair, b.x, b.z, b.y - b.x + 1, b.w - b.z + 1, false};
+    // Entries (level pair, cell) of the box, or cap + 1 when more than the
+    // window holds: 32-bit saturating products and compares only (a select
+    // on a 64-bit compare's result has been compiled with a stale scalar
+    // condition, giving a wrong LDS offset).
+    const int cap = lds_bytes / 64;
+    int ncell = B.ch <= cap ? B.ch : cap + 1;
+    ncell = B.cw <= cap / ncell ? ncell * B.cw : cap + 1;
+    ncell = npair <= cap / ncell ? ncell * npair : cap + 1;
+    B.win = ncell <= cap;
+    float* win = lds;
+    if (B.win) {
+        for (int t = threadIdx.x; t < ncell * 16; t += kAdjBlock) win[t] = 0.0f;
+        __syncthreads();
+    }
     GradSink sk;
     sk.slab = slab;
     sk.cx = V.g.nx - 1;
     sk.cy = V.g.ny - 1;
-    sk.lo = lo;
-    sk.npair = npair;
-    sk.j0 = b.x;
-    sk.i0 = b.z;
-    sk.ch = b.y - b.x + 1;
-    sk.cw = b.w - b.z + 1;
-    const int64_t wfloats = (int64_t)npair * sk.ch * sk.cw * 16;
-    const bool win_on = wfloats * 4 <= (int64_t)lds_bytes;
-    sk.lds = win_on ? win : nullptr;
-    if (win_on) {
-        for (int t = threadIdx.x; t < (int)wfloats; t += kAdjBlock) win[t] = 0.0f;
-        __syncthreads();
-    }
     GradCache gc;
     gc.l = -1;
     const int64_t i = (int64_t)bx * kAdjBlock + threadIdx.x;
     if (i < n) {
         double lx = glon[i], ly = glat[i];
+        CellCoef cc;
+        cc.l = -1;
+        cc.cell = 0;
+        double2 p = pos[(int64_t)(m - 1) * n + i];
+        Trig0 T = trig_at(p.y);
         for (int k = m - 1; k >= 0; --k) {
             const int s = s0 + k;
-            const double2 p = pos[(int64_t)k * n + i];
             const double x0 = p.x, y0 = p.y;
-            const AdjStage sa{2 * s, V.lev0[2 * s], V.w[2 * s]}, sb{2 * s + 1, V.lev0[2 * s + 1], V.w[2 * s + 1]},
-                sc{2 * s + 2, V.lev0[2 * s + 2], V.w[2 * s + 2]};
+            // the start of the step before, loading while this one is linearised
+            if (k > 0) p = pos[(int64_t)(k - 1) * n + i];
+            // schedule entries: uniform, read through the restrict pointers with scalar loads
+            const AdjStage sa{2 * s, lev0[2 * s], wt[2 * s]}, sb{2 * s + 1, lev0[2 * s + 1], wt[2 * s + 1]},
+                sc{2 * s + 2, lev0[2 * s + 2], wt[2 * s + 2]};
             Lin L1, L2, L3, L4;
-            double ud, vd;
-            const Trig0 T = trig_at(y0);
-            if (!adj_linearise(V, sa, T, x0, y0, &ud, &vd, &L1)) continue;
-            if (!adj_linearise(V, sb, T, fma(ud, hh, x0), fma(vd, hh, y0), &ud, &vd, &L2)) continue;
-            if (!adj_linearise(V, sb, T, fma(ud, hh, x0), fma(vd, hh, y0), &ud, &vd, &L3)) continue;
-            if (!adj_linearise(V, sc, T, fma(ud, h, x0), fma(vd, h, y0), &ud, &vd, &L4)) continue;
+            double ud, vd, cy, sy;
+            trig_near(T, y0, &cy, &sy);
+            T = Trig0{y0, cy, sy};
+            if (!adj_linearise(V, sa, cc, x0, y0, cy, sy, &ud, &vd, &L1)) continue;
+            if (!adj_linearise_near(V, sb, cc, T, fma(ud, hh, x0), fma(vd, hh, y0), &ud, &vd, &L2)) continue;
+            if (!adj_linearise_near(V, sb, cc, T, fma(ud, hh, x0), fma(vd, hh, y0), &ud, &vd, &L3)) continue;
+            if (!adj_linearise_near(V, sc, cc, T, fma(ud, h, x0), fma(vd, h, y0), &ud, &vd, &L4)) continue;
             const double a4x = h6 * lx, a4y = h6 * ly;
             double a3x = 2.0 * a4x, a3y = 2.0 * a4y;
             double a2x = a3x, a2y = a3y;
@@ -533,7 +611,7 @@ adj_backward_kernel(int64_t n, LevelView V, int nlev, int s0, int m, double hh, 
             // stage 4
             gx = fma(a4x, L4.dux, a4y * L4.dvx);
             gy = fma(a4x, L4.duy, a4y * L4.dvy);
-            cache_add(gc, sk, sc, L4, a4x, a4y);
+            cache_add(gc, B, win, sk, sc, L4, a4x, a4y);
             lx += gx;
             ly += gy;
             a3x = fma(h, gx, a3x);
@@ -541,7 +619,7 @@ adj_backward_kernel(int64_t n, LevelView V, int nlev, int s0, int m, double hh, 
             // stage 3
             gx = fma(a3x, L3.dux, a3y * L3.dvx);
             gy = fma(a3x, L3.duy, a3y * L3.dvy);
-            cache_add(gc, sk, sb, L3, a3x, a3y);
+            cache_add(gc, B, win, sk, sb, L3, a3x, a3y);
             lx += gx;
             ly += gy;
             a2x = fma(hh, gx, a2x);
@@ -549,7 +627,7 @@ adj_backward_kernel(int64_t n, LevelView V, int nlev, int s0, int m, double hh, 
             // stage 2
             gx = fma(a2x, L2.dux, a2y * L2.dvx);
             gy = fma(a2x, L2.duy, a2y * L2.dvy);
-            cache_add(gc, sk, sb, L2, a2x, a2y);
+            cache_add(gc, B, win, sk, sb, L2, a2x, a2y);
             lx += gx;
             ly += gy;
             a1x = fma(hh, gx, a1x);
@@ -557,17 +635,17 @@ adj_backward_kernel(int64_t n, LevelView V, int nlev, int s0, int m, double hh, 
             // stage 1
             gx = fma(a1x, L1.dux, a1y * L1.dvx);
             gy = fma(a1x, L1.duy, a1y * L1.dvy);
-            cache_add(gc, sk, sa, L1, a1x, a1y);
+            cache_add(gc, B, win, sk, sa, L1, a1x, a1y);
             lx += gx;
             ly += gy;
         }
         glon[i] = lx;
         glat[i] = ly;
     }
-    cache_flush(gc, sk);
-    if (!win_on) return;
+    cache_flush(gc, B, win, sk);
+    if (!B.win) return;
     __syncthreads();
-    const int nh = sk.ch + 1, nw = sk.cw + 1, nlw = nlev > 1 ? npair + 1 : 1;
+    const int nh = B.ch + 1, nw = B.cw + 1, nlw = nlev > 1 ? npair + 1 : 1;
     const int nodes = nlw * nh * nw * 2;
     for (int t = threadIdx.x; t < nodes; t += kAdjBlock) {
         const int comp = t & 1;
@@ -577,11 +655,11 @@ adj_backward_kernel(int64_t n, LevelView V, int nlev, int s0, int m, double hh, 
         const int nj = r % nh;
         const int lv = r / nh;
         const float acc = node_from_slots(lv, nj, ni, nlev == 1, [&](int p, int cj, int ci) -> const float* {
-            if (p < 0 || p >= npair || cj < 0 || cj >= sk.ch || ci < 0 || ci >= sk.cw) return nullptr;
-            return sk.lds + ((p * sk.ch + cj) * sk.cw + ci) * 16 + comp * 8;
+            if (p < 0 || p >= npair || cj < 0 || cj >= B.ch || ci < 0 || ci >= B.cw) return nullptr;
+            return win + ((p * B.ch + cj) * B.cw + ci) * 16 + comp * 8;
         });
         if (acc != 0.0f)
-            atomicAdd(gfield + ((int64_t)(lo + lv) * V.level_len + (int64_t)(sk.j0 + nj) * V.g.nx + (sk.i0 + ni)) * 2 +
+            atomicAdd(gfield + ((int64_t)(lo + lv) * V.level_len + (int64_t)(B.j0 + nj) * V.g.nx + (B.i0 + ni)) * 2 +
                           comp,
                       acc);
     }
@@ -617,10 +695,10 @@ adj_slab_kernel(const float* __restrict__ slab, int nlev, lm_grid g, float* __re
 
 namespace {
 
-// Dynamic LDS each backward block may use for its gradient window: 40 KB
-// lets four blocks share a CU's 160 KB; a block whose box needs more adds to
-// global memory directly.
-constexpr int kAdjWindowBytes = 40960;
+// Dynamic LDS each backward block may use for its gradient window: 52 KB
+// lets three blocks (the waves its registers allow) share a CU's 160 KB; a
+// block whose box needs more adds to global memory directly.
+constexpr int kAdjWindowBytes = 53248;
 
 int adj_check(int64_t n, int32_t nlev, const lm_grid* grid, const int32_t* lev0_host, int32_t nsteps,
               int32_t ckpt) {
@@ -820,8 +898,8 @@ int lm_adjoint_backward(int64_t n, const double* checkpoints, const float* field
                   ck + (int64_t)seg * n, F, s0, m, hh, dt, h6, pos, box);
         LM_LAUNCH_CHECK();
         LM_LAUNCH(lm::K_ADJ_BACKWARD, lm::adj_backward_kernel, dim3(nb), dim3(lm::kAdjBlock),
-                  (unsigned)kAdjWindowBytes, s, n, V, (int)nlev, s0, m, hh, dt, h6, pos, box, glon, glat, gfield, slab,
-                  lo, hi - lo + 1, kAdjWindowBytes);
+                  (unsigned)kAdjWindowBytes, s, n, V, lev0, w, (int)nlev, s0, m, hh, dt, h6, pos, box, glon, glat,
+                  gfield, slab, lo, hi - lo + 1, kAdjWindowBytes);
         LM_LAUNCH_CHECK();
     }
     LM_LAUNCH(lm::K_ADJ_SLAB, lm::adj_slab_kernel, dim3(lm::grid_for((int64_t)nlev * level_len_of(grid) * 2, lm::kAdjBlock)),

@@ -174,11 +174,6 @@ __host__ __device__ inline double u01_from_u64(uint64_t x) {
 constexpr uint64_t kStreamPairs = 0;
 constexpr uint64_t kStreamWalk = 1;
 
-// ----------------------------------------------------------------------------
-// cos(lat * pi / 180) for |lat| <= 90: Taylor series of cos about 0 through
-// the x^22 term (truncation below 1e-19 for |x| <= pi/2), Horner in z = x^2
-// with explicit fma.  Written out so CPU and GPU evaluate the same operations.
-// ----------------------------------------------------------------------------
 #define LM_DEG2RAD 0.017453292519943295
 // 1 / (1852 * 60): degrees per metre of arc.
 #define LM_INV_M_PER_DEG 0x1.2df73a213fa8bp-17
@@ -212,22 +207,35 @@ __device__ __forceinline__ void xcd_block(uint32_t* bx, uint32_t* by) {
     *bx = logical % gx;
 }
 
+// ----------------------------------------------------------------------------
+// cos(lat * pi / 180) for |lat| <= 90: Taylor series of cos about 0 through
+// the x^22 term (truncation below 1e-19 for |x| <= pi/2), Horner in z = x^2
+// with explicit fma.  Written out so CPU and GPU evaluate the same operations.
+// Coefficients c_k = (-1)^k / (2k)!, k = 11 .. 0, as exactly rounded hex
+// literals (a macro, so a kernel can also keep them in constant memory).
+// ----------------------------------------------------------------------------
+#define LM_COS_COEFS                                                                                          \
+    -0x1.0ce396db7f853p-70, 0x1.e542ba4020225p-62, -0x1.6827863b97d97p-53, 0x1.ae7f3e733b81fp-45,             \
+        -0x1.93974a8c07c9dp-37, 0x1.1eed8eff8d898p-29, -0x1.27e4fb7789f5cp-22, 0x1.a01a01a01a01ap-16,          \
+        -0x1.6c16c16c16c17p-10, 0x1.5555555555555p-5, -0.5, 1.0
+// sin: s_k = (-1)^k / (2k+1)!, k = 8 .. 0.
+#define LM_SIN_COEFS                                                                                          \
+    0x1.952c77030ad4ap-49, -0x1.ae7f3e733b81fp-41, 0x1.6124613a86d09p-33, -0x1.ae64567f544e4p-26,             \
+        0x1.71de3a556c734p-19, -0x1.a01a01a01a01ap-13, 0x1.1111111111111p-7, -0x1.5555555555555p-3, 1.0
+
+// Horner in z over coefficients c[0 .. N-1] (highest power first).
+template <int N>
+__host__ __device__ inline double lm_horner(const double* c, double z) {
+    double p = c[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k) p = fma(p, z, c[k]);
+    return p;
+}
+
 __host__ __device__ inline double cos_deg(double lat_deg) {
     const double x = lat_deg * LM_DEG2RAD;
-    const double z = x * x;
-    // c_k = (-1)^k / (2k)!, k = 11 .. 0, as exactly rounded hex literals.
-    double p = -0x1.0ce396db7f853p-70;
-    p = fma(p, z, 0x1.e542ba4020225p-62);
-    p = fma(p, z, -0x1.6827863b97d97p-53);
-    p = fma(p, z, 0x1.ae7f3e733b81fp-45);
-    p = fma(p, z, -0x1.93974a8c07c9dp-37);
-    p = fma(p, z, 0x1.1eed8eff8d898p-29);
-    p = fma(p, z, -0x1.27e4fb7789f5cp-22);
-    p = fma(p, z, 0x1.a01a01a01a01ap-16);
-    p = fma(p, z, -0x1.6c16c16c16c17p-10);
-    p = fma(p, z, 0x1.5555555555555p-5);
-    p = fma(p, z, -0.5);
-    return fma(p, z, 1.0);
+    constexpr double c[12] = {LM_COS_COEFS};
+    return lm_horner<12>(c, x * x);
 }
 
 // sin(x * pi / 180) for |x| <= 90: Taylor series through x^17 (truncation
@@ -235,18 +243,8 @@ __host__ __device__ inline double cos_deg(double lat_deg) {
 // operations on CPU and GPU.
 __host__ __device__ inline double sin_deg(double deg) {
     const double x = deg * LM_DEG2RAD;
-    const double z = x * x;
-    // s_k = (-1)^k / (2k+1)!, k = 8 .. 0, as exactly rounded hex literals.
-    double p = 0x1.952c77030ad4ap-49;
-    p = fma(p, z, -0x1.ae7f3e733b81fp-41);
-    p = fma(p, z, 0x1.6124613a86d09p-33);
-    p = fma(p, z, -0x1.ae64567f544e4p-26);
-    p = fma(p, z, 0x1.71de3a556c734p-19);
-    p = fma(p, z, -0x1.a01a01a01a01ap-13);
-    p = fma(p, z, 0x1.1111111111111p-7);
-    p = fma(p, z, -0x1.5555555555555p-3);
-    p = fma(p, z, 1.0);
-    return p * x;
+    constexpr double c[9] = {LM_SIN_COEFS};
+    return lm_horner<9>(c, x * x) * x;
 }
 
 }  // namespace lm
