@@ -77,6 +77,8 @@ def parse():
                     help="also time MicrobeSimulator at 490k with and without the output files (writes ~6 GB)")
     ap.add_argument("--fp32-steps", type=int, default=2880)
     ap.add_argument("--cpu-steps", type=int, default=48)
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint leg and print its object (for trace and counter profiles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel on BYTES bytes 10 times (byte-counter calibration)")
     return ap.parse_args()
@@ -516,6 +518,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.adjoint_only:
+        print(json.dumps(adjoint_leg(args.adjoint_reps)), flush=True)
         return
     w = configs.WORKLOADS[args.workload]()
     lon, lat = w.particles()
