@@ -202,13 +202,11 @@ struct Lin {
     int cell;  // (j << 16) | i
 };
 
-// The float32 levels as the backward reads them.  Blending corners from the
-// levels (1 MB for 20 levels of 81 x 81 nodes) instead of reading float64
-// stage fields (3.5 MB for a 16-step segment) keeps the reads in L2.
+// The float32 levels as the backward reads them.  Corners gathered from the
+// levels (1 MB for 20 levels of 81 x 81 nodes) rather than from float64
+// stage fields (3.5 MB for a 16-step segment) stay in L2.
 struct LevelView {
     const float2* glev;  // [nlev][ny][nx]
-    const int32_t* lev0;
-    const double* w;
     int64_t level_len;
     int64_t gstride;  // level_len, or 0 for a steady field
     lm_grid g;
@@ -877,8 +875,6 @@ int lm_adjoint_backward(int64_t n, const double* checkpoints, const float* field
     const double hh = 0.5 * dt, h6 = dt / 6.0;
     lm::LevelView V{};
     V.glev = reinterpret_cast<const float2*>(field);
-    V.lev0 = lev0;
-    V.w = w;
     V.level_len = level_len_of(grid);
     V.gstride = nlev > 1 ? V.level_len : 0;
     V.g = *grid;
