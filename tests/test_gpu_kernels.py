@@ -5,7 +5,7 @@ import torch
 
 import oracle
 from oracle.pairs import sorted_kdtree_pairs
-from helpers import grid_of, oracle_advect
+from helpers import oracle_advect
 from lagrangian_microbes_amd import configs, device as dev
 from lagrangian_microbes_amd.velocity_fields import VelocityField
 

@@ -10,7 +10,7 @@ import torch
 
 import oracle
 from oracle.pairs import sorted_kdtree_pairs
-from helpers import oracle_advect, oracle_two_pass
+from helpers import oracle_two_pass
 from lagrangian_microbes_amd import (InteractionSimulator, MicrobeSimulator, ParticleAdvecter, configs,
                                      rock_paper_scissors)
 from lagrangian_microbes_amd.netcdf_io import read_particle_dataset

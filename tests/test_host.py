@@ -3,7 +3,6 @@ output formats and the host rule.  No GPU."""
 import os
 from datetime import datetime, timedelta
 
-import joblib
 import numpy as np
 import pytest
 
