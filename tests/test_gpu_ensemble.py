@@ -11,8 +11,10 @@ import numpy as np
 import pytest
 import torch
 
+import oracle
 from lagrangian_microbes_amd import configs, device as dev
 from lagrangian_microbes_amd.simulator import FusedRunner
+from oracle.pairs import sorted_kdtree_pairs
 
 pytestmark = pytest.mark.gpu
 
@@ -126,13 +128,14 @@ def test_full_size_ensemble_matches_runs_alone(gpu):
     sizes = np.array([w.n for w in e.runs])
     order = np.argsort(sizes, kind="stable")
     picks = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
-    sel = {q: ([], []) for q in picks}
+    sel = {q: ([], [], []) for q in picks}
 
     def keep(k, rx, ry, rs, cnt):
         for q in picks:
             sl = runs.slice(q)
             sel[q][0].append(rx[:, sl].cpu().numpy())
             sel[q][1].append(rs[:, sl].cpu().numpy())
+            sel[q][2].append(ry[:, sl].cpu().numpy())
 
     r.run(steps, keep)
     r.check_domain()
@@ -148,3 +151,15 @@ def test_full_size_ensemble_matches_runs_alone(gpu):
         changed += int(np.sum(rs[-1] != w.species()))
     print(f"[full-size ensemble] runs {picks}: {pairs} pairs, {changed} species changed", flush=True)
     assert pairs > 0 and changed > 0  # the runs do interact
+    # and the batched run's last step of its smallest and largest runs against the oracle, on the
+    # run's own records: pairs of the float32 positions against cKDTree, the species against the
+    # sequential update of the step before
+    s, checked = steps - 1, 0
+    for q in (picks[0], picks[-1]):
+        w = e.runs[q]
+        X, S, Y = (np.concatenate(a) for a in sel[q])
+        pi, pj = sorted_kdtree_pairs(X[s], Y[s], w.radius, w.norm)
+        want = oracle.rps_sequential(S[s - 1], pi, pj, w.p, w.p, w.p, w.key, s)
+        assert np.array_equal(S[s], want), q
+        checked += len(pi)
+    assert checked > 0
