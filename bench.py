@@ -506,8 +506,8 @@ def adjoint_leg(reps):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path's multi-GPU sharding "
-              "is described in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the path is built for one GPU, "
+              "its sharding across GPUs is not", file=sys.stderr)
         sys.exit(2)
 
     import torch
