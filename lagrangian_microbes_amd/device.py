@@ -35,7 +35,10 @@ class DeviceField:
     time-blended fields a run of RK4 steps samples (t0, t0+dt/2, ..., t0+n dt),
     or the single level of a steady field, with the stride to pass to the
     advection; ``advect(...)`` blends and advects with the kernel of the
-    field's precision.
+    field's precision.  ``advect_steps(clock, ...)`` does the same for steps
+    clock .. clock + nsteps - 1 of dt from the field's first level, from a
+    schedule of every stage time of the field's span kept on the device
+    (nothing uploaded per call).
     """
 
     def __init__(self, field, device=None, precision="f64"):
@@ -50,6 +53,7 @@ class DeviceField:
         self.level_len = int(np.prod(inter.shape[1:]))
         self.grid = LmGrid(field.nx, field.ny, field.x0, field.y0, 1.0 / field.dx, 1.0 / field.dy)
         self._stage_buf = None
+        self._span = None  # (dt, lev0 host, lev0 device, w device): the stage schedule of the whole span
 
     @property
     def steady(self):
@@ -79,6 +83,53 @@ class DeviceField:
         run = advect_rk4_f64 if self.precision == "f64" else advect_rk4_f32
         return run(lon, lat, stage, stride, self.grid, nsteps, dt, rec_lon, rec_lat, rec_stride, walk_amp, key,
                    step0, oob)
+
+    SPAN_STAGES_MAX = 1 << 22  # largest span schedule kept on the device (48 MiB with its host copy)
+
+    def _span_schedule(self, dt):
+        """Stage schedule of every half step of dt in the field's span (stage q
+        at q dt/2 from the first level), on the host and the device, built once
+        per dt; None when it would exceed SPAN_STAGES_MAX stages."""
+        if self._span is None or self._span[0] != dt:
+            if self._span is not None:  # launches on any stream may still read the old schedule
+                torch.cuda.synchronize(self.device)
+            span = float(self.field.times[-1] - self.field.times[0])
+            nsteps = int(span // dt)
+            if 2 * nsteps + 1 > self.SPAN_STAGES_MAX:
+                self._span = (dt, None)
+            else:
+                lev0, w = self.field.stage_schedule(0.0, dt, nsteps)
+                lev0 = np.ascontiguousarray(lev0, dtype=np.int32)
+                self._span = (dt, (lev0, torch.from_numpy(lev0).to(self.device),
+                                   torch.from_numpy(np.ascontiguousarray(w, dtype=np.float64)).to(self.device)))
+        return self._span[1]
+
+    def advect_steps(self, lon, lat, clock, dt, nsteps, rec_lon=None, rec_lat=None, rec_stride=None,
+                     walk_amp=0.0, key=(0, 0), step0=0, oob=None):
+        """advect() over steps clock .. clock + nsteps - 1 (t0 = clock * dt on
+        the field's axis): the stage times are whole half steps of dt, so their
+        schedule is a slice of the span's, already on the device.  The slice is
+        taken when 2 dt is a whole number of seconds: then clock * dt + k dt/2
+        and (2 clock + k) dt/2 are both exact, and the weights are bit for bit
+        those advect() computes; other steps, and times past the span (which
+        raise as stage_schedule does), go through advect()."""
+        q0, q1 = 2 * int(clock), 2 * (int(clock) + int(nsteps)) + 1
+        sched = None
+        if not self.steady and dt > 0 and float(2.0 * dt).is_integer() and q1 * dt < 2.0 ** 50:
+            sched = self._span_schedule(dt)
+        if sched is None or q0 < 0 or q1 > sched[0].size:
+            return self.advect(lon, lat, clock * dt, dt, nsteps, rec_lon, rec_lat, rec_stride, walk_amp, key,
+                               step0, oob)
+        lev0_h, lev0_d, w_d = sched
+        need = (q1 - q0) * self.level_len
+        if self._stage_buf is None or self._stage_buf.numel() < need:
+            self._stage_buf = torch.empty(need, dtype=self.dtype, device=self.device)
+        out = self._stage_buf[:need]
+        blend = field_blend_f64 if self.precision == "f64" else field_blend_f32
+        blend(self.levels, self.level_len, lev0_d[q0:q1], w_d[q0:q1], out, lev0_host=lev0_h[q0:q1])
+        run = advect_rk4_f64 if self.precision == "f64" else advect_rk4_f32
+        return run(lon, lat, out, self.level_len, self.grid, nsteps, dt, rec_lon, rec_lat, rec_stride, walk_amp,
+                   key, step0, oob)
 
 
 def field_blend_f64(levels, level_len, lev0, w, out, lev0_host=None):

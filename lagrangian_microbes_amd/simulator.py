@@ -112,8 +112,8 @@ class FusedRunner:
         k = min(int(nsteps), self.chunk)
         if k <= 0:
             return 0
-        self.dfield.advect(self.lon, self.lat, self.clock_steps * self.dt_s, self.dt_s, k, self.rec_x, self.rec_y,
-                           self.n, self.walk_amp, self.walk_key, self.step, self.oob)
+        self.dfield.advect_steps(self.lon, self.lat, self.clock_steps, self.dt_s, k, self.rec_x, self.rec_y,
+                                 self.n, self.walk_amp, self.walk_key, self.step, self.oob)
         self.engine.run_chunk(self.rec_x[:k], self.rec_y[:k], self.species, self.params, self.key, self.step,
                               rec_species=self.rec_s[:k], npairs=self.npairs[:k])
         self.step += k
@@ -182,8 +182,8 @@ class FusedRunner:
                 if released[b] is not None:
                     adv.wait_event(released[b])
                 rx, ry, _ = recs(c)
-                self.dfield.advect(self.lon, self.lat, clock0[c] * self.dt_s, self.dt_s, k, rx, ry, self.n,
-                                   self.walk_amp, self.walk_key, step0[c], self.oob)
+                self.dfield.advect_steps(self.lon, self.lat, clock0[c], self.dt_s, k, rx, ry, self.n,
+                                         self.walk_amp, self.walk_key, step0[c], self.oob)
                 ev = torch.cuda.Event()
                 ev.record(adv)
                 advected[c] = ev

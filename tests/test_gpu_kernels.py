@@ -62,6 +62,34 @@ def test_advect_random_walk_and_step_offset(gpu):
     assert np.array_equal(g[0], o[0]) and np.array_equal(g[1], o[1]) and np.array_equal(g[2], o[2])
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_advect_steps_matches_advect(gpu, precision):
+    """advect_steps (the stage schedule sliced from the span's, on the device)
+    against advect (schedule computed and uploaded per call), bit for bit, at
+    a clock offset; a step whose 2 dt is not whole takes the advect path; a
+    window past the span raises."""
+    w = configs.rps_490k_30day()
+    f = w.velocity_field()
+    lon, lat = w.particles()
+    lon, lat = lon[::977], lat[::977]
+    df = dev.DeviceField(f, precision=precision)
+    n = lon.size
+    for dt, clock, k in [(w.dt, 0, 10), (w.dt, 331, 17), (w.dt, 719 - 2, 2), (1234.25, 5, 6)]:
+        out = []
+        for fn, arg in [(df.advect, clock * dt), (df.advect_steps, clock)]:
+            x, y = T(lon.astype(np.float64)), T(lat.astype(np.float64))
+            rl = torch.empty((k, n), dtype=torch.float32, device="cuda")
+            rt = torch.empty((k, n), dtype=torch.float32, device="cuda")
+            oob = fn(x, y, arg, dt, k, rl, rt, n, 0.0, (0, 0), 3)
+            out.append([a.cpu().numpy() for a in (x, y, rl, rt)] + [int(oob.item())])
+        for a, b in zip(out[0], out[1]):
+            assert np.array_equal(a, b), (dt, clock, k)
+    span_steps = int((f.times[-1] - f.times[0]) // w.dt)
+    x, y = T(lon.astype(np.float64)), T(lat.astype(np.float64))
+    with pytest.raises(ValueError):
+        df.advect_steps(x, y, span_steps - 1, w.dt, 2)
+
+
 def test_advect_out_of_grid(gpu):
     lons = np.arange(200.0, 220.01, 0.5)
     lats = np.arange(20.0, 40.01, 0.5)
