@@ -85,7 +85,7 @@ enum KernelId {
     K_BLEND_F32,
     K_SPECIES_COUNT,
     K_DIST_HIST,
-    K_RPS_HOOK,
+    K_RPS_TILE,
     K_NUM_KERNELS
 };
 bool timing_on();

@@ -7,21 +7,21 @@
 //
 // Everything that depends only on the pairs runs once for a batch of steps
 // (blockIdx.y = step), with the parallelism of all steps at once:
-//   prepare 1. init      parent[v] = v; touched marks and rows cleared
-//           1b. hook     parent[j] = i for every pair (i < j), plain stores: when
-//                        several pairs share j one of them lands, any of them a
-//                        smaller member of j's component
+//   prepare 1. init      touched marks and rows cleared
+//           1b. tile     per tile of 2^14 consecutive particles, a union-find
+//                        in LDS over the pairs with both ends in the tile;
+//                        parent[v] = v's tile-local root
 //           2. union     lock-free union-find, hooking the larger root under the
 //                        smaller, so every root is the smallest index of its
-//                        component; a pair whose j still points at its i is
-//                        already joined and skips it; the same pass draws each
+//                        component, over the pairs the tile pass left (those
+//                        that cross tiles); the same pass draws each
 //                        pair's uniform
 //                        (Philox keyed by pair rank and step), keeps its three
 //                        comparisons with pRS, pPR, pSP as bits and records
 //                        where each row (the run of pairs with one i) begins
 //                        and ends
 //           3. keys      which particles have pairs (a row of their own, or a
-//                        parent other than themselves: every j was hooked) and
+//                        parent other than themselves: every j ends below itself) and
 //                        each such particle's root (any other particle: its
 //                        own index, a run of one that is never listed)
 //           4. group     particles grouped by root, stably, in two levels (a
@@ -178,35 +178,114 @@ rps_init_kernel(int64_t n, RpsArrays A, RpsStrides st) {
     const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     const int64_t s = blockIdx.y, ro = s * st.row;
     if (v < n) {
-        A.parent[ro + v] = (uint32_t)v;
         A.touched[ro + v] = 0;
         A.rows[ro + v] = make_uint2(0u, 0u);
     }
     if (v < 4) A.lists[4 * s + v] = 0;
 }
 
-// Before the union-find: every pair points its larger index at its smaller
-// one (j at i, as lists have i < j) with a plain store.  Pairs sharing a j
-// race and one of their stores lands; each is a
-// smaller particle of j's component, so the forest stays valid (parents only
-// point down to smaller indices of the same component) and most pairs are
-// joined before the union-find runs.  Stores rather than atomics: at a few
-// hundred thousand pairs per step, a device-scope atomic per pair costs about
-// as much as the union-find work it saves.
-__global__ void __launch_bounds__(kBlock)
-rps_hook_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __restrict__ pj_all,
-                const int64_t* __restrict__ npairs, RpsArrays A, RpsStrides st) {
-    uint32_t bx, by;
-    xcd_block(&bx, &by);  // an XCD works through a contiguous range of steps: one step's arrays in its L2
-    const int64_t s = by, ro = s * st.row, po = s * st.pair;
-    const int64_t P = step_pairs(npairs, s, st.pair);
-    const int32_t* pi = pi_all + po;
-    const int32_t* pj = pj_all + po;
-    uint32_t* parent = A.parent + ro;
-    for (int64_t k = (int64_t)bx * kBlock + threadIdx.x; k < P; k += (int64_t)gridDim.x * kBlock) {
-        const uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
-        if (a < (uint64_t)n && b < (uint64_t)n && a != b) parent[max(a, b)] = min(a, b);
+// Union-find by tile before the global one: the particles are cut into
+// tiles of kTile consecutive indices, and a workgroup joins the pairs whose
+// two ends lie in its tile in an LDS forest (a pair search's neighbours are
+// spatially close and so, on a lattice start, mostly close in index).  It
+// then points every particle of the tile at its tile-local root, the
+// smallest index of its part of the component, which replaces the parent
+// set-up of the init pass.  The global pass joins only the pairs that cross
+// tiles.  Every link still goes from a larger root to a smaller one, so the
+// final roots are the smallest indices of the components, and every pair's j
+// ends below itself (its component holds i < j), which is how the keys pass
+// tells that j has pairs.
+constexpr int kTileBits = 14;
+constexpr uint32_t kTile = 1u << kTileBits;  // particles per tile: 64 KB of LDS parents
+constexpr int kTileBlock = 1024;
+
+__device__ __forceinline__ uint32_t lds_find(uint32_t* par, uint32_t v) {
+    // relaxed workgroup-scope accesses: other lanes move pointers meanwhile,
+    // only ever to smaller indices of the same component
+    uint32_t p = __hip_atomic_load(par + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    while (p != v) {
+        const uint32_t gp = __hip_atomic_load(par + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (gp != p) __hip_atomic_store(par + v, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        v = p;
+        p = gp;
     }
+    return v;
+}
+
+__device__ __forceinline__ void lds_unite(uint32_t* par, uint32_t a, uint32_t b) {
+    for (;;) {
+        a = lds_find(par, a);
+        b = lds_find(par, b);
+        if (a == b) return;
+        if (a < b) {
+            const uint32_t t = a;
+            a = b;
+            b = t;
+        }
+        uint32_t expect = a;
+        if (__hip_atomic_compare_exchange_strong(par + a, &expect, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP))
+            return;
+        a = expect;  // hooked meanwhile, below a
+    }
+}
+
+// off[s][t] = first k of step s's list with pair_i[k] >= t * kTile (t <=
+// ntiles; the list is sorted by i).
+__device__ __forceinline__ uint32_t first_pair_at_or_above(const int32_t* pi, int64_t P, int64_t v) {
+    int64_t lo = 0, hi = P;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)pi[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return (uint32_t)lo;
+}
+
+__global__ void __launch_bounds__(kBlock)
+rps_tile_offsets_kernel(int nsteps, int ntiles1, const int32_t* __restrict__ pi_all,
+                        const int64_t* __restrict__ npairs, int64_t stride, uint32_t* __restrict__ off) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= (int64_t)nsteps * ntiles1) return;
+    const int64_t s = t / ntiles1;
+    const int64_t tile = t - s * ntiles1;
+    int64_t P = step_pairs(npairs, s, stride);
+    if (P < 0) P = 0;
+    off[t] = first_pair_at_or_above(pi_all + s * stride, P, tile * (int64_t)kTile);
+}
+
+__global__ void __launch_bounds__(kTileBlock)
+rps_tile_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __restrict__ pj_all,
+                      const int64_t* __restrict__ npairs, RpsArrays A, RpsStrides st,
+                      const uint32_t* __restrict__ off, int ntiles1) {
+    __shared__ uint32_t par[kTile];
+    const int64_t s = blockIdx.y, ro = s * st.row, po = s * st.pair;
+    const uint32_t tile = blockIdx.x, base = tile << kTileBits;
+    for (uint32_t v = threadIdx.x; v < kTile; v += kTileBlock) par[v] = v;
+    __syncthreads();
+    const int64_t P = step_pairs(npairs, s, st.pair);
+    if (P > 0) {  // an overflowed step keeps every particle apart; its replay is skipped
+        const uint32_t k0 = off[s * ntiles1 + tile], k1 = off[s * ntiles1 + tile + 1];
+        const int32_t* pi = pi_all + po;
+        const int32_t* pj = pj_all + po;
+        for (uint32_t k = k0 + threadIdx.x; k < k1; k += kTileBlock) {
+            const uint32_t a = (uint32_t)pi[k], b = (uint32_t)pj[k];
+            if (a < (uint64_t)n && b < (uint64_t)n && a != b && (a >> kTileBits) == tile && (b >> kTileBits) == tile)
+                lds_unite(par, a - base, b - base);
+        }
+    }
+    __syncthreads();
+    uint32_t* parent = A.parent + ro;
+    for (uint32_t v = threadIdx.x; v < kTile; v += kTileBlock)
+        if ((uint64_t)base + v < (uint64_t)n) parent[base + v] = base + lds_find(par, v);
+}
+
+// The tile pass joined pair k (its ends in one tile, k in that tile's range
+// of the list).
+__device__ __forceinline__ bool tile_joined(const uint32_t* __restrict__ off_s, uint32_t a, uint32_t b, int64_t k) {
+    if ((a >> kTileBits) != (b >> kTileBits)) return false;
+    const uint32_t t = a >> kTileBits;
+    return (uint32_t)k >= off_s[t] && (uint32_t)k < off_s[t + 1];
 }
 
 // Runs: an ensemble of independent runs sharing the index range (run r holds
@@ -224,11 +303,13 @@ template <bool RUNS>
 __global__ void __launch_bounds__(kBlock)
 rps_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __restrict__ pj_all,
                  const int64_t* __restrict__ npairs, RpsArrays A, RpsStrides st, double pRS, double pPR,
-                 double pSP, uint64_t k0, uint64_t k1, uint64_t step0, RunsView R) {
+                 double pSP, uint64_t k0, uint64_t k1, uint64_t step0, RunsView R,
+                 const uint32_t* __restrict__ toff, int ntiles1) {
     uint32_t bx, by;
     xcd_block(&bx, &by);  // an XCD works through a contiguous range of steps: one step's arrays in its L2
     const int64_t s = by, ro = s * st.row, po = s * st.pair;
     const int64_t P = step_pairs(npairs, s, st.pair);
+    const uint32_t* off_s = toff + s * ntiles1;
     if (bx == 0 && threadIdx.x == 0) A.plen[s] = P < 0 ? 0 : P;
     const int32_t* pi = pi_all + po;
     const int32_t* pj = pj_all + po;
@@ -255,22 +336,17 @@ rps_union_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* _
         const double u = u01_from_u64(philox4x64_10(rank, step0 + (uint64_t)s, kStreamPairs, 0, q0, q1).v[0]);
         const uint8_t fbits = (uint8_t)((u < a_rs ? 1u : 0u) | (u < a_pr ? 2u : 0u) | (u < a_sp ? 4u : 0u));
         lm_pin(a, b, before, after);
-        const uint32_t hi = max(a, b), lo = min(a, b);
-        // parent[b] == a was set by the hook pass, before this kernel: joins
-        // are permanent, so a pair seen joined at any time is joined
-        const uint32_t phi = parent[hi < (uint64_t)n ? hi : 0u];
         bits[k] = fbits;
         if (a < (uint64_t)n) {
             uint32_t* row = reinterpret_cast<uint32_t*>(A.rows + ro + a);  // {begin, end}
             if (k == 0 || before != a) row[0] = (uint32_t)k;
             if (k + 1 == P || after != a) row[1] = (uint32_t)(k + 1);
-            // a has its row and the hook pointed b below a, which is how the
-            // keys pass tells both have pairs; only a j below its i (never
-            // in a list from the pair search) is marked here
+            // a has its row and b ends below itself (joined to a < b), which
+            // is how the keys pass tells both have pairs; only a j below its
+            // i (never in a list from the pair search) is marked here
             if (b < (uint64_t)n) {
                 if (b < a) A.touched[ro + b] = 1;
-                // parent[b] == a: b hangs below a (sets only merge), nothing to join
-                if (a != b && phi != lo) unite(parent, a, b);
+                if (a != b && !tile_joined(off_s, a, b, k)) unite(parent, a, b);
             }
         }
     }
@@ -328,7 +404,7 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
         const int64_t v = v0 + (int64_t)k * kBlock;
         const bool live = v < n;
         // a particle has pairs if it has a row (as some pair's i) or a parent
-        // below it (as some pair's j: hooked, and parents only move down);
+        // below it (as some pair's j: joined below itself, and parents only move down);
         // one without pairs is keyed by itself: a run of one, never listed
         const uint32_t p = live ? pv[k] : 0u;
         t[k] = live && ((p != (uint32_t)v) | (ry[k] != 0u) | (tv[k] != 0u));
@@ -342,7 +418,7 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
         if (v < n) A.touched[ro + v] = t[k];  // for the replay's copy role
     }
     // the roots by read-only walks (parents point to smaller indices, so they
-    // end), all of a thread's at once: the trees are shallow after the hook
+    // end), all of a thread's at once: the trees are shallow after the tile pass
     // pass and the union-find, and path halving here would add scattered stores
     for (;;) {
         // q[k] is always a particle index (a parent, or 0 past the end), so
@@ -1169,6 +1245,8 @@ struct lm_rps_ws {
     int32_t last_steps = 0;   // steps of the last prepare
     int64_t pair_stride_used = 0;  // per-step pair stride of the last prepare
     uint32_t* run_off = nullptr;   // [steps][runs] first pair of each run (ensembles)
+    uint32_t* tile_off = nullptr;  // [steps][tiles1] first pair of each particle tile
+    int tiles1 = 0;                // tiles of n_max particles, plus one
     int64_t run_off_cap = 0;
     unsigned play_wide = lm::kPlayWideBlocks;      // replay workgroups on wide components
     unsigned play_narrow = lm::kPlayNarrowBlocks;  // replay workgroups on small components (64 each)
@@ -1218,6 +1296,7 @@ static void rps_ws_free(lm_rps_ws* w) {
     (void)hipFree(A.depth);
     lm::sort_scratch_free(&w->sort);
     (void)hipFree(w->run_off);
+    (void)hipFree(w->tile_off);
     rps_free_pairs(w);
     delete w;
 }
@@ -1262,6 +1341,7 @@ int lm_rps_ws_create(int64_t n_max, int32_t steps_max, lm_rps_ws** out) {
     w->n_max = n_max;
     w->steps_max = steps_max;
     w->row = lm::scan_row_stride(n_max);
+    w->tiles1 = (int)((n_max + lm::kTile - 1) / lm::kTile) + 1;
     const int64_t S = steps_max, R = S * w->row;
     lm::RpsArrays& A = w->A;
     int rc;
@@ -1276,6 +1356,7 @@ int lm_rps_ws_create(int64_t n_max, int32_t steps_max, lm_rps_ws** out) {
         (rc = rws_alloc(&A.wide, R, &w->bytes)) || (rc = rws_alloc(&A.lists, 4 * S, &w->bytes)) ||
         (rc = rws_alloc(&A.plen, S, &w->bytes)) || (rc = rws_alloc(&A.depth, R, &w->bytes)) ||
         (rc = rws_alloc(&w->scan_tmp, S * lm::scan_num_blocks(n_max), &w->bytes)) ||
+        (rc = rws_alloc(&w->tile_off, S * w->tiles1, &w->bytes)) ||
         (rc = lm::sort_scratch_alloc(&w->sort, n_max, steps_max))) {
         rps_ws_free(w);
         return rc;
@@ -1352,8 +1433,13 @@ static int rps_prepare_impl(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int3
         // step has fewer pairs than particles but in dense runs; the capacity,
         // often twice the particles, would launch idle blocks)
         const unsigned npb = lm::grid_for(pair_stride < n ? pair_stride : n);
-        LM_LAUNCH(lm::K_RPS_HOOK, lm::rps_hook_kernel, dim3(npb, S), dim3(lm::kBlock), 0, s, n, pair_i, pair_j,
-                  npairs, A, sp);
+        const int ntiles = (int)((n + lm::kTile - 1) / lm::kTile);
+        LM_REQUIRE(ntiles + 1 <= ws->tiles1, LM_ERR_ARG, "internal: tile offsets");
+        LM_LAUNCH(lm::K_RPS_TILE, lm::rps_tile_offsets_kernel, dim3(lm::grid_for((int64_t)S * (ntiles + 1))),
+                  dim3(lm::kBlock), 0, s, (int)S, ntiles + 1, pair_i, npairs, pair_stride, ws->tile_off);
+        LM_LAUNCH_CHECK();
+        LM_LAUNCH(lm::K_RPS_TILE, lm::rps_tile_union_kernel, dim3(ntiles, S), dim3(lm::kTileBlock), 0, s, n, pair_i,
+                  pair_j, npairs, A, sp, (const uint32_t*)ws->tile_off, ntiles + 1);
         LM_LAUNCH_CHECK();
         lm::RunsView R{run, params, nullptr, nruns};
         if (run != nullptr) {
@@ -1370,10 +1456,12 @@ static int rps_prepare_impl(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int3
             LM_LAUNCH_CHECK();
             R.off = ws->run_off;
             LM_LAUNCH(lm::K_RPS_UNION, lm::rps_union_kernel<true>, dim3(npb, S), dim3(lm::kBlock), 0, s, n, pair_i,
-                      pair_j, npairs, A, sp, pRS, pPR, pSP, key0, key1, step0, R);
+                      pair_j, npairs, A, sp, pRS, pPR, pSP, key0, key1, step0, R, (const uint32_t*)ws->tile_off,
+                      ntiles + 1);
         } else {
             LM_LAUNCH(lm::K_RPS_UNION, lm::rps_union_kernel<false>, dim3(npb, S), dim3(lm::kBlock), 0, s, n, pair_i,
-                      pair_j, npairs, A, sp, pRS, pPR, pSP, key0, key1, step0, R);
+                      pair_j, npairs, A, sp, pRS, pPR, pSP, key0, key1, step0, R, (const uint32_t*)ws->tile_off,
+                      ntiles + 1);
         }
         LM_LAUNCH_CHECK();
         // grouping by root in two levels: a radix pass on the root's high
