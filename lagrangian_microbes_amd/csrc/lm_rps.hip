@@ -8,7 +8,7 @@
 // Everything that depends only on the pairs runs once for a batch of steps
 // (blockIdx.y = step), with the parallelism of all steps at once:
 //   prepare 1. init      touched marks and rows cleared
-//           1b. tile     per tile of 2^14 consecutive particles, a union-find
+//           1b. tile     per tile of 2^15 consecutive particles, a union-find
 //                        in LDS over the pairs with both ends in the tile;
 //                        parent[v] = v's tile-local root
 //           2. union     lock-free union-find, hooking the larger root under the
@@ -195,8 +195,8 @@ rps_init_kernel(int64_t n, RpsArrays A, RpsStrides st) {
 // final roots are the smallest indices of the components, and every pair's j
 // ends below itself (its component holds i < j), which is how the keys pass
 // tells that j has pairs.
-constexpr int kTileBits = 14;
-constexpr uint32_t kTile = 1u << kTileBits;  // particles per tile: 64 KB of LDS parents
+constexpr int kTileBits = 15;
+constexpr uint32_t kTile = 1u << kTileBits;  // particles per tile: 128 KB of LDS parents
 constexpr int kTileBlock = 1024;
 
 __device__ __forceinline__ uint32_t lds_find(uint32_t* par, uint32_t v) {
