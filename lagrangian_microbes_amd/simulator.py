@@ -171,8 +171,13 @@ class FusedRunner:
         prepared = [None] * len(sizes)   # event after chunk c's preparation
         played = [None] * nsets          # event after the replay that last used set b
         released = [None] * nsets        # event the callback returned for set b's last chunk
-        side.wait_stream(main)           # the state as the caller left it
-        adv.wait_stream(main)
+        # The events are the runner's own, recorded again every chunk: a set's
+        # event is re-recorded only after every wait on its previous record
+        # has been issued (chunk c + 3 reuses chunk c's set)
+        ev_adv, ev_prep, ev_play, ev_count, ev_join = self._events
+        ev_join[0].record(main)          # the state as the caller left it
+        side.wait_event(ev_join[0])
+        adv.wait_event(ev_join[0])
 
         def advect(c):
             b, k = c % nsets, sizes[c]
@@ -184,7 +189,7 @@ class FusedRunner:
                 rx, ry, _ = recs(c)
                 self.dfield.advect_steps(self.lon, self.lat, clock0[c], self.dt_s, k, rx, ry, self.n,
                                          self.walk_amp, self.walk_key, step0[c], self.oob)
-                ev = torch.cuda.Event()
+                ev = ev_adv[b]
                 ev.record(adv)
                 advected[c] = ev
 
@@ -194,7 +199,7 @@ class FusedRunner:
             with torch.cuda.stream(side):
                 side.wait_event(advected[c])
                 eng.prepare_set(b, rx, ry, self.params, self.key, step0[c])
-                ev = torch.cuda.Event()
+                ev = ev_prep[b]
                 ev.record(side)
                 prepared[c] = ev
 
@@ -222,7 +227,7 @@ class FusedRunner:
             # starting species.
             backup.copy_(self.species)
             worst_host.copy_(eng.npairs_sets[b][:k].max(), non_blocking=True)
-            counted = torch.cuda.Event()
+            counted = ev_count
             counted.record(main)
             eng.play_set(b, k, self.species, rs)
             counted.synchronize()
@@ -240,14 +245,16 @@ class FusedRunner:
             ret = None
             if on_chunk is not None:
                 ret = on_chunk(k, rx, ry, rs, eng.npairs_sets[b][:k])
-            ev = torch.cuda.Event()  # after the callback's own work on this stream
+            ev = ev_play[b]  # after the callback's own work on this stream
             ev.record(main)
             played[b] = ev
             released[b] = ret if isinstance(ret, torch.cuda.Event) else None
             self.step += k
             self.clock_steps += k
-        main.wait_stream(side)
-        main.wait_stream(adv)
+        ev_join[1].record(side)
+        ev_join[2].record(adv)
+        main.wait_event(ev_join[1])
+        main.wait_event(ev_join[2])
         return total
 
     def _ensure_pipeline(self):
@@ -259,6 +266,10 @@ class FusedRunner:
             self._adv = torch.cuda.Stream(self.device)
             self._backup = torch.empty_like(self.species)  # a chunk's starting species
             self._worst_host = torch.zeros((), dtype=torch.int64, pin_memory=True)
+            sets = range(self.PIPELINE_SETS)
+            self._events = ([torch.cuda.Event() for _ in sets], [torch.cuda.Event() for _ in sets],
+                            [torch.cuda.Event() for _ in sets], torch.cuda.Event(),
+                            [torch.cuda.Event() for _ in range(3)])
 
     def check_domain(self):
         n_oob = int(self.oob.item())
