@@ -91,7 +91,8 @@ struct RpsArrays {
     uint8_t* bits;                        // per pair (by rank)
     uint2* rec;                           // per pair (run order)
     unsigned long long* deps;             // per pair (run order, wide components)
-    uint2* lrec;                          // per pair: wide components' records in level order
+    uint2* lrec;                          // per pair: wide components' records in level order,
+                                          // {slot_i, slot_j | win_table << 16}
     uint32_t* lend;                       // per pair: end of each level, from a component's first pair
     uint32_t* depth;                      // per wide component: levels, 0 = windowed replay
 };
@@ -153,6 +154,24 @@ __device__ __forceinline__ void play_pair(int& si, int& sj, uint32_t bits) {
     const int nj = valid && i_wins ? si : sj;
     si = ni;
     sj = nj;
+}
+
+// A pair's outcome as a table: bit 3 (si - 1) + (sj - 1), for si, sj in
+// {1, 2, 3}, is set when play_pair leaves both particles with si (i wins, or
+// a like pair).  Levelled records carry it (slot_j | table << 16), so that a
+// row of the replay, whose chain of dependent operations sets the time of a
+// level, decides by one table lookup.
+__device__ __forceinline__ uint32_t win_table(uint32_t bits) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int si = 1; si <= 3; ++si)
+#pragma unroll
+        for (int sj = 1; sj <= 3; ++sj) {
+            int a = si, b = sj;
+            play_pair(a, b, bits);
+            if (a == si && b == si) t |= 1u << (3 * (si - 1) + (sj - 1));
+        }
+    return t;
 }
 
 // Wavefront-aggregated append of v to list (one atomic per wavefront); every
@@ -872,7 +891,9 @@ rps_levels_kernel(int64_t n, RpsArrays A, RpsStrides st) {
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u)
-                if (pos[u] < np) A.lrec[po + qb + pos[u]] = rv[u];
+                if (pos[u] < np)
+                    A.lrec[po + qb + pos[u]] =
+                        make_uint2(rv[u].x & kSlotMask, min(rv[u].y, 0xffffu) | (win_table(rv[u].x >> 29) << 16));
         }
         if (lane == 0) A.depth[ro + c] = dmax;
         __syncthreads();
@@ -902,6 +923,19 @@ __device__ __forceinline__ void play_slots(int8_t* slots, uint2 r, uint32_t top)
     play_pair(si, sj, r.x >> 29);
     slots[a] = (int8_t)si;
     slots[b] = (int8_t)sj;
+}
+
+// play_slots on a levelled record {slot_i, slot_j | table << 16}: the same
+// result as play_pair (species outside {1, 2, 3} never interact).
+__device__ __forceinline__ void play_slots_table(int8_t* slots, uint2 r, uint32_t top) {
+    const uint32_t a = min(r.x, top), b = min(r.y & 0xffffu, top);
+    const int si = slots[a], sj = slots[b];
+    const uint32_t ui = (uint32_t)(si - 1), uj = (uint32_t)(sj - 1);
+    const bool valid = (ui < 3u) & (uj < 3u);
+    const uint32_t i_wins = (r.y >> (16u + ((3u * ui + uj) & 15u))) & 1u;  // read only when valid
+    const int w = i_wins ? si : sj;
+    slots[a] = (int8_t)(valid ? w : si);
+    slots[b] = (int8_t)(valid ? w : sj);
 }
 
 // Levelled component: species in LDS by slot; the pairs of a level share no
@@ -952,7 +986,7 @@ __device__ __forceinline__ void play_rows(int8_t* slots, const uint32_t (&cnt)[k
     const uint32_t lane = threadIdx.x;
 #pragma unroll
     for (int q = 0; q < kRowBatch; ++q) {
-        if (lane < cnt[q]) play_slots(slots, rr[q], top);
+        if (lane < cnt[q]) play_slots_table(slots, rr[q], top);
         __builtin_amdgcn_wave_barrier();
     }
 }
