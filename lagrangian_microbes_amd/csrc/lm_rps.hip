@@ -1469,7 +1469,7 @@ static int rps_prepare_impl(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int3
         const unsigned npb = lm::grid_for(pair_stride < n ? pair_stride : n);
         const int ntiles = (int)((n + lm::kTile - 1) / lm::kTile);
         LM_REQUIRE(ntiles + 1 <= ws->tiles1, LM_ERR_ARG, "internal: tile offsets");
-        LM_LAUNCH(lm::K_RPS_TILE, lm::rps_tile_offsets_kernel, dim3(lm::grid_for((int64_t)S * (ntiles + 1))),
+        LM_LAUNCH(lm::K_RPS_TILE_OFFSETS, lm::rps_tile_offsets_kernel, dim3(lm::grid_for((int64_t)S * (ntiles + 1))),
                   dim3(lm::kBlock), 0, s, (int)S, ntiles + 1, pair_i, npairs, pair_stride, ws->tile_off);
         LM_LAUNCH_CHECK();
         LM_LAUNCH(lm::K_RPS_TILE, lm::rps_tile_union_kernel, dim3(ntiles, S), dim3(lm::kTileBlock), 0, s, n, pair_i,
