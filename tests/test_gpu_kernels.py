@@ -276,6 +276,50 @@ def test_rps_matches_sequential_oracle(gpu, seed, n, m, clustered):
     assert st.max_pairs == comp_pairs.max() and st.max_particles == comp_parts.max()
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+def test_rps_union_across_tiles(gpu, seed):
+    """More particles than one union-find tile (2^15): local pairs (some
+    across tile edges), long-range pairs, a chain that zigzags through every
+    tile from a small root, and pairs at the end of the partial last tile,
+    so most components are joined in a tile's LDS forest and the rest
+    through the global one.  Species (0-4, some outside {1, 2, 3}) bit-exact
+    against the sequential oracle; the partition against scipy."""
+    rng = np.random.default_rng(40 + seed)
+    n = 3 * 2 ** 15 + 4321
+    pairs = set()
+    a = rng.integers(0, n - 41, 30000)
+    for x, d in zip(a.tolist(), rng.integers(1, 41, a.size).tolist()):
+        pairs.add((x, x + d))
+    for x, y in rng.integers(0, n, (15000, 2)).tolist():
+        if x != y:
+            pairs.add((min(x, y), max(x, y)))
+    zig = [5 + 2 * k if k % 2 == 0 else n - 1 - 3 * k for k in range(60)]
+    for x, y in zip(zig[:-1], zig[1:]):
+        pairs.add((min(x, y), max(x, y)))
+    for x in range(n - 200, n - 1, 3):
+        pairs.add((x, n - 1))
+        pairs.add((x - 2 ** 15, x))
+    arr = np.array(sorted(pairs), dtype=np.int32)
+    pi, pj = arr[:, 0].copy(), arr[:, 1].copy()
+    m = pi.size
+    assert np.count_nonzero((pi >> 15) != (pj >> 15)) > 10000  # plenty of pairs cross tiles
+    sp = rng.integers(0, 5, n).astype(np.int8)
+    args = (0.55, 0.45, 0.65, (seed, 11), 123 + seed)
+    g, up = gpu_rps(sp, pi, pj, *args)
+    o = oracle.rps_sequential(sp, pi, pj, *args)
+    assert np.array_equal(g, o)
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    _, lab = connected_components(coo_matrix((np.ones(m), (pi, pj)), shape=(n, n)), directed=False)
+    touched = np.unique(np.concatenate([pi, pj]))
+    comp_pairs = np.bincount(lab[pi], minlength=n)
+    comp_parts = np.bincount(lab[touched], minlength=n)
+    st = up.component_stats()
+    assert st.pairs == m and st.touched == touched.size
+    assert st.components == np.count_nonzero(comp_pairs)
+    assert st.max_pairs == comp_pairs.max() and st.max_particles == comp_parts.max()
+
+
 @pytest.mark.parametrize("n", [10000, 20000])
 def test_rps_giant_chain_component(gpu, n):
     # one component of 20,000 pairs: a chain plus random chords (n = 20000:
