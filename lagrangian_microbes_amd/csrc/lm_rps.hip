@@ -723,27 +723,37 @@ rps_fill_kernel(int64_t n, const int32_t* __restrict__ pi_all, const int32_t* __
 // reads them back and keeps the earlier lanes' bits, then clears them (the
 // wavefront's LDS operations complete in program order, and every window
 // leaves the table zero).  Larger components compare each lane with the 63
-// before it by shuffles.
+// before it by shuffles.  Split like the levels pass below: for a batch of
+// many steps, components of up to kDepsSmallPairs pairs with a table of
+// kDepsSmallTable slots (2 KB of LDS, so that many blocks stay resident),
+// then the larger ones; for a few steps, one launch takes every component.
+constexpr int32_t kSplitSteps = 24;  // batches of at least this many steps split the deps and levels passes
+enum SizeClass { kAllComponents, kSmallComponents, kLargeComponents };
 constexpr uint32_t kDepsTable = 2048;
+constexpr uint32_t kDepsSmallPairs = 256;  // the first launch's components: 2 KB of table, more blocks resident
+constexpr uint32_t kDepsSmallTable = kDepsSmallPairs + 8;
 
+template <uint32_t TABLE, int CLASS>
 __global__ void __launch_bounds__(64)
 rps_deps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
     uint32_t bx, by;
     xcd_block(&bx, &by);  // an XCD works through a contiguous range of steps: one step's arrays in its L2
-    __shared__ unsigned long long mask[kDepsTable];
+    __shared__ unsigned long long mask[TABLE];
     const int64_t s = by, ro = s * st.row, po = s * st.pair;
     const uint32_t lane = threadIdx.x;
     const uint32_t cnt = A.lists[4 * s + 1];
     if (bx >= cnt) return;  // uniform: no component for this block
     const uint2* rec = A.rec + po;
     unsigned long long* deps = A.deps + po;
-    for (uint32_t t = lane; t < kDepsTable; t += 64) mask[t] = 0ull;
+    for (uint32_t t = lane; t < TABLE; t += 64) mask[t] = 0ull;
     __syncthreads();
     const unsigned long long bit = 1ull << lane;
     for (uint32_t c = bx; c < cnt; c += gridDim.x) {
         const uint4 dsc = A.wide[ro + c];
         const uint32_t qb = dsc.x, np = dsc.y, psz = dsc.w;
-        const bool table = psz <= kDepsTable;  // uniform over the wavefront
+        if ((CLASS == kSmallComponents && np > kDepsSmallPairs) || (CLASS == kLargeComponents && np <= kDepsSmallPairs))
+            continue;  // the other launch's component
+        const bool table = psz <= TABLE;  // uniform over the wavefront
         if (np == 0) continue;
         // the next window's records load while this one is processed
         uint2 r_next = rec[qb + min(lane, np - 1u)];
@@ -755,8 +765,8 @@ rps_deps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
             const uint32_t a = live ? (r.x & kSlotMask) : kUnset, b = live ? r.y : kUnset - 1u;
             unsigned long long d = 0ull;
             if (table) {
-                // slots are < psz <= kDepsTable; the clamp only keeps a bad record inside the table
-                const uint32_t ta = min(a, kDepsTable - 1u), tb = min(b, kDepsTable - 1u);
+                // slots are < psz <= TABLE; the clamp only keeps a bad record inside the table
+                const uint32_t ta = min(a, TABLE - 1u), tb = min(b, TABLE - 1u);
                 if (live) {
                     atomicOr(&mask[ta], bit);
                     atomicOr(&mask[tb], bit);
@@ -788,20 +798,27 @@ rps_deps_kernel(int64_t n, RpsArrays A, RpsStrides st) {
 // pairs counting-sorted by level.  Pairs of one level share no particle, so a
 // replay may run each level in parallel, levels in order.
 //
-// LDS: 16 KB, so that many blocks stay resident.  The per-slot last level
-// (uint16) is dead once the levels are known, so its words then hold the
-// per-level counts and cursors as packed uint16 pairs (level l in the
-// (l & 1)-th half of word l >> 1; every count and cursor is <= kLevelPairs,
-// so an add never carries into the other half).
+// For a batch of many steps, two launches of one kernel: components of up
+// to kLevelSmallPairs pairs with 1 KB of LDS per block, so that many blocks
+// stay resident, then the larger ones with 16 KB; for a few steps, one
+// launch takes every component (the larger components' long chains then
+// overlap the small ones instead of following them).  The per-slot last level (uint16) is dead once the
+// levels are known, so its words then hold the per-level counts and cursors
+// as packed uint16 pairs (level l in the (l & 1)-th half of word l >> 1;
+// every count and cursor is <= PAIRS, so an add never carries into the
+// other half).
 __device__ __forceinline__ uint32_t half_of(uint32_t word, uint32_t l) { return (word >> (16u * (l & 1u))) & 0xffffu; }
 
+constexpr uint32_t kLevelSmallPairs = 256;
+
+template <uint32_t PAIRS, int CLASS>
 __global__ void __launch_bounds__(64)
 rps_levels_kernel(int64_t n, RpsArrays A, RpsStrides st) {
     uint32_t bx, by;
     xcd_block(&bx, &by);  // an XCD works through a contiguous range of steps: one step's arrays in its L2
-    static_assert(kLevelPairs < 65536u, "level counts must fit 16 bits");
-    __shared__ uint32_t words[(kLevelSlots + 1) / 2];  // last[] during the level phase, then the packed counts
-    __shared__ uint16_t lvl[kLevelPairs];
+    static_assert(PAIRS < 65536u && PAIRS <= kLevelPairs, "level counts must fit 16 bits");
+    __shared__ uint32_t words[(PAIRS + 2) / 2];  // last[] during the level phase, then the packed counts
+    __shared__ uint16_t lvl[PAIRS];
     uint16_t* last = reinterpret_cast<uint16_t*>(words);
     const int64_t s = by, ro = s * st.row, po = s * st.pair;
     const uint32_t lane = threadIdx.x;
@@ -811,7 +828,9 @@ rps_levels_kernel(int64_t n, RpsArrays A, RpsStrides st) {
     for (uint32_t c = bx; c < ncomp; c += gridDim.x) {
         const uint4 dsc = A.wide[ro + c];
         const uint32_t qb = dsc.x, np = dsc.y, psz = dsc.w;
-        if (np > kLevelPairs || psz > kLevelSlots || psz == 0) {
+        if ((CLASS == kSmallComponents && np > PAIRS) || (CLASS == kLargeComponents && np <= kLevelSmallPairs))
+            continue;  // the other launch's component
+        if (np > PAIRS || psz > PAIRS + 1 || psz == 0) {
             if (lane == 0) A.depth[ro + c] = 0;  // windowed replay
             continue;
         }
@@ -851,7 +870,7 @@ rps_levels_kernel(int64_t n, RpsArrays A, RpsStrides st) {
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) dmax = max(dmax, (uint32_t)__shfl_xor(dmax, d, 64));
         __syncthreads();
-        // last[] is dead: its words become the per-level counts (dmax <= np <= kLevelPairs levels)
+        // last[] is dead: its words become the per-level counts (dmax <= np <= PAIRS levels)
         for (uint32_t t = lane; t < (dmax + 1u) / 2u; t += 64) words[t] = 0u;
         __syncthreads();
         for (uint32_t q = lane; q < np; q += 64) {
@@ -1427,9 +1446,27 @@ static int rps_prepare_tail(lm_rps_ws* ws, int64_t n, unsigned S, const int32_t*
     LM_LAUNCH_CHECK();
     LM_LAUNCH(lm::K_RPS_FILL, lm::rps_fill_kernel, dim3(npb, S), dim3(lm::kBlock), 0, s, n, pair_i, pair_j, A, sp);
     LM_LAUNCH_CHECK();
-    LM_LAUNCH(lm::K_RPS_DEPS, lm::rps_deps_kernel, dim3(1024, S), dim3(64), 0, s, n, A, sp);
+    if ((int32_t)S >= lm::kSplitSteps) {
+        LM_LAUNCH(lm::K_RPS_DEPS, (lm::rps_deps_kernel<lm::kDepsSmallTable, lm::kSmallComponents>), dim3(1024, S),
+                  dim3(64), 0, s, n, A, sp);
+        LM_LAUNCH_CHECK();
+        LM_LAUNCH(lm::K_RPS_DEPS, (lm::rps_deps_kernel<lm::kDepsTable, lm::kLargeComponents>), dim3(256, S),
+                  dim3(64), 0, s, n, A, sp);
+    } else {
+        LM_LAUNCH(lm::K_RPS_DEPS, (lm::rps_deps_kernel<lm::kDepsTable, lm::kAllComponents>), dim3(1024, S), dim3(64),
+                  0, s, n, A, sp);
+    }
     LM_LAUNCH_CHECK();
-    LM_LAUNCH(lm::K_RPS_LEVELS, lm::rps_levels_kernel, dim3(1024, S), dim3(64), 0, s, n, A, sp);
+    if ((int32_t)S >= lm::kSplitSteps) {
+        LM_LAUNCH(lm::K_RPS_LEVELS, (lm::rps_levels_kernel<lm::kLevelSmallPairs, lm::kSmallComponents>),
+                  dim3(1024, S), dim3(64), 0, s, n, A, sp);
+        LM_LAUNCH_CHECK();
+        LM_LAUNCH(lm::K_RPS_LEVELS, (lm::rps_levels_kernel<lm::kLevelPairs, lm::kLargeComponents>), dim3(256, S),
+                  dim3(64), 0, s, n, A, sp);
+    } else {
+        LM_LAUNCH(lm::K_RPS_LEVELS, (lm::rps_levels_kernel<lm::kLevelPairs, lm::kAllComponents>), dim3(1024, S),
+                  dim3(64), 0, s, n, A, sp);
+    }
     LM_LAUNCH_CHECK();
     return LM_OK;
 }
