@@ -440,11 +440,12 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
     // end), all of a thread's at once: the trees are shallow after the tile pass
     // pass and the union-find, and path halving here would add scattered stores
     for (;;) {
-        // q[k] is always a particle index (a parent, or 0 past the end), so
-        // the walks' loads go out unconditionally, all at once
+        // q[k] is always a particle index (a parent, or 0 past the end); only
+        // the walks still going load, all at once (the others' lanes are
+        // masked off and cost the address unit nothing)
         uint32_t nq[kKeyItems];
 #pragma unroll
-        for (int k = 0; k < kKeyItems; ++k) nq[k] = A.parent[ro + q[k]];
+        for (int k = 0; k < kKeyItems; ++k) nq[k] = act[k] ? A.parent[ro + q[k]] : q[k];
 #pragma unroll
         for (int k = 0; k < kKeyItems; ++k) lm_pin(nq[k]);
         bool more = false;
@@ -465,8 +466,8 @@ rps_keys_kernel(int64_t n, RpsArrays A, RpsStrides st, bool write_vals) {
         if (v >= n) continue;
         A.pkey[ro + v] = x[k];
         // the touched flag joins the row record, so the grouping gathers both in one 8-byte load
-        // (row ends stay below 2^29)
-        if (t[k]) reinterpret_cast<uint32_t*>(A.rows + ro + v)[1] |= 0x80000000u;
+        // (row ends stay below 2^29; ry[k] is the row end this thread loaded above)
+        if (t[k]) reinterpret_cast<uint32_t*>(A.rows + ro + v)[1] = ry[k] | 0x80000000u;
         if (write_vals) A.pval[ro + v] = (uint32_t)v;  // else the radix pass generates the indices
     }
 }
