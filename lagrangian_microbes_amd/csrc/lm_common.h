@@ -86,7 +86,6 @@ enum KernelId {
     K_SPECIES_COUNT,
     K_DIST_HIST,
     K_RPS_TILE,
-    K_RPS_TILE_OFFSETS,
     K_NUM_KERNELS
 };
 bool timing_on();

@@ -27,7 +27,7 @@ static const char* const kKernelNames[K_NUM_KERNELS] = {
     "scan_reduce",    "scan_apply",      "rps_init",  "rps_union",   "rps_keys",   "rps_group", "rps_comps",
     "rps_play",       "rps_fill",  "rps_deps",    "rps_levels",    "philox",
     "copy16", "sort_hist", "sort_scatter", "adj_forward", "adj_recompute", "adj_backward", "adj_blend", "adj_slab", "advect_rk4_f32", "field_blend_f32", "species_count", "dist_hist",
-    "rps_tile", "rps_tile_offsets"};
+    "rps_tile"};
 
 struct TimedLaunch {
     int kid;

@@ -192,17 +192,6 @@ __device__ __forceinline__ void wave_append(bool flag, uint4 v, uint4* __restric
 // ---------------------------------------------------------------------------
 // prepare
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock)
-rps_init_kernel(int64_t n, RpsArrays A, RpsStrides st) {
-    const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    const int64_t s = blockIdx.y, ro = s * st.row;
-    if (v < n) {
-        A.touched[ro + v] = 0;
-        A.rows[ro + v] = make_uint2(0u, 0u);
-    }
-    if (v < 4) A.lists[4 * s + v] = 0;
-}
-
 // Union-find by tile before the global one: the particles are cut into
 // tiles of kTile consecutive indices, and a workgroup joins the pairs whose
 // two ends lie in its tile in an LDS forest (a pair search's neighbours are
@@ -249,8 +238,7 @@ __device__ __forceinline__ void lds_unite(uint32_t* par, uint32_t a, uint32_t b)
     }
 }
 
-// off[s][t] = first k of step s's list with pair_i[k] >= t * kTile (t <=
-// ntiles; the list is sorted by i).
+// The first k of a step's list with pair_i[k] >= v (the list is sorted by i).
 __device__ __forceinline__ uint32_t first_pair_at_or_above(const int32_t* pi, int64_t P, int64_t v) {
     int64_t lo = 0, hi = P;
     while (lo < hi) {
@@ -261,16 +249,23 @@ __device__ __forceinline__ uint32_t first_pair_at_or_above(const int32_t* pi, in
     return (uint32_t)lo;
 }
 
+// Init, and the tile pass's ranges: off[s][t] (t <= ntiles) for the first
+// threads of a step (off == NULL: no pairs, no tiles).
 __global__ void __launch_bounds__(kBlock)
-rps_tile_offsets_kernel(int nsteps, int ntiles1, const int32_t* __restrict__ pi_all,
-                        const int64_t* __restrict__ npairs, int64_t stride, uint32_t* __restrict__ off) {
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= (int64_t)nsteps * ntiles1) return;
-    const int64_t s = t / ntiles1;
-    const int64_t tile = t - s * ntiles1;
-    int64_t P = step_pairs(npairs, s, stride);
-    if (P < 0) P = 0;
-    off[t] = first_pair_at_or_above(pi_all + s * stride, P, tile * (int64_t)kTile);
+rps_init_kernel(int64_t n, RpsArrays A, RpsStrides st, const int32_t* __restrict__ pi_all,
+                const int64_t* __restrict__ npairs, uint32_t* __restrict__ off, int ntiles1) {
+    const int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t s = blockIdx.y, ro = s * st.row;
+    if (off != nullptr && v < ntiles1) {
+        int64_t P = step_pairs(npairs, s, st.pair);
+        if (P < 0) P = 0;
+        off[s * ntiles1 + v] = first_pair_at_or_above(pi_all + s * st.pair, P, v * (int64_t)kTile);
+    }
+    if (v < n) {
+        A.touched[ro + v] = 0;
+        A.rows[ro + v] = make_uint2(0u, 0u);
+    }
+    if (v < 4) A.lists[4 * s + v] = 0;
 }
 
 __global__ void __launch_bounds__(kTileBlock)
@@ -1498,18 +1493,17 @@ static int rps_prepare_impl(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int3
     lm::RpsArrays& A = ws->A;
     const unsigned S = (unsigned)nsteps;
     const unsigned nbp = lm::grid_for(n);
-    LM_LAUNCH(lm::K_RPS_INIT, lm::rps_init_kernel, dim3(nbp, S), dim3(lm::kBlock), 0, s, n, A, sp);
+    const int ntiles = (int)((n + lm::kTile - 1) / lm::kTile);
+    LM_REQUIRE(ntiles + 1 <= ws->tiles1, LM_ERR_ARG, "internal: tile offsets");
+    LM_REQUIRE((int64_t)nbp * lm::kBlock >= ntiles + 1, LM_ERR_ARG, "internal: tile offsets grid");
+    LM_LAUNCH(lm::K_RPS_INIT, lm::rps_init_kernel, dim3(nbp, S), dim3(lm::kBlock), 0, s, n, A, sp, pair_i, npairs,
+              pair_stride > 0 ? ws->tile_off : nullptr, ntiles + 1);
     LM_LAUNCH_CHECK();
     if (pair_stride > 0) {
         // grid-stride over a step's pairs: at most one thread per particle (a
         // step has fewer pairs than particles but in dense runs; the capacity,
         // often twice the particles, would launch idle blocks)
         const unsigned npb = lm::grid_for(pair_stride < n ? pair_stride : n);
-        const int ntiles = (int)((n + lm::kTile - 1) / lm::kTile);
-        LM_REQUIRE(ntiles + 1 <= ws->tiles1, LM_ERR_ARG, "internal: tile offsets");
-        LM_LAUNCH(lm::K_RPS_TILE_OFFSETS, lm::rps_tile_offsets_kernel, dim3(lm::grid_for((int64_t)S * (ntiles + 1))),
-                  dim3(lm::kBlock), 0, s, (int)S, ntiles + 1, pair_i, npairs, pair_stride, ws->tile_off);
-        LM_LAUNCH_CHECK();
         LM_LAUNCH(lm::K_RPS_TILE, lm::rps_tile_union_kernel, dim3(ntiles, S), dim3(lm::kTileBlock), 0, s, n, pair_i,
                   pair_j, npairs, A, sp, (const uint32_t*)ws->tile_off, ntiles + 1);
         LM_LAUNCH_CHECK();

@@ -50,7 +50,6 @@ TIMER_NAMES = {
     "bin_keys_kernel": "bin_keys", "bin_table_kernel": "bin_table", "pair_count_kernel": "pair_count",
     "pair_emit_kernel": "pair_emit", "radix_hist_kernel": "sort_hist", "radix_scatter_kernel": "sort_scatter",
     "scan_reduce_kernel": "scan_reduce", "scan_apply_kernel": "scan_apply", "rps_init_kernel": "rps_init",
-    "rps_tile_offsets_kernel": "rps_tile_offsets",
     "rps_tile_union_kernel": "rps_tile",
     "rps_union_kernel": "rps_union", "rps_keys_kernel": "rps_keys", "rps_group_blk_kernel": "rps_group",
     "rps_comps_kernel": "rps_comps", "rps_fill_kernel": "rps_fill", "rps_deps_kernel": "rps_deps",
