@@ -326,12 +326,14 @@ pair_count_kernel(int64_t n, CellGrid g, Metric m, Strides st, const uint32_t* _
         float4 o[kCand];
 #pragma unroll
         for (int u = 0; u < kCand; ++u) {
-            // past the end: the last candidate again (a valid slot), discarded below
-            const uint32_t t = min(t0 + u, total - 1u);
+            // past the end the lane is masked off (no address-unit work) and
+            // the candidate is discarded below
+            const uint32_t t = t0 + u;
             uint32_t d = dlt[0];
 #pragma unroll
             for (int r = 1; r < 6; ++r) d = t >= pre[r] ? dlt[r] : d;
-            o[u] = recs[t + d];
+            o[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (t < total) o[u] = recs[t + d];
         }
         // every load of the batch is issued before any use of the first: the
         // scheduler would otherwise start testing candidate 0 early and wait
