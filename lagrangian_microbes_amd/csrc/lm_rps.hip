@@ -197,9 +197,8 @@ __device__ __forceinline__ void wave_append(bool flag, uint4 v, uint4* __restric
 // two ends lie in its tile in an LDS forest (a pair search's neighbours are
 // spatially close and so, on a lattice start, mostly close in index).  It
 // then points every particle of the tile at its tile-local root, the
-// smallest index of its part of the component, which replaces the parent
-// set-up of the init pass.  The global pass joins only the pairs that cross
-// tiles.  Every link still goes from a larger root to a smaller one, so the
+// smallest index of its part of the component (the init pass leaves the
+// parents alone).  The union pass joins only the pairs that cross tiles.  Every link still goes from a larger root to a smaller one, so the
 // final roots are the smallest indices of the components, and every pair's j
 // ends below itself (its component holds i < j), which is how the keys pass
 // tells that j has pairs.
