@@ -65,7 +65,7 @@ constexpr uint32_t kNarrowSlots = kNarrowPairs + 1;  // a component of P pairs h
 constexpr uint32_t kLevelPairs = 4096;               // components of up to this many pairs replay by levels
 constexpr uint32_t kLevelSlots = kLevelPairs + 1;    // ... and so have at most this many particles
 constexpr uint32_t kSlotMask = (1u << 29) - 1u;
-constexpr int kCompItems = 8;                        // sorted positions per thread when listing components
+constexpr int kCompItems = 4;                        // sorted positions per thread when listing components
 
 // Per-step strides (elements) of a batch.
 struct RpsStrides {
