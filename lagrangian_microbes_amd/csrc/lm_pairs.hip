@@ -149,7 +149,7 @@ bin_keys_kernel(int64_t n, const float* __restrict__ x, const float* __restrict_
 // order inside a bucket is free: the pair list is sorted later).  FINE_MAX
 // buckets per block: 4096 up to 2^22 buckets, 8192 up to 2^23.
 constexpr uint32_t kFineMax = 4096;
-constexpr int kPlace = 4;  // particles per thread in flight when placing records
+constexpr int kPlace = 8;  // particles per thread in flight when placing records
 constexpr uint32_t kFineMaxWide = 8192;
 
 template <uint32_t FINE_MAX>

@@ -1554,7 +1554,7 @@ static int rps_prepare_impl(lm_rps_ws* ws, int32_t nsteps, int64_t n, const int3
             in = 1;
         }
         if (fb == (int)lm::kGroupFineBits) {
-            LM_LAUNCH(lm::K_RPS_GROUP, (lm::rps_group_blk_kernel<lm::kGroupFineBits, 2>), dim3(1u << super_bits, S),
+            LM_LAUNCH(lm::K_RPS_GROUP, (lm::rps_group_blk_kernel<lm::kGroupFineBits, 4>), dim3(1u << super_bits, S),
                       dim3(64 * lm::kGroupWaves), 0, s, n, A, sp, (const uint32_t*)pk[in], (const uint32_t*)pv[in],
                       pk[1 - in], pv[1 - in], dstart, ws->sort.row, dstride, 1u << fine_bits);
         } else if (fb == (int)lm::kGroupFineBitsWide) {
